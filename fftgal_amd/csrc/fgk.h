@@ -1,0 +1,144 @@
+/*
+ * fgk.h -- the thin C-ABI between the C host layer (the host C files) and the HIP
+ * kernels for gfx950 (the .hip files).  Plain C types only: device pointers are
+ * void* / typed pointers, streams are opaque.  Every function returns 0 on
+ * success or a negative code; fgk_last_error() gives the message.
+ *
+ * Internal to libfftgal_amd.so: the public API is include/fft.h, gal.h,
+ * power.h, fftgal_ext.h and fftgal_batch.h.
+ */
+#ifndef FGK_H
+#define FGK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *fgk_stream_t;     /* hipStream_t */
+typedef void *fgk_event_t;      /* hipEvent_t */
+
+/* ---- runtime (fgk_runtime.hip) ---- */
+int  fgk_init(void);                       /* idempotent; selects device 0 */
+const char *fgk_last_error(void);
+void fgk_set_error(const char *fmt, ...);
+int  fgk_malloc(void **ptr, size_t bytes);
+int  fgk_free(void *ptr);
+int  fgk_host_alloc(void **ptr, size_t bytes);   /* pinned host memory */
+int  fgk_host_free(void *ptr);
+int  fgk_h2d(void *dst, const void *src, size_t bytes, fgk_stream_t s);
+int  fgk_d2h(void *dst, const void *src, size_t bytes, fgk_stream_t s);
+int  fgk_d2d(void *dst, const void *src, size_t bytes, fgk_stream_t s);
+int  fgk_fill_zero(void *dst, size_t bytes, fgk_stream_t s);  /* kernel, graph safe */
+int  fgk_stream_create(fgk_stream_t *s);
+int  fgk_stream_destroy(fgk_stream_t s);
+int  fgk_stream_sync(fgk_stream_t s);
+int  fgk_event_create(fgk_event_t *e);
+int  fgk_event_destroy(fgk_event_t e);
+int  fgk_event_record(fgk_event_t e, fgk_stream_t s);
+int  fgk_event_elapsed_ms(float *ms, fgk_event_t a, fgk_event_t b);
+int  fgk_device_info(char *name, size_t len, int *cu_count, size_t *mem_bytes);
+/* hipGraph capture of the work enqueued on a stream between begin/end */
+typedef void *fgk_graph_t;
+int  fgk_graph_begin(fgk_stream_t s);
+int  fgk_graph_end(fgk_stream_t s, fgk_graph_t *g);
+int  fgk_graph_launch(fgk_graph_t g, fgk_stream_t s);
+int  fgk_graph_destroy(fgk_graph_t g);
+
+/* ---- FFT engine (fgk_fft.hip) ---- */
+/* plan for cubic grids of side Ng (even), precision 4 (float) or 8 (double);
+ * twiddle tables are generated in FP64 on the host and stored in the plan's
+ * precision on the device */
+typedef struct fgk_fftplan fgk_fftplan;
+int  fgk_fft_plan_create(fgk_fftplan **plan, int Ng, int precision);
+void fgk_fft_plan_destroy(fgk_fftplan *plan);
+int  fgk_fft_supported(int Ng);     /* 1 if Ng = 2^a 3^b 5^c 7^d 11^e 13^f, even */
+/* FP64, in place on the padded layout f[(i*Ng+j)*Ng_pad+k], unnormalised,
+ * forward sign -1; the result is multiplied by scale */
+int  fgk_fft_r2c_f64(fgk_fftplan *p, double *f, double scale, fgk_stream_t s);
+int  fgk_fft_c2r_f64(fgk_fftplan *p, double *f, double scale, fgk_stream_t s);
+/* FP32 forward r2c whose real input is an int64 fixed-point grid
+ * src[(i*Ng+j)*Ng+k] times in_scale; src rows are cleared to zero after they
+ * are read when clear_src != 0; output in the padded float layout */
+int  fgk_fft_r2c_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *src,
+                                double in_scale, int clear_src, fgk_stream_t s);
+/* the three passes separately (for the profiler and the tuner) */
+int  fgk_fft_rows_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *src,
+                                 double in_scale, int clear_src, fgk_stream_t s);
+int  fgk_fft_cols_f32(fgk_fftplan *p, float *f, int axis, int sign, fgk_stream_t s);
+/* tuning knobs: columns per workgroup for the strided passes, rows per
+ * workgroup for the row pass (0 = default) */
+int  fgk_fft_set_tuning(fgk_fftplan *p, int cols_per_wg, int rows_per_wg);
+
+/* ---- painting (fgk_paint.hip) ---- */
+/* FP64 PCS paint with FP64 global atomics (reference order of operations):
+ * f[...] += w (weight_sign = +1) or f[...] -= alpha*w (weight_sign = -1) */
+int  fgk_paint_f64(double *f, int Ng, const double *x, const double *y,
+                   const double *z, long Np, double Hinv, const double off[3],
+                   double alpha, int weight_sign, fgk_stream_t s);
+int  fgk_scale_f64(double *f, long n, double scale, fgk_stream_t s);
+int  fgk_avg_f64(double *f, const double *g, long n, fgk_stream_t s);  /* f = 0.5*(g+f) */
+int  fgk_widen_f32_f64(double *dst, const float *src, long n, fgk_stream_t s);
+
+/* deterministic fixed-point tile paint of float32 positions into one
+ * (plain, offset 0) or two (plus dual, offset 1/2) int64 grids */
+typedef struct fgk_paintplan fgk_paintplan;
+int  fgk_paint_plan_create(fgk_paintplan **pp, int Ng, int tile, long Np_max,
+                           int interlace);
+void fgk_paint_plan_destroy(fgk_paintplan *pp);
+int  fgk_paint_fixed(fgk_paintplan *pp, const float *x, const float *y,
+                     const float *z, long Np, double Hinv,
+                     int64_t *gridA, int64_t *gridB, fgk_stream_t s);
+double fgk_paint_fixed_scale(const fgk_paintplan *pp);  /* 2^-S */
+int  fgk_paint_plan_tile(const fgk_paintplan *pp);
+/* the paint stages separately (profiling / tests) */
+int  fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x,
+                     const float *y, const float *z, long Np, double Hinv,
+                     int64_t *gridA, int64_t *gridB, fgk_stream_t s);
+
+/* ---- k space (fgk_kspace.hip) ---- */
+/* phase[d][j] complex table (3*Ng*2 doubles) built on the host */
+int  fgk_offset_phase_f64(double *f, int Ng, const double *phase_dev, fgk_stream_t s);
+int  fgk_deconv_f64(double *f, int Ng, const double *winv_dev, fgk_stream_t s);
+int  fgk_P_f64(double *f, int Ng, double Vinv, double Pshot, fgk_stream_t s);
+
+/* binning.  Bins are accumulated into bins_dev (doubles) laid out as
+ *   [NK][4]         Pl sums (l = 0,2,4,6)      at offset 0
+ *   [NK][W]         Pmu sums                    at offset NK*4
+ * and, when geometry != 0, also
+ *   [NK]            K sums (count*Kamp)         at offset NK*(4+W)
+ *   [NK][W]         K sums per wedge            at offset NK*(5+W)
+ * where W = Nmu+1 (wedge Nmu collects mu = 1) or 0 without wedges,
+ * with integer counts in counts_dev (int64): [NK] then [NK][W].
+ * bins_dev/counts_dev must be zeroed by the caller. */
+typedef struct fgk_binspec {
+    int Ng;
+    int NK;
+    int Nmu;            /* 0: no wedges */
+    double dKinv;       /* KF/dK */
+    double los[3];      /* already normalised */
+    int do_pl;          /* accumulate the Legendre sums */
+    int geometry;       /* also accumulate K sums and counts */
+} fgk_binspec;
+size_t fgk_bins_doubles(const fgk_binspec *b);
+size_t fgk_bins_counts(const fgk_binspec *b);
+/* geometry only: K sums and counts (b->geometry must be 1); P sums untouched */
+int  fgk_bin_geometry(const fgk_binspec *b, double *bins_dev, int64_t *counts_dev,
+                      fgk_stream_t s);
+/* bin the FP64 grid holding P(k) in the real slots (after P()) */
+int  fgk_bin_pgrid_f64(const double *f, const fgk_binspec *b, double *bins_dev,
+                       int64_t *counts_dev, fgk_stream_t s);
+/* fused: phase (dual grid), interlace average, deconvolution, |delta|^2*norm
+ * - Pshot, binning.  A = plain grid, B = dual grid (NULL: no interlacing).
+ * tables: phase (3*Ng complex double), winv (Ng double) on the device */
+int  fgk_fused_bin_f32(const float *A, const float *B, const fgk_binspec *b,
+                       const double *phase_dev, const double *winv_dev,
+                       double norm, double Vinv, double Pshot,
+                       double *bins_dev, int64_t *counts_dev, fgk_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,496 @@
+// Stockham mixed-radix FFT engine for cubic grids on gfx950.
+//
+// Replaces the reference's FFTW calls (src/fft.c:28-29 plan r2c/c2r 3d,
+// src/fft.c:174,186 fftw_execute) with three LDS-staged passes over the
+// padded in-place layout f[(i*Ng+j)*Ng_pad + k] (src/fft.h:10-13,20):
+//   rows  (k, contiguous): real <-> half spectrum through a length Ng/2
+//          complex transform plus the split post/pre-processing step;
+//   cols  (j, stride Ng_pad/2 complex) and (i, stride Ng*(Ng/2+1)):
+//          C consecutive k columns per workgroup, loaded as coalesced row
+//          segments, transformed in LDS with the columns interleaved so
+//          that butterfly reads are bank-conflict free.
+// Each workgroup keeps its pencils in LDS for all stages (ping-pong
+// buffers), so one pass reads and writes the grid exactly once.
+// Twiddles exp(-2 pi i t/Ng) are generated in FP64 on the host (libm) and
+// stored in the plan's precision.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../fgk.h"
+#include "fgk_common.h"
+
+#define FGK_MAX_STAGES 24
+
+struct Stages {
+    int n;
+    int ns;
+    int radix[FGK_MAX_STAGES];
+};
+
+struct fgk_fftplan {
+    int Ng;
+    int prec;            // 4 or 8
+    Stages sN;           // length Ng
+    Stages sH;           // length Ng/2
+    void *tw;            // Ng complex, exp(-2 pi i t / Ng)
+    int cols_per_wg;     // 0 = automatic
+    int rows_per_wg;     // 0 = automatic
+};
+
+static int factor_stages(int n, Stages *s) {
+    s->n = n;
+    s->ns = 0;
+    int m = n;
+    static const int order[] = {8, 4, 2, 3, 5, 7, 11, 13};
+    for (int q = 0; q < 8; ++q) {
+        int r = order[q];
+        while (m % r == 0) {
+            if (s->ns >= FGK_MAX_STAGES) return -1;
+            s->radix[s->ns++] = r;
+            m /= r;
+        }
+    }
+    return m == 1 ? 0 : -1;
+}
+
+extern "C" int fgk_fft_supported(int Ng) {
+    if (Ng < 2 || (Ng & 1)) return 0;
+    Stages s;
+    return factor_stages(Ng, &s) == 0 && factor_stages(Ng / 2, &s) == 0;
+}
+
+// ------------------------------------------------------------ butterflies
+
+template <typename T>
+__device__ __forceinline__ cpx<T> root(const cpx<T> *tw, int e, int sign) {
+    cpx<T> w = tw[e];
+    if (sign > 0) w.y = -w.y;
+    return w;
+}
+
+template <typename T, int R> struct Bfly {
+    // generic prime radix: direct DFT with roots from the twiddle table
+    __device__ static __forceinline__ void run(cpx<T> *v, int sign, const cpx<T> *tw,
+                                               int n, int tws) {
+        cpx<T> o[R];
+        const int rs = (n / R) * tws;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            cpx<T> acc = v[0];
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                cpx<T> w = root(tw, ((q * r) % R) * rs, sign);
+                acc = cadd(acc, cmul(v[r], w));
+            }
+            o[q] = acc;
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = o[q];
+    }
+};
+
+template <typename T> struct Bfly<T, 2> {
+    __device__ static __forceinline__ void run(cpx<T> *v, int, const cpx<T> *, int, int) {
+        cpx<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <typename T> struct Bfly<T, 4> {
+    __device__ static __forceinline__ void run(cpx<T> *v, int sign, const cpx<T> *, int, int) {
+        cpx<T> s0 = cadd(v[0], v[2]), d0 = csub(v[0], v[2]);
+        cpx<T> s1 = cadd(v[1], v[3]), d1 = csub(v[1], v[3]);
+        cpx<T> e = cmul_si(d1, sign);
+        v[0] = cadd(s0, s1);
+        v[1] = cadd(d0, e);
+        v[2] = csub(s0, s1);
+        v[3] = csub(d0, e);
+    }
+};
+
+template <typename T> struct Bfly<T, 3> {
+    __device__ static __forceinline__ void run(cpx<T> *v, int sign, const cpx<T> *, int, int) {
+        const T h = (T)0.86602540378443864676;
+        cpx<T> t1 = cadd(v[1], v[2]);
+        cpx<T> t2 = csub(v[0], cscale(t1, (T)0.5));
+        cpx<T> t3 = cscale(csub(v[1], v[2]), (T)sign * h);
+        v[0] = cadd(v[0], t1);
+        v[1] = cmk(t2.x - t3.y, t2.y + t3.x);
+        v[2] = cmk(t2.x + t3.y, t2.y - t3.x);
+    }
+};
+
+template <typename T> struct Bfly<T, 8> {
+    __device__ static __forceinline__ void run(cpx<T> *v, int sign, const cpx<T> *, int, int) {
+        cpx<T> e[4] = {v[0], v[2], v[4], v[6]};
+        cpx<T> o[4] = {v[1], v[3], v[5], v[7]};
+        Bfly<T, 4>::run(e, sign, nullptr, 0, 0);
+        Bfly<T, 4>::run(o, sign, nullptr, 0, 0);
+        const T r2 = (T)0.70710678118654752440;
+        // w^1 = r2 (1 + sign i), w^2 = sign i, w^3 = r2 (-1 + sign i)
+        cpx<T> w1o = cmk(r2 * (o[1].x - sign * o[1].y), r2 * (o[1].y + sign * o[1].x));
+        cpx<T> w2o = cmul_si(o[2], sign);
+        cpx<T> w3o = cmk(r2 * (-o[3].x - sign * o[3].y), r2 * (-o[3].y + sign * o[3].x));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], w1o);
+        v[5] = csub(e[1], w1o);
+        v[2] = cadd(e[2], w2o);
+        v[6] = csub(e[2], w2o);
+        v[3] = cadd(e[3], w3o);
+        v[7] = csub(e[3], w3o);
+    }
+};
+
+// One Stockham stage of radix R over all pencils held in LDS.  Element
+// (t, p) of pencil p lives at buf[t*st + p*sp].  pfast selects whether
+// consecutive threads take consecutive pencils (column layout) or
+// consecutive butterflies (row layout).
+template <typename T, int R>
+__device__ __forceinline__ void stage_run(const cpx<T> *__restrict__ a, cpx<T> *__restrict__ b,
+                                          int n, int Ns, int P, int lgP, int pfast, int st,
+                                          int sp, const cpx<T> *__restrict__ tw, int tws,
+                                          int sign) {
+    const int m = n / R;
+    const int tstep = (n / (Ns * R)) * tws;
+    const int total = m * P;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        int p, j;
+        if (pfast) {
+            p = idx & (P - 1);
+            j = idx >> lgP;
+        } else {
+            p = idx / m;
+            j = idx - p * m;
+        }
+        const int k = j % Ns;
+        cpx<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = a[(j + r * m) * st + p * sp];
+        if (k) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], root(tw, r * k * tstep, sign));
+        }
+        Bfly<T, R>::run(v, sign, tw, n, tws);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) b[(base + r * Ns) * st + p * sp] = v[r];
+    }
+}
+
+template <typename T>
+__device__ cpx<T> *lds_fft(cpx<T> *a, cpx<T> *b, const Stages &S, int P, int lgP, int pfast,
+                           int st, int sp, const cpx<T> *tw, int tws, int sign) {
+    int Ns = 1;
+    for (int s = 0; s < S.ns; ++s) {
+        const int R = S.radix[s];
+        switch (R) {
+        case 8: stage_run<T, 8>(a, b, S.n, Ns, P, lgP, pfast, st, sp, tw, tws, sign); break;
+        case 4: stage_run<T, 4>(a, b, S.n, Ns, P, lgP, pfast, st, sp, tw, tws, sign); break;
+        case 2: stage_run<T, 2>(a, b, S.n, Ns, P, lgP, pfast, st, sp, tw, tws, sign); break;
+        case 3: stage_run<T, 3>(a, b, S.n, Ns, P, lgP, pfast, st, sp, tw, tws, sign); break;
+        case 5: stage_run<T, 5>(a, b, S.n, Ns, P, lgP, pfast, st, sp, tw, tws, sign); break;
+        case 7: stage_run<T, 7>(a, b, S.n, Ns, P, lgP, pfast, st, sp, tw, tws, sign); break;
+        case 11: stage_run<T, 11>(a, b, S.n, Ns, P, lgP, pfast, st, sp, tw, tws, sign); break;
+        default: stage_run<T, 13>(a, b, S.n, Ns, P, lgP, pfast, st, sp, tw, tws, sign); break;
+        }
+        __syncthreads();
+        cpx<T> *t = a;
+        a = b;
+        b = t;
+        Ns *= R;
+    }
+    return a;
+}
+
+// blockIdx -> logical block such that each XCD (blockIdx % 8) owns a
+// contiguous range of logical blocks; neighbouring blocks share cache lines
+// at their edges, and this keeps those lines in one XCD's L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+    unsigned q = nb >> 3;
+    if (b < (q << 3)) return (b & 7u) * q + (b >> 3);
+    return b;
+}
+
+// ------------------------------------------------------------ row kernels
+
+// MODE 0: real input in place in f; MODE 1: int64 fixed-point input src
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256)
+k_rows_r2c(T *f, int64_t *src, double in_scale, int clear_src, T out_scale, int N, int Ng_pad,
+           long nrows, Stages S, const cpx<T> *tw, int P) {
+    extern __shared__ char smem_raw[];
+    cpx<T> *a = (cpx<T> *)smem_raw;
+    const int M = N / 2;
+    cpx<T> *b = a + (size_t)P * M;
+    const long row0 = (long)xcd_block(blockIdx.x, gridDim.x) * P;
+    for (int idx = threadIdx.x; idx < P * M; idx += blockDim.x) {
+        const int p = idx / M;
+        const int m = idx - p * M;
+        const long row = row0 + p;
+        cpx<T> z = cmk((T)0, (T)0);
+        if (row < nrows) {
+            if (MODE == 0) {
+                const cpx<T> *r = (const cpx<T> *)(f + row * (long)Ng_pad);
+                z = r[m];
+            } else {
+                int64_t *r = src + row * (long)N + 2 * m;
+                longlong2 v = *(const longlong2 *)r;
+                z = cmk((T)((double)v.x * in_scale), (T)((double)v.y * in_scale));
+                if (clear_src) *(longlong2 *)r = make_longlong2(0, 0);
+            }
+        }
+        a[idx] = z;
+    }
+    __syncthreads();
+    cpx<T> *res = lds_fft<T>(a, b, S, P, 0, 0, 1, M, tw, 2, -1);
+    const int K1 = M + 1;
+    for (int idx = threadIdx.x; idx < P * K1; idx += blockDim.x) {
+        const int p = idx / K1;
+        const int k = idx - p * K1;
+        const long row = row0 + p;
+        if (row >= nrows) continue;
+        const int k1 = (k == M) ? 0 : k;
+        const int k2 = (k == 0) ? 0 : M - k;
+        cpx<T> Z = res[p * M + k1];
+        cpx<T> Zc = res[p * M + k2];
+        Zc.y = -Zc.y;
+        cpx<T> E = cscale(cadd(Z, Zc), (T)0.5);
+        cpx<T> D = csub(Z, Zc);
+        cpx<T> O = cmk((T)0.5 * D.y, (T)-0.5 * D.x);
+        cpx<T> X = cadd(E, cmul(O, tw[k]));
+        X = cscale(X, out_scale);
+        ((cpx<T> *)(f + row * (long)Ng_pad))[k] = X;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_rows_c2r(T *f, T out_scale, int N, int Ng_pad, long nrows, Stages S, const cpx<T> *tw, int P) {
+    extern __shared__ char smem_raw[];
+    const int M = N / 2;
+    const int K1 = M + 1;
+    cpx<T> *xs = (cpx<T> *)smem_raw;          // P*(M+1) staged half spectra
+    cpx<T> *a = xs + (size_t)P * K1;
+    cpx<T> *b = a + (size_t)P * M;
+    const long row0 = (long)xcd_block(blockIdx.x, gridDim.x) * P;
+    for (int idx = threadIdx.x; idx < P * K1; idx += blockDim.x) {
+        const int p = idx / K1;
+        const int k = idx - p * K1;
+        const long row = row0 + p;
+        xs[idx] = row < nrows ? ((const cpx<T> *)(f + row * (long)Ng_pad))[k] : cmk((T)0, (T)0);
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < P * M; idx += blockDim.x) {
+        const int p = idx / M;
+        const int k = idx - p * M;
+        cpx<T> X = xs[p * K1 + k];
+        cpx<T> Xc = xs[p * K1 + (M - k)];
+        Xc.y = -Xc.y;
+        cpx<T> E = cadd(X, Xc);
+        cpx<T> O = cmulc(csub(X, Xc), tw[k]);   // times exp(+2 pi i k/N)
+        a[idx] = cmk(E.x - O.y, E.y + O.x);      // E + i O
+    }
+    __syncthreads();
+    cpx<T> *res = lds_fft<T>(a, b, S, P, 0, 0, 1, M, tw, 2, +1);
+    for (int idx = threadIdx.x; idx < P * M; idx += blockDim.x) {
+        const int p = idx / M;
+        const int m = idx - p * M;
+        const long row = row0 + p;
+        if (row >= nrows) continue;
+        ((cpx<T> *)(f + row * (long)Ng_pad))[m] = cscale(res[idx], out_scale);
+    }
+}
+
+// --------------------------------------------------------- column kernel
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_cols(cpx<T> *g, int N, int Nk, long stride_t, long stride_o, int C, int lgC, Stages S,
+       const cpx<T> *tw, int sign, T scale) {
+    extern __shared__ char smem_raw[];
+    cpx<T> *a = (cpx<T> *)smem_raw;
+    cpx<T> *b = a + (size_t)N * C;
+    const int nkb = (Nk + C - 1) / C;
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int o = lb / nkb;
+    const int k0 = (lb - o * nkb) * C;
+    cpx<T> *base = g + (long)o * stride_o + k0;
+    const int kc = Nk - k0 < C ? Nk - k0 : C;
+    for (int idx = threadIdx.x; idx < N * C; idx += blockDim.x) {
+        const int c = idx & (C - 1);
+        const int t = idx >> lgC;
+        a[idx] = c < kc ? base[(long)t * stride_t + c] : cmk((T)0, (T)0);
+    }
+    __syncthreads();
+    cpx<T> *res = lds_fft<T>(a, b, S, C, lgC, 1, C, 1, tw, 1, sign);
+    for (int idx = threadIdx.x; idx < N * C; idx += blockDim.x) {
+        const int c = idx & (C - 1);
+        const int t = idx >> lgC;
+        if (c < kc) base[(long)t * stride_t + c] = cscale(res[idx], scale);
+    }
+}
+
+// ------------------------------------------------------------------ host
+
+extern "C" int fgk_fft_plan_create(fgk_fftplan **out, int Ng, int precision) {
+    *out = NULL;
+    if (!fgk_fft_supported(Ng)) {
+        fgk_set_error("grid size Ng=%d unsupported: need even Ng = 2^a 3^b 5^c 7^d 11^e 13^f", Ng);
+        return -2;
+    }
+    if (precision != 4 && precision != 8) {
+        fgk_set_error("precision must be 4 or 8");
+        return -2;
+    }
+    fgk_fftplan *p = (fgk_fftplan *)calloc(1, sizeof(fgk_fftplan));
+    if (!p) { fgk_set_error("out of host memory"); return -1; }
+    p->Ng = Ng;
+    p->prec = precision;
+    factor_stages(Ng, &p->sN);
+    factor_stages(Ng / 2, &p->sH);
+    size_t bytes = (size_t)Ng * 2 * precision;
+    void *host = malloc(bytes);
+    if (!host) { free(p); fgk_set_error("out of host memory"); return -1; }
+    for (int t = 0; t < Ng; ++t) {
+        double ang = 2.0 * M_PI * (double)t / (double)Ng;
+        double c = cos(ang), s = -sin(ang);
+        if (precision == 8) {
+            ((double *)host)[2 * t] = c;
+            ((double *)host)[2 * t + 1] = s;
+        } else {
+            ((float *)host)[2 * t] = (float)c;
+            ((float *)host)[2 * t + 1] = (float)s;
+        }
+    }
+    if (fgk_malloc(&p->tw, bytes)) { free(host); free(p); return -1; }
+    hipError_t e = hipMemcpy(p->tw, host, bytes, hipMemcpyHostToDevice);
+    free(host);
+    if (e != hipSuccess) {
+        fgk_free(p->tw);
+        free(p);
+        return fgk_hip_fail(e, "hipMemcpy twiddles", __FILE__, __LINE__);
+    }
+    *out = p;
+    return 0;
+}
+
+extern "C" void fgk_fft_plan_destroy(fgk_fftplan *p) {
+    if (!p) return;
+    fgk_free(p->tw);
+    free(p);
+}
+
+extern "C" int fgk_fft_set_tuning(fgk_fftplan *p, int cols_per_wg, int rows_per_wg) {
+    if (cols_per_wg && (cols_per_wg & (cols_per_wg - 1))) {
+        fgk_set_error("cols_per_wg must be a power of two");
+        return -2;
+    }
+    p->cols_per_wg = cols_per_wg;
+    p->rows_per_wg = rows_per_wg;
+    return 0;
+}
+
+static const size_t kLdsBudget = 64 * 1024;
+
+// LDS per row: two ping-pong pencils of Ng/2 points, plus the staged half
+// spectrum (Ng/2+1 points) for the c2r pre-processing
+static size_t rows_lds(int P, int M, size_t elem, int extra_row) {
+    return (size_t)P * (2 * M + extra_row * (M + 1)) * elem;
+}
+
+static int rows_per_wg(const fgk_fftplan *p, size_t elem, int extra_row) {
+    const int M = p->Ng / 2;
+    if (p->rows_per_wg > 0) return p->rows_per_wg;
+    int P = 1;
+    while (P * M < 1024 && rows_lds(2 * P, M, elem, extra_row) <= kLdsBudget) P *= 2;
+    return P;
+}
+
+static int cols_per_wg(const fgk_fftplan *p, size_t elem) {
+    if (p->cols_per_wg > 0) return p->cols_per_wg;
+    int C = 16;
+    while (C > 1 && (size_t)2 * p->Ng * C * elem > kLdsBudget) C >>= 1;
+    return C;
+}
+
+static int ilog2(int c) {
+    int l = 0;
+    while ((1 << l) < c) ++l;
+    return l;
+}
+
+template <typename T>
+static int launch_cols(fgk_fftplan *p, T *f, int axis, int sign, T scale, hipStream_t s) {
+    const int N = p->Ng, Nk = N / 2 + 1;
+    const int C = cols_per_wg(p, sizeof(cpx<T>));
+    const long rowc = Nk;                  // complex elements between j rows
+    const long planec = (long)N * Nk;      // complex elements between i planes
+    long stride_t = axis == 1 ? rowc : planec;
+    long stride_o = axis == 1 ? planec : rowc;
+    const long nb = (long)N * ((Nk + C - 1) / C);
+    size_t lds = (size_t)2 * N * C * sizeof(cpx<T>);
+    hipLaunchKernelGGL(k_cols<T>, dim3((unsigned)nb), dim3(256), lds, s, (cpx<T> *)f, N, Nk,
+                       stride_t, stride_o, C, ilog2(C), p->sN, (const cpx<T> *)p->tw, sign,
+                       scale);
+    FGK_LAUNCHED();
+    return 0;
+}
+
+extern "C" int fgk_fft_cols_f32(fgk_fftplan *p, float *f, int axis, int sign, fgk_stream_t s) {
+    return launch_cols<float>(p, f, axis, sign, 1.0f, (hipStream_t)s);
+}
+
+extern "C" int fgk_fft_r2c_f64(fgk_fftplan *p, double *f, double scale, fgk_stream_t st) {
+    if (p->prec != 8) { fgk_set_error("plan precision is not FP64"); return -2; }
+    hipStream_t s = (hipStream_t)st;
+    const int N = p->Ng, M = N / 2;
+    const long nrows = (long)N * N;
+    const int P = rows_per_wg(p, sizeof(cpx<double>), 0);
+    size_t lds = (size_t)2 * P * M * sizeof(cpx<double>);
+    hipLaunchKernelGGL((k_rows_r2c<double, 0>), dim3(fgk_ceil_div(nrows, P)), dim3(256), lds, s,
+                       f, (int64_t *)nullptr, 1.0, 0, 1.0, N, 2 * (M + 1), nrows, p->sH,
+                       (const cpx<double> *)p->tw, P);
+    FGK_LAUNCHED();
+    if (launch_cols<double>(p, f, 1, -1, 1.0, s)) return -1;
+    return launch_cols<double>(p, f, 0, -1, scale, s);
+}
+
+extern "C" int fgk_fft_c2r_f64(fgk_fftplan *p, double *f, double scale, fgk_stream_t st) {
+    if (p->prec != 8) { fgk_set_error("plan precision is not FP64"); return -2; }
+    hipStream_t s = (hipStream_t)st;
+    const int N = p->Ng, M = N / 2;
+    const long nrows = (long)N * N;
+    if (launch_cols<double>(p, f, 0, +1, 1.0, s)) return -1;
+    if (launch_cols<double>(p, f, 1, +1, 1.0, s)) return -1;
+    const int P = rows_per_wg(p, sizeof(cpx<double>), 1);
+    size_t lds = (size_t)P * (2 * M + M + 1) * sizeof(cpx<double>);
+    hipLaunchKernelGGL(k_rows_c2r<double>, dim3(fgk_ceil_div(nrows, P)), dim3(256), lds, s, f,
+                       scale, N, 2 * (M + 1), nrows, p->sH, (const cpx<double> *)p->tw, P);
+    FGK_LAUNCHED();
+    return 0;
+}
+
+extern "C" int fgk_fft_rows_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *src,
+                                           double in_scale, int clear_src, fgk_stream_t st) {
+    if (p->prec != 4) { fgk_set_error("plan precision is not FP32"); return -2; }
+    hipStream_t s = (hipStream_t)st;
+    const int N = p->Ng, M = N / 2;
+    const long nrows = (long)N * N;
+    const int P = rows_per_wg(p, sizeof(cpx<float>), 0);
+    size_t lds = (size_t)2 * P * M * sizeof(cpx<float>);
+    hipLaunchKernelGGL((k_rows_r2c<float, 1>), dim3(fgk_ceil_div(nrows, P)), dim3(256), lds, s,
+                       f, src, in_scale, clear_src, 1.0f, N, 2 * (M + 1), nrows, p->sH,
+                       (const cpx<float> *)p->tw, P);
+    FGK_LAUNCHED();
+    return 0;
+}
+
+extern "C" int fgk_fft_r2c_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *src,
+                                          double in_scale, int clear_src, fgk_stream_t s) {
+    if (fgk_fft_rows_f32_from_fixed(p, f, src, in_scale, clear_src, s)) return -1;
+    if (launch_cols<float>(p, f, 1, -1, 1.0f, (hipStream_t)s)) return -1;
+    return launch_cols<float>(p, f, 0, -1, 1.0f, (hipStream_t)s);
+}

@@ -1,0 +1,24 @@
+/* Tuning parameters of the fast path and their cache file.
+ *
+ * The cache plays the role of the reference's FFTW wisdom file
+ * (src/fft.c:26-45, named ${Ng}.wsdm in src/app/job_P.sh:13): fft_init's and
+ * fftgal_power_desc's `wisdom` string names it, "FFTW_ESTIMATE" or NULL means
+ * built-in defaults.  Text format, one line per configuration:
+ *     Ng interlace paint_tile fft_cols fft_rows
+ * (0 in the fft fields = automatic choice inside the FFT plan). */
+#ifndef FFTGAL_TUNE_H
+#define FFTGAL_TUNE_H
+
+typedef struct fftgal_tuning {
+    int paint_tile;
+    int fft_cols;
+    int fft_rows;
+} fftgal_tuning;
+
+/* defaults, overridden by a matching line of the cache file if present;
+ * returns 1 when the cache supplied the values */
+int fftgal_tune_lookup(const char *path, int Ng, int interlace, fftgal_tuning *t);
+/* append or replace the line for (Ng, interlace) */
+int fftgal_tune_store(const char *path, int Ng, int interlace, const fftgal_tuning *t);
+
+#endif

@@ -1,0 +1,81 @@
+"""Synthetic catalogs for tests and the benchmark (workload generation, not
+part of the estimator).
+
+uniform(Np, L, seed)
+    Np positions uniform in [0, L)^3 (the plain small-size case).
+
+lognormal(Np, L, seed, Ngs=256, ...)
+    Clustered mock from a Gaussian random field with a power-law spectrum
+    P(k) = A (k/k0)^n on an Ngs^3 seed grid:
+      1. white noise w (N(0,1) per cell) -> delta_G(k) = DFT(w) sqrt(P(k)/H^3),
+         H = L/Ngs, so that |delta_k|^2/V = P(k) in the reference's Fourier
+         convention (delta_k = H^3 DFT(delta_x), P = |delta_k|^2/V); DC = 0;
+      2. rho = exp(delta_G - sigma^2/2), normalised to mean 1;
+      3. exactly Np particles: multinomial counts over rho, each particle
+         uniform inside its seed cell;
+      4. linear redshift-space shift along z from the Zel'dovich
+         displacement psi_z(k) = i k_z / k^2 delta_G(k):  z += f psi_z[cell];
+      5. wrapped into [0, L), returned in a random order, float32 by default.
+    Deterministic for a given seed (numpy PCG64 and pocketfft).
+"""
+import numpy as np
+
+
+def uniform(Np, L, seed, dtype=np.float64):
+    rng = np.random.default_rng(seed)
+    pos = rng.random((3, Np)) * L
+    return pos[0].astype(dtype), pos[1].astype(dtype), pos[2].astype(dtype)
+
+
+def gaussian_field(Ngs, L, seed, A=2e4, k0=0.1, n=-1.5):
+    """delta_G(k) (rfft layout) of a Gaussian field with P(k) = A (k/k0)^n."""
+    rng = np.random.default_rng(seed)
+    H = L / Ngs
+    w = rng.standard_normal((Ngs, Ngs, Ngs))
+    W = np.fft.rfftn(w)
+    del w
+    kf = 2 * np.pi / L
+    ki = np.fft.fftfreq(Ngs, d=1.0 / Ngs) * kf
+    kk = np.fft.rfftfreq(Ngs, d=1.0 / Ngs) * kf
+    k2 = ki[:, None, None] ** 2 + ki[None, :, None] ** 2 + kk[None, None, :] ** 2
+    k2[0, 0, 0] = 1.0
+    amp = np.sqrt(A * (np.sqrt(k2) / k0) ** n / H ** 3)
+    amp[0, 0, 0] = 0.0
+    W *= amp
+    return W, ki, kk, k2
+
+
+def lognormal(Np, L, seed, Ngs=256, A=2e4, k0=0.1, n=-1.5, f=0.76, rsd=True,
+              dtype=np.float32):
+    rng = np.random.default_rng([seed, 1])
+    W, ki, kk, k2 = gaussian_field(Ngs, L, seed, A, k0, n)
+    delta = np.fft.irfftn(W, s=(Ngs, Ngs, Ngs), axes=(0, 1, 2))
+    sigma2 = float(delta.var())
+    rho = np.exp(delta - 0.5 * sigma2)
+    del delta
+    rho /= rho.sum()
+    counts = rng.multinomial(int(Np), rho.ravel())
+    del rho
+    cells = np.repeat(np.arange(Ngs ** 3, dtype=np.int64), counts)
+    del counts
+    H = L / Ngs
+    ci = cells // (Ngs * Ngs)
+    cj = (cells // Ngs) % Ngs
+    ck = cells % Ngs
+    x = (ci + rng.random(Np)) * H
+    y = (cj + rng.random(Np)) * H
+    z = (ck + rng.random(Np)) * H
+    if rsd and f != 0:
+        psi = np.fft.irfftn(1j * kk[None, None, :] / k2 * W, s=(Ngs, Ngs, Ngs), axes=(0, 1, 2))
+        z += f * psi.ravel()[cells]
+        del psi
+    del cells, ci, cj, ck, W
+    x %= L
+    y %= L
+    z %= L
+    perm = rng.permutation(Np)
+    out = tuple(np.ascontiguousarray(a[perm]).astype(dtype) for a in (x, y, z))
+    # float32 rounding can land exactly on L: wrap again
+    for a in out:
+        a[a >= L] -= L
+    return out, sigma2

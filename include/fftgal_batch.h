@@ -1,0 +1,93 @@
+/* Mixed-precision power-spectrum pipeline on one MI355X (gfx950).
+ *
+ * Additive API (no counterpart in eelregit/fftgal).  It computes, for
+ * catalogs of float32 positions already in device memory, what the
+ * reference computes with
+ *     fft_p2k (src/fft.c:257-276) or fft_p2g + fft_x2k + fft_deconv,
+ *     P (src/power.c:12-27), Pl (src/power.c:30-101), Pmu (power.c:104-178)
+ * but as three stages on the device:
+ *   paint  -- tile-sorted PCS scatter of the plain and half-cell dual grids
+ *             in one pass, int64 fixed-point accumulation (bit-identical
+ *             run to run);
+ *   FFT    -- FP32 Stockham r2c (rows) + 2 strided complex passes, twiddles
+ *             generated in FP64;
+ *   bin    -- one fused kernel: dual-grid offset phase, interlace average,
+ *             PCS deconvolution, |delta|^2/V - Pshot in FP64, Legendre
+ *             l = 0,2,4,6 and |mu|-wedge binning with FP64 sums.
+ * The stages of one enqueue are captured once in a hipGraph and replayed.
+ * Mode counts equal the reference's exactly; P_l agrees with the FP64 path
+ * within the FP32 FFT error.
+ *
+ * All functions return 0 on success and a negative code on failure;
+ * fftgal_last_error() describes the failure.
+ */
+#ifndef FFTGAL_BATCH_H
+#define FFTGAL_BATCH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fftgal_power fftgal_power_t;
+
+typedef struct fftgal_power_desc {
+    int Ng;              /* grid cells per side: even, prime factors <= 13 */
+    double L;            /* grid box side [Mpc/h] = catalog box / fold */
+    int interlace;       /* 1: plain + dual grid (fft_p2k); 0: plain grid */
+    double dK;           /* k-bin width [h/Mpc] */
+    double los[3];       /* line of sight (normalised internally) */
+    int Nmu;             /* |mu| wedges (0: none) */
+    long Np_max;         /* largest catalog accepted by enqueue */
+    int paint_tile;      /* paint tile side in cells (0: tuned / default) */
+    const char *wisdom;  /* tuning cache file (NULL or "FFTW_ESTIMATE": defaults) */
+} fftgal_power_desc;
+
+const char *fftgal_last_error(void);
+
+int fftgal_power_create(fftgal_power_t **out, const fftgal_power_desc *desc);
+int fftgal_power_destroy(fftgal_power_t *p);
+/* number of k bins (reference NK, src/power.c:36) */
+int fftgal_power_NK(const fftgal_power_t *p);
+
+/* Enqueue one catalog: Np float32 positions x, y, z in device memory
+ * (any real coordinates; the paint wraps them periodically into the grid
+ * box), V = catalog volume (nbar = Np/V, Pshot = V/Np).  Asynchronous on
+ * the plan's stream. */
+int fftgal_power_enqueue(fftgal_power_t *p, const float *x, const float *y, const float *z,
+                         long Np, double V);
+/* wait for the last enqueue */
+int fftgal_power_sync(fftgal_power_t *p);
+/* normalised results of the last enqueue (after sync):
+ * K[NK], N[NK], Pl[NK*4] as Pl(); Kmu, Nmu_count, Pmu [NK*Nmu] as Pmu().
+ * Any pointer may be NULL. */
+int fftgal_power_results(fftgal_power_t *p, double *K, long *N, double *Pl, double *Kmu,
+                         long *Nmu_count, double *Pmu);
+
+/* ---- stage access for profiling and tests ---- */
+/* device address of the int64 grids (plain, dual) after the paint stage,
+ * layout [Ng][Ng][Ng], units 2^-fftgal_power_fixed_bits() particles */
+int fftgal_power_paint_only(fftgal_power_t *p, const float *x, const float *y, const float *z,
+                            long Np, void **gridA, void **gridB);
+int fftgal_power_fixed_bits(const fftgal_power_t *p);
+/* device address of the FP32 spectra (plain, dual), padded layout */
+int fftgal_power_spectra(fftgal_power_t *p, void **A, void **B);
+/* tuning in use */
+int fftgal_power_tuning(const fftgal_power_t *p, int *paint_tile, int *fft_cols, int *fft_rows);
+/* use the hipGraph (1, default) or plain stream launches (0) */
+int fftgal_power_use_graph(fftgal_power_t *p, int on);
+/* the plan's stream (hipStream_t) */
+void *fftgal_power_stream(fftgal_power_t *p);
+/* run one catalog eagerly with events between the pipeline stages and
+ * return each stage's time in milliseconds (fftgal_power_stages() values;
+ * names from fftgal_power_stage_name); results as after an enqueue */
+int fftgal_power_stages(void);
+const char *fftgal_power_stage_name(int i);
+int fftgal_power_profile(fftgal_power_t *p, const float *x, const float *y, const float *z,
+                         long Np, double V, float *stage_ms);
+/* synchronous copy from device memory (e.g. the grids above) to host */
+int fftgal_memcpy_d2h(void *dst, const void *src, unsigned long bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

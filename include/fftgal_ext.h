@@ -1,0 +1,48 @@
+/* Additive entry points next to the drop-in API (fft.h, power.h).
+ *
+ * None of these exists in eelregit/fftgal; they expose what the GPU backing
+ * makes possible without changing any reference signature:
+ *  - keeping the grid on the device between calls (no PCIe traffic for
+ *    callers such as src/app/P.c that never touch grid->f);
+ *  - painting from catalogs already in device memory;
+ *  - Pl/Pmu results as arrays instead of text files, with any number of
+ *    mu wedges (src/power.c:111 fixes Nmu = 3).
+ */
+#ifndef FFTGAL_EXT_H
+#define FFTGAL_EXT_H
+
+#include "fft.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* on != 0: the device grid is authoritative between calls; host f is only
+ * refreshed by fft_sync_host() / fft_exportf().  Switching off copies the
+ * device grid to f first. */
+void fft_set_device_resident(fft_t *self, int on);
+/* copy the device grid to host f */
+void fft_sync_host(fft_t *self);
+/* device address of the FP64 grid (same padded layout as f) */
+double *fft_device_grid(fft_t *self);
+
+/* fft_p2g / fft_p2k for a catalog of Np FP64 positions in device memory
+ * (no randoms); V is the catalog volume as in gal_t.V */
+void fft_p2g_device(fft_t *self, const double *x, const double *y, const double *z,
+                    long Np, double V, double offset[3]);
+void fft_p2k_device(fft_t *self, const double *x, const double *y, const double *z,
+                    long Np, double V);
+/* P() without a gal_t: alpha = Np/Nrand (Nrand = 0: no randoms) */
+void P_params(fft_t *grid, long Np, double V, long Nrand);
+
+/* number of k bins of Pl/Pmu (src/power.c:36) */
+int fftgal_NK(int Ng, double L, double dK);
+/* Pl as arrays: K[NK], N[NK], P[NK*4] (l = 0,2,4,6); returns NK */
+int Pl_bins(fft_t *grid, double los[3], double dK, double *K, long *N, double *P);
+/* Pmu with Nmu wedges as arrays: K, N, P each [NK*Nmu]; returns NK */
+int Pmu_bins(fft_t *grid, double los[3], double dK, int Nmu, double *K, long *N, double *P);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,468 @@
+/*
+ * FP64 CPU restatement of the reference power-spectrum path.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h): the checker, never the product.
+ *
+ * Follows eelregit/fftgal src/fft.c, src/power.c, src/gal.c, src/gal.h in
+ * evaluation order; the FFT is the oracle's own (fft64.c) instead of FFTW.
+ * Deliberate differences from the reference, none of which changes a result:
+ *   - no stderr timing lines, no FFTW planning / wisdom (fft.c:26-45);
+ *   - the sum-of-counts check uses long arithmetic (power.c:76,153 overflow
+ *     int for Ng >= 1291) and is returned, not asserted;
+ *   - Pmu takes Nmu as a parameter (power.c:111 fixes it to 3).
+ */
+#include <stdlib.h>
+#include <stdio.h>
+#include <string.h>
+#include <math.h>
+#include <assert.h>
+#include "oracle.h"
+
+/* sqrt(3) to double precision (the value gsl_math.h names M_SQRT3, used at power.c:36,110) */
+#define ORC_SQRT3 1.73205080756887729352744634151
+
+#define FIDX(g, i, j, k) (((long)(i) * (g)->Ng + (j)) * (g)->Ng_pad + (k))
+
+static inline double sq(double x) { return x * x; }          /* gsl_pow_2 */
+static inline double cube(double x) { return x * x * x; }    /* gsl_pow_3 */
+static inline double quart(double x) { double s = x * x; return s * s; } /* gsl_pow_4 */
+
+/* ---------------------------------------------------------------- catalog */
+
+/* src/gal.h:19-26 */
+void oracle_hat(double vec[3]) {
+    double amp = sqrt(vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2]);
+    if (amp > 1e-7) {
+        vec[0] /= amp;
+        vec[1] /= amp;
+        vec[2] /= amp;
+    }
+}
+
+/* src/gal.c:11-21 */
+ogal_t *oracle_gal_init(long Np, double V) {
+    ogal_t *g = (ogal_t *)calloc(1, sizeof(ogal_t));
+    assert(g != NULL);
+    g->Np = Np;
+    g->V = V;
+    size_t n = Np > 0 ? (size_t)Np : 1;
+    g->x = (double *)malloc(n * sizeof(double));
+    g->y = (double *)malloc(n * sizeof(double));
+    g->z = (double *)malloc(n * sizeof(double));
+    assert(g->x && g->y && g->z);
+    return g;
+}
+
+/* src/gal.c:41-48 */
+void oracle_gal_free(ogal_t *g) {
+    if (g != NULL) {
+        oracle_gal_free(g->rand);
+        free(g->x); free(g->y); free(g->z);
+        free(g->vx); free(g->vy); free(g->vz);
+    }
+    free(g);
+}
+
+/* src/gal.c:51-60 */
+void oracle_gal_wrap(ogal_t *g, double L) {
+    double Linv = 1 / L;
+    for (long p = 0; p < g->Np; ++p) {
+        g->x[p] -= L * floor(g->x[p] * Linv);
+        g->y[p] -= L * floor(g->y[p] * Linv);
+        g->z[p] -= L * floor(g->z[p] * Linv);
+    }
+}
+
+/* src/gal.c:63-77: plane-parallel shift x + (v.los)/(aH) los, new catalog */
+ogal_t *oracle_gal_rsd(const ogal_t *g, double los[3], double aH) {
+    oracle_hat(los);
+    double aHinv = 1 / aH;
+    ogal_t *d = oracle_gal_init(g->Np, g->V);
+    for (long p = 0; p < d->Np; ++p) {
+        double vdl = los[0] * g->vx[p] + los[1] * g->vy[p] + los[2] * g->vz[p];
+        d->x[p] = g->x[p] + aHinv * vdl * los[0];
+        d->y[p] = g->y[p] + aHinv * vdl * los[1];
+        d->z[p] = g->z[p] + aHinv * vdl * los[2];
+    }
+    return d;
+}
+
+/* ------------------------------------------------------------------- grid */
+
+/* src/fft.c:13-24 (allocation; the plans are the oracle's own) */
+offt_t *oracle_fft_init(int Ng, double L) {
+    assert(Ng >= 1);
+    offt_t *g = (offt_t *)calloc(1, sizeof(offt_t));
+    assert(g != NULL);
+    g->Ng = Ng;
+    g->L = L;
+    g->Ng_pad = 2 * (Ng / 2 + 1);
+    g->Ng3_pad = (long)Ng * Ng * g->Ng_pad;
+    g->f = (double *)calloc((size_t)g->Ng3_pad, sizeof(double));
+    assert(g->f != NULL);
+    g->pN = (offt64_plan *)calloc(1, sizeof(offt64_plan));
+    g->pH = (offt64_plan *)calloc(1, sizeof(offt64_plan));
+    assert(g->pN && g->pH);
+    assert(offt64_plan_init(g->pN, Ng) == 0);
+    assert(offt64_plan_init(g->pH, Ng / 2 > 0 ? Ng / 2 : 1) == 0);
+    return g;
+}
+
+/* src/fft.c:50-54 */
+void oracle_fft_free(offt_t *g) {
+    if (g) {
+        free(g->f);
+        offt64_plan_free(g->pN);
+        offt64_plan_free(g->pH);
+        free(g->pN);
+        free(g->pH);
+    }
+    free(g);
+}
+
+/* PCS kernel weights and wrapped node indices for one axis (fft.c:68-88) */
+static inline void pcs_axis(double u, int Ng, int idx[4], double w[4]) {
+    int g1 = (int)floor(u);
+    double d = u - g1;
+    g1 = (g1 % Ng + Ng) % Ng;
+    idx[0] = (g1 - 1 + Ng) % Ng;
+    idx[1] = g1;
+    idx[2] = (g1 + 1) % Ng;
+    idx[3] = (g1 + 2) % Ng;
+    w[0] = cube(1 - d) / 6;
+    w[1] = ((3 * d - 6) * sq(d) + 4) / 6;
+    w[2] = (((-3 * d + 3) * d + 3) * d + 1) / 6;
+    w[3] = cube(d) / 6;
+}
+
+static void paint_catalog(offt_t *g, const ogal_t *c, const double off[3],
+                          double Hinv, double scale, int negate) {
+    const int Ng = g->Ng;
+    for (long p = 0; p < c->Np; ++p) {
+        int gi[4], gj[4], gk[4];
+        double wx[4], wy[4], wz[4];
+        pcs_axis(c->x[p] * Hinv - off[0], Ng, gi, wx);
+        pcs_axis(c->y[p] * Hinv - off[1], Ng, gj, wy);
+        pcs_axis(c->z[p] * Hinv - off[2], Ng, gk, wz);
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+                for (int k = 0; k < 4; ++k) {
+                    if (negate)
+                        g->f[FIDX(g, gi[i], gj[j], gk[k])] -= scale * wx[i] * wy[j] * wz[k];
+                    else
+                        g->f[FIDX(g, gi[i], gj[j], gk[k])] += wx[i] * wy[j] * wz[k];
+                }
+    }
+}
+
+/* src/fft.c:57-131 */
+void oracle_fft_p2g(offt_t *g, const ogal_t *part, double offset[3]) {
+    const int Ng = g->Ng;
+    for (int i = 0; i < 3; ++i) {
+        offset[i] = remainder(offset[i], Ng);
+        g->offset[i] = offset[i];
+    }
+    memset(g->f, 0, (size_t)g->Ng3_pad * sizeof(double));
+    double Hinv = Ng / g->L;
+    paint_catalog(g, part, offset, Hinv, 1.0, 0);
+    if (part->rand != NULL) {
+        double alpha = (double)part->Np / part->rand->Np;
+        paint_catalog(g, part->rand, offset, Hinv, alpha, 1);
+    }
+    double H3 = cube(g->L / Ng);
+    double nbar = part->Np / part->V;
+    double Npcinv = 1 / nbar / H3;
+    for (long q = 0; q < g->Ng3_pad; ++q)
+        g->f[q] *= Npcinv;
+}
+
+/* src/fft.c:196-233 */
+void oracle_fft_offset_phase(offt_t *g, int onoff) {
+    assert(onoff != 0);
+    onoff = onoff > 0 ? 1 : -1;
+    const int Ng = g->Ng;
+    double *ph = (double *)malloc((size_t)3 * Ng * 2 * sizeof(double));
+    assert(ph != NULL);
+#define PH(d, j, c) ph[((d) * Ng + (j)) * 2 + (c)]
+    for (int d = 0; d < 3; ++d) { PH(d, 0, 0) = 1; PH(d, 0, 1) = 0; }
+    for (int d = 0; d < 3; ++d)
+        for (int j = 1; j <= Ng / 2; ++j) {
+            double arg = -onoff * 2 * M_PI * j / Ng * g->offset[d];
+            PH(d, j, 0) = cos(arg);
+            PH(d, j, 1) = sin(arg);
+            PH(d, Ng - j, 0) = PH(d, j, 0);      /* overwrites j = Ng/2: Nyquist = -Ng/2 */
+            PH(d, Ng - j, 1) = -PH(d, j, 1);
+        }
+    for (int i = 0; i < Ng; ++i)
+        for (int j = 0; j < Ng; ++j)
+            for (int k = 0; k <= Ng / 2; ++k) {
+                double pr = PH(0, i, 0) * PH(1, j, 0) * PH(2, k, 0)
+                          - PH(0, i, 0) * PH(1, j, 1) * PH(2, k, 1)
+                          - PH(0, i, 1) * PH(1, j, 0) * PH(2, k, 1)
+                          - PH(0, i, 1) * PH(1, j, 1) * PH(2, k, 0);
+                double pi = PH(0, i, 1) * PH(1, j, 0) * PH(2, k, 0)
+                          + PH(0, i, 0) * PH(1, j, 1) * PH(2, k, 0)
+                          + PH(0, i, 0) * PH(1, j, 0) * PH(2, k, 1)
+                          - PH(0, i, 1) * PH(1, j, 1) * PH(2, k, 1);
+                double *c = g->f + FIDX(g, i, j, 2 * k);
+                double fr = c[0], fi = c[1];
+                c[0] = fr * pr - fi * pi;
+                c[1] = fr * pi + fi * pr;
+            }
+#undef PH
+    free(ph);
+}
+
+/* src/fft.c:172-181 */
+void oracle_fft_x2k(offt_t *g, int offset_phase_on) {
+    offt64_r2c_3d(g);
+    double H3 = cube(g->L / g->Ng);
+    for (long q = 0; q < g->Ng3_pad; ++q)
+        g->f[q] *= H3;
+    if (offset_phase_on)
+        oracle_fft_offset_phase(g, 1);
+}
+
+/* src/fft.c:184-193 */
+void oracle_fft_k2x(offt_t *g, int offset_phase_off) {
+    offt64_c2r_3d(g);
+    double L3inv = 1 / cube(g->L);
+    for (long q = 0; q < g->Ng3_pad; ++q)
+        g->f[q] *= L3inv;
+    if (offset_phase_off)
+        oracle_fft_offset_phase(g, -1);
+}
+
+/* src/fft.c:236-254 */
+void oracle_fft_deconv(offt_t *g) {
+    const int Ng = g->Ng;
+    double *Winv = (double *)malloc((size_t)Ng * sizeof(double));
+    assert(Winv != NULL);
+    Winv[0] = 1;
+    for (int i = 1; i <= Ng / 2; ++i) {
+        double arg = M_PI * i / Ng;
+        Winv[i] = quart(arg / sin(arg));
+        Winv[Ng - i] = Winv[i];
+    }
+    for (int i = 0; i < Ng; ++i)
+        for (int j = 0; j < Ng; ++j)
+            for (int k = 0; k <= Ng / 2; ++k) {
+                double W3 = Winv[i] * Winv[j] * Winv[k];
+                g->f[FIDX(g, i, j, 2 * k)] *= W3;
+                g->f[FIDX(g, i, j, 2 * k + 1)] *= W3;
+            }
+    free(Winv);
+}
+
+/* src/fft.c:279-283 */
+double *oracle_fft_exportf(const offt_t *g) {
+    double *d = (double *)malloc((size_t)g->Ng3_pad * sizeof(double));
+    assert(d != NULL);
+    memcpy(d, g->f, (size_t)g->Ng3_pad * sizeof(double));
+    return d;
+}
+
+/* src/fft.c:286-289 */
+void oracle_fft_importf(offt_t *g, const double *src) {
+    assert(src != NULL);
+    memcpy(g->f, src, (size_t)g->Ng3_pad * sizeof(double));
+}
+
+/* src/fft.c:257-276: interlacing = dual (offset 1/2, phase on) + plain, average, deconvolve */
+void oracle_fft_p2k(offt_t *g, const ogal_t *part) {
+    double od[3] = {0.5, 0.5, 0.5};
+    oracle_fft_p2g(g, part, od);
+    oracle_fft_x2k(g, 1);
+    double *fd = oracle_fft_exportf(g);
+    double on[3] = {0, 0, 0};
+    oracle_fft_p2g(g, part, on);
+    oracle_fft_x2k(g, 0);
+    for (long q = 0; q < g->Ng3_pad; ++q)
+        g->f[q] = 0.5 * (fd[q] + g->f[q]);
+    oracle_fft_deconv(g);
+    free(fd);
+}
+
+/* ------------------------------------------------------------- estimators */
+
+/* src/power.c:12-27 */
+void oracle_P(offt_t *g, const ogal_t *part) {
+    const int Ng = g->Ng;
+    double Vinv = 1 / part->V;
+    double alpha = part->rand == NULL ? 0 : (double)part->Np / part->rand->Np;
+    double Pshot = (1 + alpha) * part->V / part->Np;
+    g->f[FIDX(g, 0, 0, 0)] = g->f[FIDX(g, 0, 0, 1)] = 0;
+    for (int i = 0; i < Ng; ++i)
+        for (int j = 0; j < Ng; ++j)
+            for (int k = (i == 0 && j == 0); k <= Ng / 2; ++k) {
+                double *c = g->f + FIDX(g, i, j, 2 * k);
+                double d2 = sq(c[0]) + sq(c[1]);
+                c[0] = d2 * Vinv - Pshot;
+                c[1] = 0;
+            }
+}
+
+/* power.c:36,110 */
+int oracle_NK(int Ng, double L, double dK) {
+    double KF = 2 * M_PI / L;
+    double dKinv = KF / dK;
+    return (int)floor(ORC_SQRT3 * (Ng / 2) * dKinv) + 1;
+}
+
+/* signed integer frequency, Nyquist index carries -Ng/2 (power.c:46-51) */
+static double *kval_table(int Ng) {
+    double *kv = (double *)malloc((size_t)(Ng > 0 ? Ng : 1) * sizeof(double));
+    assert(kv != NULL);
+    kv[0] = 0;
+    for (int i = 1; i <= Ng / 2; ++i) {
+        kv[i] = i;
+        kv[Ng - i] = -i;
+    }
+    return kv;
+}
+
+/* src/power.c:30-76 (accumulation + normalisation; no file) */
+int oracle_Pl_bins(const offt_t *g, double los[3], double dK,
+                   double *K, long *N, double *P) {
+    const int Ng = g->Ng;
+    oracle_hat(los);
+    double KF = 2 * M_PI / g->L;
+    double dKinv = KF / dK;
+    int NK = (int)floor(ORC_SQRT3 * (Ng / 2) * dKinv) + 1;
+    const int Nl = 4;
+    for (int b = 0; b < NK; ++b) {
+        K[b] = 0;
+        N[b] = 0;
+        for (int l = 0; l < Nl; ++l) P[b * Nl + l] = 0;
+    }
+    double *kv = kval_table(Ng);
+    for (int i = 0; i < Ng; ++i)
+        for (int j = 0; j < Ng; ++j)
+            for (int k = (i == 0 && j == 0); k <= Ng / 2; ++k) {
+                int count = 1 + (2 * k % Ng > 0);
+                double d2 = g->f[FIDX(g, i, j, 2 * k)];
+                d2 *= count;
+                double Kamp = sqrt(sq(kv[i]) + sq(kv[j]) + sq(kv[k]));
+                int iK = (int)floor(Kamp * dKinv);
+                double mu2 = sq((kv[i] * los[0] + kv[j] * los[1] + kv[k] * los[2]) / Kamp);
+                K[iK] += count * Kamp;
+                N[iK] += count;
+                P[iK * Nl + 0] += d2;
+                P[iK * Nl + 1] += d2 * (1.5 * mu2 - 0.5);
+                P[iK * Nl + 2] += d2 * ((4.375 * mu2 - 3.75) * mu2 + 0.375);
+                P[iK * Nl + 3] += d2 * (((14.4375 * mu2 - 19.6875) * mu2 + 6.5625) * mu2 - 0.3125);
+            }
+    free(kv);
+    long Ntot = 0;
+    for (int b = 0; b < NK; ++b) {
+        K[b] *= KF / N[b];
+        for (int l = 0; l < Nl; ++l)
+            P[b * Nl + l] *= (4 * l + 1.0) / N[b];
+        Ntot += N[b];
+    }
+    assert(Ntot == (long)Ng * Ng * Ng - 1);
+    return NK;
+}
+
+/* src/power.c:104-153 with Nmu as a parameter */
+int oracle_Pmu_bins(const offt_t *g, double los[3], double dK, int Nmu,
+                    double *Kout, long *Nout, double *Pout) {
+    const int Ng = g->Ng;
+    oracle_hat(los);
+    double KF = 2 * M_PI / g->L;
+    double dKinv = KF / dK;
+    int NK = (int)floor(ORC_SQRT3 * (Ng / 2) * dKinv) + 1;
+    const int W = Nmu + 1;
+    double *K = (double *)calloc((size_t)NK * W, sizeof(double));
+    double *P = (double *)calloc((size_t)NK * W, sizeof(double));
+    long *N = (long *)calloc((size_t)NK * W, sizeof(long));
+    assert(K && P && N);
+    double *kv = kval_table(Ng);
+    for (int i = 0; i < Ng; ++i)
+        for (int j = 0; j < Ng; ++j)
+            for (int k = (i == 0 && j == 0); k <= Ng / 2; ++k) {
+                int count = 1 + (2 * k % Ng > 0);
+                double d2 = g->f[FIDX(g, i, j, 2 * k)];
+                d2 *= count;
+                double Kamp = sqrt(sq(kv[i]) + sq(kv[j]) + sq(kv[k]));
+                int iK = (int)floor(Kamp * dKinv);
+                double mu = (kv[i] * los[0] + kv[j] * los[1] + kv[k] * los[2]) / Kamp;
+                int imu = (int)floor(fabs(mu * Nmu));
+                K[iK * W + imu] += count * Kamp;
+                N[iK * W + imu] += count;
+                P[iK * W + imu] += d2;
+            }
+    free(kv);
+    for (int b = 0; b < NK; ++b) {     /* corner case mu = 1 (power.c:139-143) */
+        K[b * W + Nmu - 1] += K[b * W + Nmu];
+        N[b * W + Nmu - 1] += N[b * W + Nmu];
+        P[b * W + Nmu - 1] += P[b * W + Nmu];
+    }
+    long Ntot = 0;
+    for (int b = 0; b < NK; ++b)
+        for (int m = 0; m < Nmu; ++m) {
+            long n = N[b * W + m];
+            if (n) {
+                K[b * W + m] *= KF / n;
+                P[b * W + m] /= n;
+                Ntot += n;
+            }
+            Kout[b * Nmu + m] = K[b * W + m];
+            Nout[b * Nmu + m] = n;
+            Pout[b * Nmu + m] = P[b * W + m];
+        }
+    assert(Ntot == (long)Ng * Ng * Ng - 1);
+    free(K); free(P); free(N);
+    return NK;
+}
+
+/* src/power.c:78-99 */
+void oracle_Pl(offt_t *g, const ogal_t *part, double los[3], double dK, const char *file) {
+    int NK = oracle_NK(g->Ng, g->L, dK);
+    double *K = (double *)malloc((size_t)NK * sizeof(double));
+    long *N = (long *)malloc((size_t)NK * sizeof(long));
+    double *P = (double *)malloc((size_t)NK * 4 * sizeof(double));
+    assert(K && N && P);
+    oracle_Pl_bins(g, los, dK, K, N, P);
+    double alpha = part->rand == NULL ? 0 : (double)part->Np / part->rand->Np;
+    FILE *fp = fopen(file, "w");
+    assert(fp != NULL);
+    fprintf(fp, "# NK %d\n# dK %g\n# Nl %d\n# Ng %d\n# L %g\n# Np %ld\n# V %g\n# alpha %g\n",
+            NK, dK, 4, g->Ng, g->L, part->Np, part->V, alpha);
+    fprintf(fp, "# los[3] %g %g %g\n# K N", los[0], los[1], los[2]);
+    for (int l = 0; l < 4; ++l) fprintf(fp, " P%d", 2 * l);
+    fprintf(fp, "\n");
+    for (int b = 0; b < NK; ++b) {
+        fprintf(fp, "%e %9ld", K[b], N[b]);
+        for (int l = 0; l < 4; ++l) fprintf(fp, " % e", P[b * 4 + l]);
+        fprintf(fp, "\n");
+    }
+    fclose(fp);
+    free(K); free(N); free(P);
+}
+
+/* src/power.c:155-176 (Nmu = 3 as in power.c:111) */
+void oracle_Pmu(offt_t *g, const ogal_t *part, double los[3], double dK, const char *file) {
+    const int Nmu = 3;
+    int NK = oracle_NK(g->Ng, g->L, dK);
+    double *K = (double *)malloc((size_t)NK * Nmu * sizeof(double));
+    long *N = (long *)malloc((size_t)NK * Nmu * sizeof(long));
+    double *P = (double *)malloc((size_t)NK * Nmu * sizeof(double));
+    assert(K && N && P);
+    oracle_Pmu_bins(g, los, dK, Nmu, K, N, P);
+    double alpha = part->rand == NULL ? 0 : (double)part->Np / part->rand->Np;
+    FILE *fp = fopen(file, "w");
+    assert(fp != NULL);
+    fprintf(fp, "# NK %d\n# dK %g\n# Nmu %d\n# Ng %d\n# L %g\n# Np %ld\n# V %g\n# alpha %g\n",
+            NK, dK, Nmu, g->Ng, g->L, part->Np, part->V, alpha);
+    fprintf(fp, "# los[3] %g %g %g\n#", los[0], los[1], los[2]);
+    for (int m = 0; m < Nmu; ++m) fprintf(fp, " K%d N%d P%d", m, m, m);
+    fprintf(fp, "\n");
+    for (int b = 0; b < NK; ++b) {
+        fprintf(fp, "%e %9ld % e", K[b * Nmu], N[b * Nmu], P[b * Nmu]);
+        for (int m = 1; m < Nmu; ++m)
+            fprintf(fp, " %e %9ld % e", K[b * Nmu + m], N[b * Nmu + m], P[b * Nmu + m]);
+        fprintf(fp, "\n");
+    }
+    fclose(fp);
+    free(K); free(N); free(P);
+}

@@ -1,0 +1,264 @@
+"""Pins of the FP64 oracle (CPU).  Each check is independent of the
+oracle's own formulas: the reference's known-answer test
+(src/tests/octet.c), numpy.fft golden vectors, brute-force DFT, the PCS
+kernel written as the cubic B-spline M4, full-grid enumeration of the
+k-space bins with numpy's Legendre polynomials, and closed forms."""
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fft_golden.npz")
+SIZES = (1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 16)
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(GOLD)
+
+
+@pytest.mark.parametrize("Ng", SIZES)
+def test_fft_against_numpy_golden(orc, gold, Ng):
+    X = orc.fft_r2c_3d(gold[f"x_{Ng}"])
+    ref = gold[f"X_{Ng}"]
+    assert np.abs(X - ref).max() <= 1e-14 * max(np.abs(ref).max(), 1)
+    y = orc.fft_c2r_3d(gold[f"Y_{Ng}"], Ng)
+    ref = gold[f"y_{Ng}"]
+    assert np.abs(y - ref).max() <= 1e-14 * max(np.abs(ref).max(), 1)
+
+
+@pytest.mark.parametrize("Ng", [2, 3, 4])
+def test_fft_brute_force_dft(orc, Ng):
+    rng = np.random.default_rng(Ng)
+    x = rng.standard_normal((Ng, Ng, Ng))
+    n = np.arange(Ng)
+    X = np.zeros((Ng, Ng, Ng // 2 + 1), dtype=complex)
+    for i in range(Ng):
+        for j in range(Ng):
+            for k in range(Ng // 2 + 1):
+                ph = np.exp(-2j * np.pi * (i * n[:, None, None] + j * n[None, :, None]
+                                           + k * n[None, None, :]) / Ng)
+                X[i, j, k] = (x * ph).sum()
+    assert np.abs(orc.fft_r2c_3d(x) - X).max() <= 1e-13 * np.abs(X).max()
+
+
+def test_fft_parseval_and_roundtrip(orc):
+    Ng = 24
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal((Ng, Ng, Ng))
+    X = orc.fft_r2c_3d(x)
+    w = np.full(X.shape[2], 2.0)
+    w[0] = 1.0
+    w[-1] = 1.0          # Ng even: k = Ng/2 counted once
+    e_k = (np.abs(X) ** 2 * w).sum() / Ng ** 3
+    assert abs(e_k - (x ** 2).sum()) <= 1e-12 * e_k
+    back = orc.fft_c2r_3d(X, Ng) / Ng ** 3
+    assert np.abs(back - x).max() <= 1e-13
+
+
+def test_octet_known_answer(orc):
+    """src/tests/octet.c:8-62: Ng = 2, L = 2, one particle at (-8,-8,-8),
+    offset (7,7,7) -> -1; eps = 1e-15."""
+    eps = 1e-15
+    g = orc.Grid(2, 2.0)
+    c = orc.Catalog([-8.0], [-8.0], [-8.0], 8.0)
+    off = g.p2g(c, (7, 7, 7))
+    assert list(off) == [-1.0, -1.0, -1.0]
+    ijk = np.indices((2, 2, 2))
+    exp = 8.0 / 27 * (ijk[0] + 1) * (ijk[1] + 1) * (ijk[2] + 1)
+    assert np.abs(g.f[:, :, :2] - exp).max() < eps
+    g.x2k(1)
+    exp = 8.0 / (1 + 2 * ijk[0][:, :, 0]) / (1 + 2 * ijk[1][:, :, 0])
+    assert np.abs(g.f[:, :, 0] - exp).max() < eps
+    assert np.abs(g.f[:, :, 1]).max() < eps
+    g.k2x(0)
+    exp = 64.0 / 27 / (ijk[0] + 1) / (ijk[1] + 1) / (ijk[2] + 1)
+    assert np.abs(g.f[:, :, :2] - exp).max() < eps
+
+
+def m4(s):
+    """cubic B-spline (PCS kernel) by its piecewise definition"""
+    s = np.abs(s)
+    return np.where(s < 1, (4 - 6 * s ** 2 + 3 * s ** 3) / 6,
+                    np.where(s < 2, (2 - s) ** 3 / 6, 0.0))
+
+
+def test_paint_single_particle_is_the_b_spline(orc):
+    Ng, L = 8, 16.0
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        pos = rng.random(3) * 3 * L - L
+        off = rng.random(3) * 4 - 2
+        g = orc.Grid(Ng, L)
+        c = orc.Catalog([pos[0]], [pos[1]], [pos[2]], L ** 3)
+        o = g.p2g(c, off)
+        u = pos * Ng / L - o
+        exp = np.zeros((Ng, Ng, Ng))
+        n = np.arange(-3 * Ng, 4 * Ng)   # every periodic image of every node
+        w = [m4(u[a] - n) for a in range(3)]
+        for a in range(3):
+            w[a] = np.bincount(n % Ng, weights=w[a], minlength=Ng)
+        exp = w[0][:, None, None] * w[1][None, :, None] * w[2][None, None, :]
+        exp *= Ng ** 3      # 1 / (nbar H^3) with Np = 1, V = L^3
+        assert np.abs(g.f[:, :, :Ng] - exp).max() <= 1e-12
+
+
+def test_paint_mass_fold_and_translation(orc):
+    Ng, L, fold = 16, 100.0, 2
+    rng = np.random.default_rng(4)
+    x, y, z = rng.random((3, 2000)) * L
+    V = L ** 3
+    # folding: grid box L/fold, catalog volume still L^3 -> mean fold^3
+    g = orc.Grid(Ng, L / fold)
+    c = orc.Catalog(x, y, z, V)
+    g.p2g(c, (0, 0, 0))
+    assert abs(g.f[:, :, :Ng].mean() - fold ** 3) <= 1e-12 * fold ** 3
+    # one-cell translation along x is a cyclic shift of the grid
+    H = L / fold / Ng
+    g2 = orc.Grid(Ng, L / fold)
+    c2 = orc.Catalog(x + H, y, z, V)
+    g2.p2g(c2, (0, 0, 0))
+    assert np.abs(np.roll(g.f[:, :, :Ng], 1, axis=0) - g2.f[:, :, :Ng]).max() <= 1e-10
+
+
+def test_offset_phase_closed_form(orc):
+    Ng, L = 6, 1.0
+    g = orc.Grid(Ng, L)
+    g.p2g(orc.Catalog([0.1], [0.2], [0.3], 1.0), (0.25, -0.5, 1.25))
+    o = np.array(g.offset)
+    g.f[:] = 0
+    g.f[:, :, 0::2] = 1.0           # every mode = 1 + 0i
+    g.offset_phase(1)
+    kv = np.where(np.arange(Ng) * 2 < Ng, np.arange(Ng), np.arange(Ng) - Ng)
+    kz = kv[: Ng // 2 + 1]
+    exp = np.exp(-2j * np.pi * (o[0] * kv[:, None, None] + o[1] * kv[None, :, None]
+                                + o[2] * kz[None, None, :]) / Ng)
+    assert np.abs(g.fk() - exp).max() <= 1e-14
+    g.offset_phase(-1)
+    assert np.abs(g.fk() - 1).max() <= 1e-14
+
+
+def test_deconv_closed_form(orc):
+    Ng = 10
+    g = orc.Grid(Ng, 5.0)
+    g.f[:] = 0
+    g.f[:, :, 0::2] = 1.0
+    g.deconv()
+    a = np.pi * np.abs(np.where(np.arange(Ng) * 2 <= Ng, np.arange(Ng), Ng - np.arange(Ng))) / Ng
+    w = np.ones(Ng)
+    w[1:] = (a[1:] / np.sin(a[1:])) ** 4
+    exp = w[:, None, None] * w[None, :, None] * w[None, None, : Ng // 2 + 1]
+    assert np.abs(g.fk().real - exp).max() <= 1e-13 * exp.max()
+
+
+def full_grid_bins(Pk_half, Ng, L, los, dK, Nmu):
+    """Bins by enumerating modes one by one (weight 1 each, DC excluded):
+    every stored mode f = (Kval_i, Kval_j, Kval_k) of the half spectrum and,
+    for 0 < k < Ng/2, its partner -f carrying the same P (Hermitian
+    symmetry).  Kval gives the Nyquist index -Ng/2 (src/power.c:46-51); with
+    a line of sight along a Nyquist axis this pairing, not the index grid,
+    defines mu of the partner.  Legendre polynomials from numpy."""
+    from numpy.polynomial import legendre
+    kv = np.where(2 * np.arange(Ng) < Ng, np.arange(Ng), np.arange(Ng) - Ng).astype(float)
+    kh = kv[: Ng // 2 + 1]
+    FX, FY, FZ = np.meshgrid(kv, kv, kh, indexing="ij")
+    inner = (np.arange(Ng // 2 + 1) > 0) & (2 * np.arange(Ng // 2 + 1) != Ng)
+    sel = np.broadcast_to(inner[None, None, :], FX.shape)
+    KX = np.concatenate([FX.ravel(), -FX[sel]])
+    KY = np.concatenate([FY.ravel(), -FY[sel]])
+    KZ = np.concatenate([FZ.ravel(), -FZ[sel]])
+    P = np.concatenate([Pk_half.ravel(), Pk_half[sel]])
+    kamp = np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
+    # mode counts independently from the full index grid (fftfreq)
+    ff = np.fft.fftfreq(Ng, 1.0 / Ng)
+    kfull = np.sqrt(ff[:, None, None] ** 2 + ff[None, :, None] ** 2 + ff[None, None, :] ** 2)
+    assert np.array_equal(np.sort(kfull.ravel()), np.sort(kamp))
+    l = np.array(los, float)
+    nl = np.linalg.norm(l)
+    if nl > 1e-7:
+        l = l / nl
+    m = kamp > 0
+    mu = np.zeros_like(kamp)
+    mu[m] = (KX * l[0] + KY * l[1] + KZ * l[2])[m] / kamp[m]
+    KF = 2 * np.pi / L
+    ib = np.floor(kamp * KF / dK).astype(int)
+    NK = int(np.floor(np.sqrt(3) * (Ng // 2) * KF / dK)) + 1
+    N = np.bincount(ib[m], minlength=NK)
+    K = np.bincount(ib[m], weights=kamp[m], minlength=NK)
+    Pl = np.zeros((NK, 4))
+    for q in range(4):
+        c = np.zeros(2 * q + 1)
+        c[-1] = 1
+        Lq = legendre.legval(mu[m], c)
+        Pl[:, q] = np.bincount(ib[m], weights=P[m] * Lq, minlength=NK)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        Kout = K * KF / N
+        Pl = Pl * (4 * np.arange(4) + 1) / N[:, None]
+    im = np.minimum(np.floor(np.abs(mu) * Nmu).astype(int), Nmu - 1)
+    Nw = np.zeros((NK, Nmu), dtype=np.int64)
+    Pw = np.zeros((NK, Nmu))
+    np.add.at(Nw, (ib[m], im[m]), 1)
+    np.add.at(Pw, (ib[m], im[m]), P[m])
+    with np.errstate(invalid="ignore", divide="ignore"):
+        Pw = np.where(Nw > 0, Pw / np.maximum(Nw, 1), 0.0)
+    return Kout, N, Pl, Nw, Pw
+
+
+@pytest.mark.parametrize("Ng,los,dKf", [(8, (0, 0, 1), 1.0), (10, (1, 1, 0), 1.0),
+                                        (12, (1, 2, 3), 0.7), (16, (0, 0, 0), 2.0)])
+def test_P_Pl_Pmu_against_full_grid_enumeration(orc, Ng, los, dKf):
+    L = 50.0
+    rng = np.random.default_rng(Ng)
+    x, y, z = rng.random((3, 300)) * L
+    c = orc.Catalog(x, y, z, L ** 3)
+    g = orc.Grid(Ng, L)
+    g.p2k(c)
+    fk = g.fk()
+    g.P(c)
+    # P closed form (src/power.c:12-27)
+    Pexp = np.abs(fk) ** 2 / L ** 3 - L ** 3 / 300
+    Pexp[0, 0, 0] = 0
+    assert np.abs(g.fk().real - Pexp).max() <= 1e-12 * np.abs(Pexp).max()
+    assert np.abs(g.fk().imag).max() == 0
+    dK = dKf * 2 * np.pi / L
+    Nmu = 3
+    K, N, Pl = g.Pl_bins(los, dK)
+    Km, Nm, Pm = g.Pmu_bins(los, dK, Nmu)
+    Kr, Nr, Plr, Nwr, Pwr = full_grid_bins(g.fk().real, Ng, L, los, dK, Nmu)
+    assert np.array_equal(N, Nr) and N.sum() == Ng ** 3 - 1
+    ok = N > 0
+    assert np.allclose(K[ok], Kr[ok], rtol=1e-13)
+    assert np.all(np.isnan(Pl[~ok]))          # empty bins print NaN (0/0)
+    assert np.allclose(Pl[ok], Plr[ok], rtol=1e-9, atol=1e-9 * np.abs(Plr[ok]).max())
+    assert np.array_equal(Nm, Nwr) and Nm.sum() == Ng ** 3 - 1
+    assert np.allclose(Pm, Pwr, rtol=1e-9, atol=1e-9 * np.abs(Pwr).max())
+
+
+def test_pl_of_unit_power_is_closed_form(orc):
+    """P(k) = 1 everywhere, los = 0 (mu = 0): P0 = 1, P2 = 5 L2(0) = -2.5,
+    P4 = 9 L4(0) = 3.375, P6 = 13 L6(0) = -4.0625 in every non-empty bin."""
+    Ng, L = 12, 30.0
+    g = orc.Grid(Ng, L)
+    g.f[:] = 0
+    g.f[:, :, 0::2] = 1.0
+    K, N, Pl = g.Pl_bins((0, 0, 0), 2 * np.pi / L)
+    ok = N > 0
+    assert np.allclose(Pl[ok], [1.0, -2.5, 3.375, -4.0625], rtol=1e-14)
+
+
+def test_gal_rsd_closed_form(orc):
+    L = 100.0
+    rng = np.random.default_rng(8)
+    x, y, z = rng.random((3, 50)) * L
+    v = rng.standard_normal((3, 50)) * 200
+    c = orc.Catalog(x, y, z, L ** 3, *v)
+    los = np.array([0.0, 3.0, 4.0])
+    aH = 70.0
+    r = orc.load().oracle_gal_rsd(c.ptr, los.ctypes.data_as(orc.oracle.dptr), aH)
+    lh = np.array([0, 0.6, 0.8])
+    assert np.allclose(los, lh, rtol=0, atol=1e-16)
+    s = (lh @ v) / aH
+    for a, (name, p0) in enumerate((("x", x), ("y", y), ("z", z))):
+        got = np.ctypeslib.as_array(getattr(r.contents, name), shape=(50,))
+        assert np.allclose(got, p0 + s * lh[a], rtol=0, atol=1e-12)
+    orc.load().oracle_gal_free(r)
