@@ -1,0 +1,322 @@
+"""Benchmark: Mparticles/s to P_l(k) on one MI355X.
+
+Workload (default): lognormal mock of 134,217,728 float32 particles in a
+periodic L = 2000 Mpc/h box (power-law P(k) = 2e4 (k/0.1)^-1.5 on a 256^3
+seed grid, Zel'dovich RSD along z with f = 0.76), Ng = 512, PCS paint with
+interlacing, multipoles l = 0,2,4 (6 computed too) and 10 |mu| wedges,
+dK = kF (NK = 444 bins, 256 below k_Nyquist).
+
+A step is one catalog through the mixed-precision pipeline (fixed-point
+paint of both grids, two FP32 r2c FFTs, fused deconvolve/interlace/bin,
+bin sums to host), positions already in HBM, replayed as a hipGraph.
+Also measured in the same run:
+  * per-stage times (HIP events on the pipeline's stream) and the HBM
+    roofline of the dominant kernel;
+  * the plain FP64 GPU form (drop-in API, device-resident, FP64 atomics
+    and FP64 FFT) on the same catalog, and the P_l error of the fast path
+    against it;
+  * the FP64 CPU oracle (restatement of the reference) on a bounded sample.
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("Mparticles/s to P_l(k) at Ng=512, batched mixed vs plain FP64; "
+          "HBM GB/s vs peak")
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--np", type=int, default=1 << 27)
+    ap.add_argument("--ng", type=int, default=512)
+    ap.add_argument("--L", type=float, default=2000.0)
+    ap.add_argument("--nmu", type=int, default=10)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 21)
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the timed graph loop (for rocprofv3 runs)")
+    return ap.parse_args()
+
+
+def stage_bytes(name, Np, Ng):
+    """Algorithmic (compulsory) bytes of one stage of one catalog."""
+    n3 = Ng ** 3
+    nc = Ng * Ng * (Ng + 2)
+    table = {
+        "paint_count": 12 * Np,                 # read float32 xyz
+        "paint_scan": 0,
+        "paint_scatter": 12 * Np + 8 * Np,      # read xyz, write packed 8 B
+        "paint_tiles": 8 * Np + 2 * 16 * n3,    # read packed, RMW both int64 grids once
+        "fft_rows_A": 8 * n3 + 4 * nc,          # read int64 grid, write FP32 half spectrum
+        "fft_rows_B": 8 * n3 + 4 * nc,
+        "fft_cols_j_A": 8 * nc, "fft_cols_i_A": 8 * nc,   # read + write FP32
+        "fft_cols_j_B": 8 * nc, "fft_cols_i_B": 8 * nc,
+        "fused_bin": 2 * 4 * nc,                # read both spectra
+    }
+    return table.get(name, 0)
+
+
+def b_alg(Np, Ng, G=2, s=4, bp=12):
+    """Compulsory bytes of the whole pipeline: b_p Np + G s (Ng^3 + 3 Nc)."""
+    return bp * Np + G * s * (Ng ** 3 + 3 * Ng * Ng * (Ng + 2))
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fp:
+        return json.load(fp)
+
+
+def cpu_baseline(x, y, z, V, Ng, L, nmu, nsample):
+    """FP64 oracle (single thread) on the full grid with a particle subsample;
+    the paint time is scaled linearly to the full particle count."""
+    import numpy as np
+    import oracle
+    Np = len(x)
+    n = min(nsample, Np)
+    xs, ys, zs = (a[:n].astype(np.float64) for a in (x, y, z))
+    cat = oracle.Catalog(xs, ys, zs, V)
+    g = oracle.Grid(Ng, L)
+    t = {}
+    t0 = time.perf_counter()
+    g.p2g(cat, (0.5, 0.5, 0.5))
+    t["p2g_dual"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g.x2k(1)
+    t["x2k_dual"] = time.perf_counter() - t0
+    d = oracle.oracle.load().oracle_fft_exportf(g.ptr)
+    t0 = time.perf_counter()
+    g.p2g(cat, (0, 0, 0))
+    t["p2g"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g.x2k(0)
+    t["x2k"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fd = np.ctypeslib.as_array(d, shape=g.f.shape)
+    g.f[:] = 0.5 * (fd + g.f)
+    g.deconv()
+    g.P(cat)
+    g.Pl_bins((0, 0, 1), 2 * np.pi / L)
+    g.Pmu_bins((0, 0, 1), 2 * np.pi / L, nmu)
+    t["avg_deconv_P_Pl_Pmu"] = time.perf_counter() - t0
+    import ctypes
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+    libc.free(ctypes.cast(d, ctypes.c_void_p))
+    g.free()
+    cat.free()
+    # the paint loops scale with Np; its grid sweeps (zero, normalise) do not,
+    # so scaling the whole p2g time overestimates the full-size cost slightly
+    # less than linearly would: report the linear extrapolation (conservative
+    # for the baseline's speed)
+    paint = t["p2g_dual"] + t["p2g"]
+    rest = t["x2k_dual"] + t["x2k"] + t["avg_deconv_P_Pl_Pmu"]
+    t_full = paint * (Np / n) + rest
+    return {
+        "value": round(Np / t_full / 1e6, 4),
+        "unit": "Mparticles/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"FP64 oracle, 1 thread, Ng={Ng} full grid pipeline (2 paints, 2 FFTs, "
+                   f"deconv, P, Pl, Pmu) on the first {n} of {Np} particles; paint time "
+                   f"scaled by {Np / n:.0f}x to the full catalog; sample wall "
+                   f"{paint + rest:.1f}s"),
+        "stage_s": {k: round(v, 3) for k, v in t.items()},
+        "est_full_s": round(t_full, 2),
+        "cpu": _cpu_model(),
+    }
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fp:
+            for line in fp:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import fftgal_amd as fa
+    from fftgal_amd import mockgen
+
+    fa.load()
+    Np, Ng, L, nmu = args.np, args.ng, args.L, args.nmu
+    V = L ** 3
+    dK = 2 * np.pi / L
+    t0 = time.perf_counter()
+    (x, y, z), sigma2 = mockgen.lognormal(Np, L, seed=0, Ngs=256)
+    t_mock = time.perf_counter() - t0
+    print(f"# mock: {Np} particles in {t_mock:.1f}s (sigma_G^2 = {sigma2:.2f})",
+          file=sys.stderr, flush=True)
+    dev = [torch.from_numpy(a).to("cuda:0") for a in (x, y, z)]
+    torch.cuda.synchronize()
+    ptr = [d.data_ptr() for d in dev]
+
+    t0 = time.perf_counter()
+    plan = fa.PowerPlan(Ng, L, interlace=True, dK=dK, los=(0, 0, 1), Nmu=nmu, Np_max=Np)
+    t_plan = time.perf_counter() - t0
+    stream = torch.cuda.ExternalStream(plan.stream())
+
+    for _ in range(args.warmup):
+        plan.enqueue(*ptr, Np, V)
+    plan.sync()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        plan.enqueue(*ptr, Np, V)
+    ev1.record(stream)
+    plan.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = ev0.elapsed_time(ev1) / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    fast = plan.results()
+    value = Np / (ms_per_step * 1e-3) / 1e6
+    print(f"# fast: {ms_per_step:.3f} ms/catalog (events {ev_ms:.3f}) -> {value:.1f} Mparticles/s",
+          file=sys.stderr, flush=True)
+    if args.profile_only:
+        # calibration for HBM counters: a plain device copy of known size
+        nbytes = 1 << 30
+        a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(2):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"ms_per_step": ms_per_step, "value": value,
+                          "copy_calibration_bytes": {"read": nbytes, "write": nbytes}}))
+        return
+
+    # per-stage times, HIP events on the pipeline's stream
+    nprof = max(3, min(args.steps, 5))
+    acc = {}
+    for _ in range(nprof):
+        st = plan.profile(*ptr, Np, V)
+        for k, v in st.items():
+            acc[k] = acc.get(k, 0.0) + v / nprof
+    stages = {}
+    for k, ms in acc.items():
+        b = stage_bytes(k, Np, Ng)
+        stages[k] = {"ms": round(ms, 4), "alg_bytes": b,
+                     "GBps": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 and b else None}
+    dom = max((k for k in acc if stage_bytes(k, Np, Ng)), key=lambda k: acc[k])
+    dom_ms = acc[dom]
+    ach = stage_bytes(dom, Np, Ng) / (dom_ms * 1e-3) / 1e9
+    traffic = None
+    tr = load_traffic()
+    if tr and dom in tr.get("stages", {}):
+        traffic = tr["stages"][dom].get("bytes")
+    roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                "kernel_ms": round(dom_ms, 4)}
+    B = b_alg(Np, Ng)
+    pipe = {"B_alg_bytes": B, "GBps": round(B / (ms_per_step * 1e-3) / 1e9, 1),
+            "frac_of_peak": round(B / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "stage_sum_ms": round(sum(acc.values()), 4)}
+    plan.close()
+
+    # plain FP64 GPU form on the same catalog
+    plain = None
+    err = None
+    if not args.no_plain:
+        dd = [d.double() for d in dev]
+        torch.cuda.synchronize()
+        g = fa.Grid(Ng, L, resident=True)
+        lib = g.lib
+        times = []
+        for _ in range(args.plain_steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            lib.fft_p2k_device(g.ptr, dd[0].data_ptr(), dd[1].data_ptr(), dd[2].data_ptr(), Np, V)
+            lib.P_params(g.ptr, Np, V, 0)
+            Kp, Npl, Pp = g.Pl_bins((0, 0, 1), dK)
+            Km, Nm, Pm = g.Pmu_bins((0, 0, 1), dK, nmu)
+            times.append(time.perf_counter() - t0)
+        g.free()
+        del dd
+        tp = min(times)
+        plain = {"value": round(Np / tp / 1e6, 2), "unit": "Mparticles/s",
+                 "ms_per_catalog": round(tp * 1e3, 2),
+                 "form": "drop-in API, device resident: FP64 atomic paint x2, FP64 FFT x2, "
+                         "phase, average, deconv, P, Pl, Pmu kernels"}
+        sel = (Npl > 0) & (np.arange(len(Npl)) < Ng // 2)
+        err = {"counts_equal": bool(np.array_equal(Npl, fast["N"]) and
+                                    np.array_equal(Nm, fast["Nmu"])),
+               "k_range": "0 < k < k_Nyquist"}
+        pshot = V / Np
+        for l in range(3):
+            d = np.abs(fast["Pl"][sel, l] - Pp[sel, l])
+            err[f"P{2*l}_max_rel"] = float((d / np.abs(Pp[sel, l])).max())
+            err[f"P{2*l}_max_rel_to_P0"] = float((d / (np.abs(Pp[sel, 0]) + pshot)).max())
+        wm = Nm[: Ng // 2] > 0
+        dw = np.abs(fast["Pmu"][: Ng // 2][wm] - Pm[: Ng // 2][wm])
+        err["wedges_max_rel"] = float((dw / np.abs(Pm[: Ng // 2][wm])).max())
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(x, y, z, V, Ng, L, nmu, args.cpu_sample)
+
+    name = torch.cuda.get_device_name(0)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mparticles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": ("synthetic lognormal mock (numpy, seed 0): P(k)=2e4 (k/0.1)^-1.5 on a 256^3 "
+                 "seed grid, Poisson-sampled, Zel'dovich RSD along z f=0.76, random order"),
+        "config": {"workload": (f"lognormal mock, {Np} float32 particles, L={L:g} Mpc/h, "
+                                f"Ng={Ng}, PCS paint + interlacing, l=0,2,4(,6) and "
+                                f"{nmu} |mu|-wedges, dK=kF"),
+                   "Np": Np, "Ng": Ng, "L": L, "interlace": True, "Nmu": nmu,
+                   "precision": "int64 fixed-point paint, FP32 FFT (FP64 twiddles), "
+                                "FP64 k-space math and bin sums",
+                   "parallelism": "1 GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_fp64_gpu": plain,
+        "speedup_vs_plain_fp64": round(value / plain["value"], 2) if plain else None,
+        "error_vs_plain_fp64": err,
+        "pipeline": pipe,
+        "stages": stages,
+        "event_ms_per_step": round(ev_ms, 4),
+        "setup_s": {"mock": round(t_mock, 1), "plan": round(t_plan, 2)},
+        "device": name,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

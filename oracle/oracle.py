@@ -60,6 +60,8 @@ def load():
         "oracle_fft_offset_phase": (None, [G, C.c_int]),
         "oracle_fft_deconv": (None, [G]),
         "oracle_fft_p2k": (None, [G, P]),
+        "oracle_fft_exportf": (dptr, [G]),
+        "oracle_fft_importf": (None, [G, dptr]),
         "oracle_P": (None, [G, P]),
         "oracle_NK": (C.c_int, [C.c_int, C.c_double, C.c_double]),
         "oracle_Pl_bins": (C.c_int, [G, dptr, C.c_double, dptr, lptr, dptr]),

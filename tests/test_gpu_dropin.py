@@ -61,7 +61,7 @@ def test_p2g_vs_oracle_with_randoms(torch_cuda, lib, orc):
     import fftgal_amd as fa
     rng = np.random.default_rng(7)
     Ng, L = 16, 100.0
-    x, y, z = (rng.random(3000) * 2.3 - 0.6) * L   # outside [0, L) too: wrap
+    x, y, z = (rng.random((3, 3000)) * 2.3 - 0.6) * L   # outside [0, L) too: wrap
     xr, yr, zr = rng.random((3, 6000)) * L
     go, gg = orc.Grid(Ng, L), fa.Grid(Ng, L)
     ro = orc.Catalog(xr, yr, zr, L ** 3)
@@ -195,3 +195,14 @@ def test_resident_mode_and_export(torch_cuda, lib, orc):
     libc = ctypes.CDLL(None)
     libc.free.argtypes = [ctypes.c_void_p]
     libc.free(ctypes.cast(exp, ctypes.c_void_p))
+
+
+def test_c_octet_program(torch_cuda, lib):
+    """The octet test as a C program compiled against include/ and linked
+    with libfftgal_amd.so (tests/c, built by __graft_entry__.build())."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "c", "test_octet")
+    assert os.path.exists(exe), "tests/c/test_octet not built"
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "test PASSED" in out.stdout
