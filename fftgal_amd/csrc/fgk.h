@@ -93,7 +93,10 @@ int  fgk_paint_fixed(fgk_paintplan *pp, const float *x, const float *y,
                      int64_t *gridA, int64_t *gridB, fgk_stream_t s);
 double fgk_paint_fixed_scale(const fgk_paintplan *pp);  /* 2^-S */
 int  fgk_paint_plan_tile(const fgk_paintplan *pp);
-/* the paint stages separately (profiling / tests) */
+/* the paint stages separately (profiling / tests): fgk_paint_nstages()
+ * stages, named by fgk_paint_stage_name() */
+int  fgk_paint_nstages(void);
+const char *fgk_paint_stage_name(int i);
 int  fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x,
                      const float *y, const float *z, long Np, double Hinv,
                      int64_t *gridA, int64_t *gridB, fgk_stream_t s);

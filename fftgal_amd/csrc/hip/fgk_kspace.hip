@@ -107,7 +107,7 @@ extern "C" int fgk_P_f64(double *f, int Ng, double Vinv, double Pshot, fgk_strea
 #define BI 4
 #define BJ 4
 #define BK 64
-#define BIN_LDS_BYTES (48 * 1024)
+#define BIN_LDS_BYTES (12 * 1024)
 
 extern "C" size_t fgk_bins_doubles(const fgk_binspec *b) {
     size_t W = b->Nmu > 0 ? (size_t)b->Nmu + 1 : 0;

@@ -8,24 +8,28 @@
 //
 // 2. The deterministic fixed-point tile paint of the fast path, for float32
 //    positions and the interlaced pair of grids (offset 0 and offset 1/2,
-//    src/fft.c:257-276):
-//      count   -- each particle's tile (T^3 cells, by the plain grid's PCS
-//                 base cell); a per-workgroup tile histogram in LDS whose
-//                 LDS atomic return value is the particle's rank in
-//                 (workgroup, tile)
-//      scan    -- column sums and prefix sums over the (workgroup x tile)
-//                 histogram give every (workgroup, tile) its output range,
-//                 and per tile the number of paint work items
-//      scatter -- each particle written once, as a packed 64-bit tile-local
-//                 position (3 x 21-bit fixed point), to its sorted slot
-//      paint   -- one workgroup per work item (<= cap particles of one tile)
-//                 accumulates the 4x4x4 stencils of both grids into LDS
-//                 footprints ((T+3)^3 plain, (T+4)^3 dual) as int32 fixed
-//                 point, then flushes the non-zero footprint cells into the
-//                 int64 grids with global integer atomics.
+//    src/fft.c:257-276).  Particles are first grouped by paint tile (T^3
+//    cells, by the plain grid's PCS base cell) with a two-level MSD
+//    counting sort -- coarse bucket (<= 256 per grid) then tile within the
+//    bucket (<= 128) -- whose scatters stage each 4096-particle chunk in LDS
+//    ordered by digit, so that global writes are contiguous runs.  Ranks
+//    come from a wave-level match of equal digits (one LDS atomic per
+//    distinct digit per wave), so heavily clustered catalogs, where most
+//    particles share a few buckets, do not serialise on one LDS counter.
+//    Positions travel as 64-bit words: 7-bit tile-within-bucket digit and a
+//    19-bit tile-local fixed-point coordinate per axis (2^-15 cell for
+//    T = 16).  The paint then runs one workgroup per work item (<= cap
+//    particles of one tile): the 4x4x4 stencils of both grids go into LDS
+//    footprints ((T+3)^3 plain, (T+4)^3 dual) as int32 fixed point, each
+//    lane walking the 16 stencil rows in a lane-dependent order so that
+//    lanes holding particles of the same dense cell hit different
+//    addresses; non-zero footprint cells are flushed into the int64 grids
+//    with global integer atomics.
 //    Integer addition is associative, so the grids are bit-identical run to
-//    run whatever the scheduling.  Weights are rounded to 2^-S per particle
-//    and cell, S chosen so that cap particles cannot overflow int32.
+//    run whatever the scheduling.  Each weight is rounded once to 2^-S, S
+//    chosen so that cap particles cannot overflow int32.
+//    Grids with more tiles than the two levels can address fall back to a
+//    single-level sort whose ranks come from global atomics.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdlib.h>
@@ -133,259 +137,484 @@ extern "C" int fgk_widen_f32_f64(double *dst, const float *src, long n, fgk_stre
 
 // ------------------------------------------------------ fixed-point tiles
 
-#define PAINT_THREADS 256
-#define COUNT_THREADS 512
+#define PAINT_THREADS 1024
+#define SORT_THREADS 256
+#define SORT_CHUNK 4096                  // particles per sort workgroup
+#define SORT_PER_THREAD (SORT_CHUNK / SORT_THREADS)
 #define SCAN_THREADS 1024
-#define LDS_HIST_MAX_TILES 32768   // packed uint16 pairs: 64 KB of LDS
+#define RB 32                            // sort chunks per scan block
+#define MAX_D1 256                       // coarse buckets
+#define MAX_D2 128                       // tiles per coarse bucket
+#define POS_BITS 19
+#define POS_MASK ((1u << POS_BITS) - 1u)
+#define FINE_SHIFT 57
+
+struct KeyGeo {
+    double Hinv;
+    int Ng, T, nt, F;
+    int cg0, cg1, cg2;   // tiles per coarse bucket per axis (powers of two)
+    int nc0, nc1, nc2;   // coarse buckets per axis
+    int lg0, lg1, lg2;   // log2 of cg0, cg1, cg2
+    uint64_t Tmul;       // ceil(2^32 / T): i / T == (i * Tmul) >> 32 for 0 <= i < 2^24
+};
 
 struct fgk_paintplan {
-    int Ng;
-    int T;           // tile side in cells
-    int nt;          // tiles per dimension
-    int interlace;
-    int F;           // fraction bits of the packed local position
-    int S;           // fixed-point bits of the painted weights
-    int cap;         // particles per paint work item
-    int lds_hist;    // 1: per-workgroup LDS histograms; 0: global atomics
-    long ntiles;
+    int Ng, T, nt, interlace, F, S, cap;
+    int two_level;
+    KeyGeo g;            // Hinv filled per call
+    int nb1, nb2, bits1, bits2;
+    long nslots;         // nb1 * nb2 tile slots
     long Np_max;
-    long chunk;      // particles per count/scatter workgroup
-    long nwg;        // count/scatter workgroups at Np_max
-    uint16_t *H;     // [nwg][ntiles] histogram (lds_hist)
-    uint32_t *O;     // [nwg][ntiles] output offsets (lds_hist)
-    uint32_t *tot;   // [ntiles] per tile counts
-    uint32_t *base;  // [ntiles+1] exclusive prefix of tot
-    uint32_t *items; // [ntiles+1] exclusive prefix of work items per tile
-    void *rank;      // [Np_max] uint16 (lds_hist) or uint32
-    uint64_t *sorted;// [Np_max] packed tile-local positions
+    long nw1max, nw2max; // sort chunks (multiples of RB)
+    uint32_t *H1, *O1, *S1, *B1, *T1;
+    uint32_t *H2, *O2, *S2, *B2;
+    uint32_t *seg1;      // {0, blocks} for the single level-1 segment
+    uint32_t *base1;     // [nb1+1]
+    uint32_t *chunk0;    // [nb1+1] first level-2 chunk of each bucket (multiple of RB)
+    uint32_t *blk0;      // [nb1+1] chunk0 / RB
+    uint32_t *T2;        // [nslots] particles per tile
+    uint32_t *tilebase;  // [nslots+1]
+    uint32_t *items;     // [nslots+1] paint work items, exclusive prefix
+    uint32_t *rank32;    // [Np_max] single-level path
+    uint64_t *sorted1, *sorted2;
 };
 
-struct TileGeo {
-    int Ng, T, nt, F;
-    double Hinv;
-};
-
-// plain-grid PCS base cell (wrapped) and the particle's tile, plus the
-// packed tile-local position li + d in 21-bit fixed point per axis
-__device__ __forceinline__ void tile_axis(float xf, const TileGeo &g, int &tcoord,
-                                          uint32_t &q) {
-    double u = (double)xf * g.Hinv;
-    double fl = floor(u);
-    double d = u - fl;
-    long gi = (long)fl;
-    int gw = (int)(((gi % g.Ng) + g.Ng) % g.Ng);
-    tcoord = gw / g.T;
-    int li = gw - tcoord * g.T;
-    const double sc = (double)(1u << g.F);
-    long qq = (long)li * (1l << g.F) + (long)rint(d * sc);
-    long qmax = (long)g.T * (1l << g.F) - 1;
-    if (qq > qmax) qq = qmax;
-    q = (uint32_t)qq;
+// wrapped plain-grid PCS base cell of one coordinate, its tile and the
+// tile-local position li + d as POS_BITS-bit fixed point (F fraction bits).
+// The cell index is taken in FP64 as the reference does (src/fft.c:68-76).
+__device__ __forceinline__ void axis_key(float xf, const KeyGeo &g, int &tc, uint32_t &q) {
+    const double u = (double)xf * g.Hinv;
+    const double fl = floor(u);
+    const double d = u - fl;
+    int gi = (int)fl;
+    if ((unsigned)gi >= (unsigned)g.Ng) {
+        gi %= g.Ng;
+        if (gi < 0) gi += g.Ng;
+    }
+    tc = (int)(((uint64_t)(uint32_t)gi * g.Tmul) >> 32);
+    const int li = gi - tc * g.T;
+    int qq = (li << g.F) + (int)rint(d * (double)(1 << g.F));
+    const int qmax = (g.T << g.F) - 1;
+    q = (uint32_t)(qq < qmax ? qq : qmax);
 }
 
-__device__ __forceinline__ void particle_tile(const float *x, const float *y, const float *z,
-                                              long p, const TileGeo &g, int &t, uint64_t &pk) {
-    int ti, tj, tk;
-    uint32_t qi, qj, qk;
-    tile_axis(x[p], g, ti, qi);
-    tile_axis(y[p], g, tj, qj);
-    tile_axis(z[p], g, tk, qk);
-    t = (ti * g.nt + tj) * g.nt + tk;
-    pk = ((uint64_t)qi << 42) | ((uint64_t)qj << 21) | (uint64_t)qk;
+__device__ __forceinline__ void particle_key(const float *x, const float *y, const float *z,
+                                             long p, const KeyGeo &g, uint32_t &coarse,
+                                             uint64_t &pk) {
+    int tx, ty, tz;
+    uint32_t qx, qy, qz;
+    axis_key(x[p], g, tx, qx);
+    axis_key(y[p], g, ty, qy);
+    axis_key(z[p], g, tz, qz);
+    const int cx = tx >> g.lg0, cy = ty >> g.lg1, cz = tz >> g.lg2;
+    coarse = (uint32_t)((cx * g.nc1 + cy) * g.nc2 + cz);
+    const uint32_t fine = (uint32_t)((((tx & (g.cg0 - 1)) << g.lg1 | (ty & (g.cg1 - 1))) << g.lg2) |
+                                     (tz & (g.cg2 - 1)));
+    pk = ((uint64_t)fine << FINE_SHIFT) | ((uint64_t)qx << (2 * POS_BITS)) |
+         ((uint64_t)qy << POS_BITS) | (uint64_t)qz;
 }
 
-__global__ void __launch_bounds__(COUNT_THREADS)
-k_count_lds(const float *__restrict__ x, const float *__restrict__ y,
-            const float *__restrict__ z, long Np, TileGeo g, int ntiles, long chunk,
-            uint16_t *__restrict__ H, uint16_t *__restrict__ rank) {
-    extern __shared__ uint32_t hist[];
-    const int nw = (ntiles + 1) >> 1;
-    for (int i = threadIdx.x; i < nw; i += blockDim.x) hist[i] = 0;
+// Rank of this lane among the valid lanes of its wave holding the same
+// digit, added to the workgroup histogram with one LDS atomic per distinct
+// digit per wave.  All lanes of the wave must call it together.
+__device__ __forceinline__ uint32_t wave_rank(uint32_t d, bool valid, int nbits,
+                                              uint32_t *hist) {
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < nbits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(valid && bit);
+        peers &= bit ? bal : ~bal;
+    }
+    const int lane = __lane_id();
+    const uint32_t below = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+    const int leader = __ffsll((unsigned long long)peers) - 1;
+    uint32_t base = 0;
+    if (valid && lane == leader) base = atomicAdd(&hist[d], (uint32_t)__popcll(peers));
+    base = __shfl(base, leader < 0 ? 0 : leader);
+    return base + below;
+}
+
+// exclusive scan of hist[0..n) (n <= 256) into off[], by the first wave
+__device__ __forceinline__ void small_exclusive_scan(const uint32_t *hist, uint32_t *off, int n) {
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        uint32_t v[4], s = 0;
+        for (int q = 0; q < 4; ++q) {
+            const int i = lane * 4 + q;
+            v[q] = i < n ? hist[i] : 0u;
+            s += v[q];
+        }
+        uint32_t incl = s;
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        uint32_t run = incl - s;
+        for (int q = 0; q < 4; ++q) {
+            const int i = lane * 4 + q;
+            if (i < n) off[i] = run;
+            run += v[q];
+        }
+    }
+}
+
+// level-2 chunk j -> its coarse bucket and particle range in sorted1
+__device__ __forceinline__ bool chunk_range2(long j, const uint32_t *chunk0, const uint32_t *base1,
+                                             const uint32_t *T1, int nb1, long &start, int &n) {
+    if (j >= (long)chunk0[nb1]) return false;
+    int lo = 0, hi = nb1 - 1;              // last bucket with chunk0[b] <= j
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((long)chunk0[mid] <= j) lo = mid; else hi = mid - 1;
+    }
+    const long jl = j - chunk0[lo];
+    start = (long)base1[lo] + jl * SORT_CHUNK;
+    const long end = (long)base1[lo] + T1[lo];
+    const long m = end - start;
+    n = m <= 0 ? 0 : (m < SORT_CHUNK ? (int)m : SORT_CHUNK);
+    return true;
+}
+
+// LEVEL: 1 reads float positions (digit = coarse bucket); 2 reads packed
+// words of sorted1 (digit = tile within bucket).  COUNT: histogram only.
+template <int LEVEL, int COUNT>
+__global__ void __launch_bounds__(SORT_THREADS)
+k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+       long Np, KeyGeo g, const uint64_t *__restrict__ in, const uint32_t *__restrict__ chunk0,
+       const uint32_t *__restrict__ base1, const uint32_t *__restrict__ T1, int nb1, int nd,
+       int nbits, uint32_t *__restrict__ H, const uint32_t *__restrict__ O,
+       uint64_t *__restrict__ out) {
+    __shared__ uint32_t hist[MAX_D1];
+    __shared__ uint32_t off[MAX_D1];
+    __shared__ uint64_t stage[COUNT ? 1 : SORT_CHUNK];
+    __shared__ uint8_t sdig[COUNT ? 1 : SORT_CHUNK];
+    __shared__ long s_start;
+    __shared__ int s_n;
+    const long j = blockIdx.x;
+    if (threadIdx.x == 0) {
+        long st = 0;
+        int n = 0;
+        if (LEVEL == 1) {
+            st = j * SORT_CHUNK;
+            const long m = Np - st;
+            n = m <= 0 ? 0 : (m < SORT_CHUNK ? (int)m : SORT_CHUNK);
+        } else if (!chunk_range2(j, chunk0, base1, T1, nb1, st, n)) {
+            n = 0;
+        }
+        s_start = st;
+        s_n = n;
+    }
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) hist[i] = 0;
     __syncthreads();
-    const long p0 = blockIdx.x * chunk;
-    const long p1 = p0 + chunk < Np ? p0 + chunk : Np;
-    for (long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-        int t;
-        uint64_t pk;
-        particle_tile(x, y, z, p, g, t, pk);
-        const uint32_t sh = (t & 1) << 4;
-        uint32_t old = atomicAdd(&hist[t >> 1], 1u << sh);
-        rank[p] = (uint16_t)((old >> sh) & 0xffffu);
+    const long st = s_start;
+    const int n = s_n;
+    uint32_t dk[SORT_PER_THREAD], rk[SORT_PER_THREAD];
+    uint64_t vk[SORT_PER_THREAD];
+#pragma unroll
+    for (int k = 0; k < SORT_PER_THREAD; ++k) {
+        const int i = k * SORT_THREADS + threadIdx.x;
+        const bool valid = i < n;
+        uint32_t d = 0;
+        uint64_t pk = 0;
+        if (valid) {
+            if (LEVEL == 1) {
+                particle_key(x, y, z, st + i, g, d, pk);
+            } else {
+                pk = in[st + i];
+                d = (uint32_t)(pk >> FINE_SHIFT);
+            }
+        }
+        rk[k] = wave_rank(d, valid, nbits, hist);
+        dk[k] = d;
+        vk[k] = pk;
     }
     __syncthreads();
-    uint16_t *row = H + (long)blockIdx.x * ntiles;
-    for (int i = threadIdx.x; i < ntiles; i += blockDim.x)
-        row[i] = (uint16_t)((hist[i >> 1] >> ((i & 1) << 4)) & 0xffffu);
+    if (COUNT) {
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) H[j * nd + i] = hist[i];
+        return;
+    }
+    small_exclusive_scan(hist, off, nd);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SORT_PER_THREAD; ++k) {
+        const int i = k * SORT_THREADS + threadIdx.x;
+        if (i < n) {
+            const uint32_t pos = off[dk[k]] + rk[k];
+            stage[pos] = vk[k];
+            sdig[pos] = (uint8_t)dk[k];
+        }
+    }
+    __syncthreads();
+    const uint32_t *orow = O + j * nd;
+    for (int pos = threadIdx.x; pos < n; pos += blockDim.x) {
+        const uint32_t d = sdig[pos];
+        out[orow[d] + (pos - off[d])] = stage[pos];
+    }
 }
 
-__global__ void k_count_global(const float *__restrict__ x, const float *__restrict__ y,
-                               const float *__restrict__ z, long Np, TileGeo g,
-                               uint32_t *__restrict__ tot, uint32_t *__restrict__ rank) {
-    long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (p >= Np) return;
-    int t;
-    uint64_t pk;
-    particle_tile(x, y, z, p, g, t, pk);
-    rank[p] = atomicAdd(&tot[t], 1u);
-}
-
-// tot[t] = sum over workgroups of H[w][t]
-__global__ void k_colsum(const uint16_t *__restrict__ H, long nwg, int ntiles, uint32_t *tot) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
+// S[b][d] = sum of H[j][d] over the RB chunks j of block b
+__global__ void k_blocksum(const uint32_t *__restrict__ H, long nblk, int nd,
+                           uint32_t *__restrict__ S) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= nblk * nd) return;
+    const long b = t / nd;
+    const int d = (int)(t - b * nd);
     uint32_t s = 0;
-    for (long w = 0; w < nwg; ++w) s += H[w * ntiles + t];
-    tot[t] = s;
+    for (int r = 0; r < RB; ++r) s += H[(b * RB + r) * nd + d];
+    S[t] = s;
 }
 
-// O[w][t] = base[t] + sum_{w' < w} H[w'][t]
-__global__ void k_colscan(const uint16_t *__restrict__ H, long nwg, int ntiles,
-                          const uint32_t *__restrict__ base, uint32_t *__restrict__ O) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
-    uint32_t s = base[t];
-    for (long w = 0; w < nwg; ++w) {
-        O[w * ntiles + t] = s;
-        s += H[w * ntiles + t];
+// per segment s and digit d: exclusive prefix of S over the segment's
+// blocks [blk0[s], blk0[s+1]) into B, and the segment total into T
+__global__ void k_segscan(const uint32_t *__restrict__ S, const uint32_t *__restrict__ blk0,
+                          int nseg, int nd, uint32_t *__restrict__ B, uint32_t *__restrict__ T) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nseg * nd) return;
+    const int s = t / nd, d = t - s * nd;
+    uint32_t run = 0;
+    for (long b = blk0[s]; b < (long)blk0[s + 1]; ++b) {
+        B[b * nd + d] = run;
+        run += S[b * nd + d];
     }
+    T[t] = run;
 }
 
-// single-workgroup exclusive scans of the tile counts and of the number of
-// paint items per tile (ceil(count / cap)); writes n+1 entries each
+// single-workgroup exclusive scan of v[0..n) into base[0..n] and of an
+// auxiliary per-entry count into aux[0..n]:
+//   mode 1: aux = chunks of RB-aligned level-2 work, ceil(v/SORT_CHUNK) rounded up to RB;
+//           also aux_blk = aux / RB
+//   mode 2: aux = paint items, ceil(v/cap)
 __global__ void __launch_bounds__(SCAN_THREADS)
-k_scan_tiles(const uint32_t *__restrict__ tot, long n, int cap, uint32_t *__restrict__ base,
-             uint32_t *__restrict__ items) {
+k_digitscan(const uint32_t *__restrict__ v, long n, int mode, int cap,
+            uint32_t *__restrict__ base, uint32_t *__restrict__ aux,
+            uint32_t *__restrict__ aux_blk) {
     __shared__ uint32_t sa[SCAN_THREADS], sb[SCAN_THREADS];
     const long per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
-    const long b0 = threadIdx.x * per;
-    const long b1 = b0 + per < n ? b0 + per : n;
+    const long i0 = threadIdx.x * per;
+    const long i1 = i0 + per < n ? i0 + per : n;
     uint32_t ca = 0, cb = 0;
-    for (long i = b0; i < b1; ++i) {
-        uint32_t c = tot[i];
+    for (long i = i0; i < i1; ++i) {
+        const uint32_t c = v[i];
         ca += c;
-        cb += (c + cap - 1) / cap;
+        if (mode == 1) cb += ((c + SORT_CHUNK - 1) / SORT_CHUNK + RB - 1) / RB * RB;
+        else cb += (c + cap - 1) / cap;
     }
     sa[threadIdx.x] = ca;
     sb[threadIdx.x] = cb;
     __syncthreads();
-    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
-        uint32_t va = threadIdx.x >= off ? sa[threadIdx.x - off] : 0;
-        uint32_t vb = threadIdx.x >= off ? sb[threadIdx.x - off] : 0;
+    for (int o = 1; o < SCAN_THREADS; o <<= 1) {
+        const uint32_t va = threadIdx.x >= o ? sa[threadIdx.x - o] : 0;
+        const uint32_t vb = threadIdx.x >= o ? sb[threadIdx.x - o] : 0;
         __syncthreads();
         sa[threadIdx.x] += va;
         sb[threadIdx.x] += vb;
         __syncthreads();
     }
     uint32_t ra = sa[threadIdx.x] - ca, rb = sb[threadIdx.x] - cb;
-    for (long i = b0; i < b1; ++i) {
+    for (long i = i0; i < i1; ++i) {
         base[i] = ra;
-        items[i] = rb;
-        uint32_t c = tot[i];
+        aux[i] = rb;
+        if (aux_blk) aux_blk[i] = rb / RB;
+        const uint32_t c = v[i];
         ra += c;
-        rb += (c + cap - 1) / cap;
+        if (mode == 1) rb += ((c + SORT_CHUNK - 1) / SORT_CHUNK + RB - 1) / RB * RB;
+        else rb += (c + cap - 1) / cap;
     }
     if (threadIdx.x == SCAN_THREADS - 1) {
         base[n] = sa[SCAN_THREADS - 1];
-        items[n] = sb[SCAN_THREADS - 1];
+        aux[n] = sb[SCAN_THREADS - 1];
+        if (aux_blk) aux_blk[n] = sb[SCAN_THREADS - 1] / RB;
     }
 }
 
-template <int LDSH>
-__global__ void __launch_bounds__(COUNT_THREADS)
-k_scatter(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
-          long Np, TileGeo g, int ntiles, long chunk, const uint32_t *__restrict__ off,
-          const void *__restrict__ rank, uint64_t *__restrict__ sorted) {
-    const long p0 = blockIdx.x * chunk;
-    const long p1 = p0 + chunk < Np ? p0 + chunk : Np;
-    const uint32_t *offrow = LDSH ? off + (long)blockIdx.x * ntiles : off;
-    for (long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-        int t;
-        uint64_t pk;
-        particle_tile(x, y, z, p, g, t, pk);
-        uint32_t r = LDSH ? (uint32_t)((const uint16_t *)rank)[p] : ((const uint32_t *)rank)[p];
-        sorted[offrow[t] + r] = pk;
+// O[j][d] = base[s*nd + d] + B[b][d] + sum of H[j'][d] for the earlier
+// chunks j' of block b (s = segment of block b)
+__global__ void k_colscan(const uint32_t *__restrict__ H, const uint32_t *__restrict__ blk0,
+                          int nseg, long nblk, int nd, const uint32_t *__restrict__ base,
+                          const uint32_t *__restrict__ B, uint32_t *__restrict__ O) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= nblk * nd) return;
+    const long b = t / nd;
+    const int d = (int)(t - b * nd);
+    if (b >= (long)blk0[nseg]) return;
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((long)blk0[mid] <= b) lo = mid; else hi = mid - 1;
     }
+    uint32_t run = base[(long)lo * nd + d] + B[t];
+    for (int r = 0; r < RB; ++r) {
+        const long j = b * RB + r;
+        O[j * nd + d] = run;
+        run += H[j * nd + d];
+    }
+}
+
+// the single level-1 segment: blocks [0, nblk)
+__global__ void k_set_seg1(uint32_t *seg, uint32_t nblk) {
+    if (threadIdx.x == 0) {
+        seg[0] = 0;
+        seg[1] = nblk;
+    }
+}
+
+// single-level path (grids with more tiles than two levels address)
+__global__ void k_count_global(const float *__restrict__ x, const float *__restrict__ y,
+                               const float *__restrict__ z, long Np, KeyGeo g,
+                               uint32_t *__restrict__ tot, uint32_t *__restrict__ rank) {
+    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= Np) return;
+    uint32_t t;
+    uint64_t pk;
+    particle_key(x, y, z, p, g, t, pk);
+    rank[p] = atomicAdd(&tot[t], 1u);
+}
+
+__global__ void k_scatter_global(const float *__restrict__ x, const float *__restrict__ y,
+                                 const float *__restrict__ z, long Np, KeyGeo g,
+                                 const uint32_t *__restrict__ base,
+                                 const uint32_t *__restrict__ rank, uint64_t *__restrict__ out) {
+    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= Np) return;
+    uint32_t t;
+    uint64_t pk;
+    particle_key(x, y, z, p, g, t, pk);
+    out[base[t] + rank[p]] = pk;
 }
 
 __device__ __forceinline__ void pcs_w_f32(float d, float w[4]) {
-    float e = 1.0f - d;
+    const float e = 1.0f - d;
+    const float d2 = d * d;
     w[0] = e * e * e * (1.0f / 6.0f);
-    w[1] = ((3.0f * d - 6.0f) * d * d + 4.0f) * (1.0f / 6.0f);
+    w[1] = ((3.0f * d - 6.0f) * d2 + 4.0f) * (1.0f / 6.0f);
     w[2] = (((-3.0f * d + 3.0f) * d + 3.0f) * d + 1.0f) * (1.0f / 6.0f);
-    w[3] = d * d * d * (1.0f / 6.0f);
+    w[3] = d2 * d * (1.0f / 6.0f);
 }
 
-__device__ __forceinline__ int wrapi(int v, int Ng) {
-    v %= Ng;
-    return v < 0 ? v + Ng : v;
+__device__ __forceinline__ float sel4(const float w[4], int a) {
+    return (a & 2) ? ((a & 1) ? w[3] : w[2]) : ((a & 1) ? w[1] : w[0]);
 }
 
+// round-to-nearest-even of a*b (|a*b| < 2^22) with one FMA: adding
+// 1.5*2^23 puts the rounded integer in the low mantissa bits
+__device__ __forceinline__ int fixed_rne(float a, float b) {
+    return __float_as_int(__builtin_fmaf(a, b, 12582912.0f)) - 0x4B400000;
+}
+
+// Flush a (side^3) LDS footprint whose cell (0,0,0) sits at grid cell
+// (oi, oj, ok) into the int64 grid with one integer atomic per non-zero cell.
+// Each wave takes groups of 64/side whole rows along k; a lane's k offset is
+// fixed, so the periodic wrap needs no division (footprints extend at most
+// 2 cells below 0 and side cells past the grid's last index).
 __device__ void flush_footprint(const int *__restrict__ fp, int side, int oi, int oj, int ok,
                                 int Ng, unsigned long long *__restrict__ G) {
-    const int n = side * side * side;
-    for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
-        int v = fp[idx];
+    const int lane = __lane_id();
+    const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int rpg = 64 / side;                 // rows per wave step
+    const int ro = lane / side, fk = lane - ro * side;
+    if (ro >= rpg) return;
+    int gk = ok + fk;
+    gk = gk < 0 ? gk + Ng : (gk >= Ng ? gk - Ng : gk);
+    if (gk < 0) gk += Ng;        // grids smaller than a footprint wrap more than once
+    if (gk >= Ng) gk %= Ng;
+    const int nrows = side * side;
+    for (int r = wv * rpg + ro; r < nrows; r += nwv * rpg) {
+        const int v = fp[r * side + fk];
         if (!v) continue;
-        int fi = idx / (side * side);
-        int r = idx - fi * side * side;
-        int fj = r / side;
-        int fk = r - fj * side;
-        long gi = wrapi(oi + fi, Ng), gj = wrapi(oj + fj, Ng), gk = wrapi(ok + fk, Ng);
-        atomicAdd(G + (gi * Ng + gj) * Ng + gk, (unsigned long long)(long long)v);
+        const int fi = r / side, fj = r - fi * side;
+        int gi = oi + fi, gj = oj + fj;
+        gi = gi < 0 ? gi + Ng : (gi >= Ng ? gi - Ng : gi);
+        gj = gj < 0 ? gj + Ng : (gj >= Ng ? gj - Ng : gj);
+        if ((unsigned)gi >= (unsigned)Ng) gi = ((gi % Ng) + Ng) % Ng;
+        if ((unsigned)gj >= (unsigned)Ng) gj = ((gj % Ng) + Ng) % Ng;
+        atomicAdd(G + ((long)gi * Ng + gj) * Ng + gk, (unsigned long long)(long long)v);
     }
 }
 
-__global__ void __launch_bounds__(PAINT_THREADS)
-k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ base,
-              const uint32_t *__restrict__ items, int ntiles, int nt, int T, int Ng, int F,
-              float wscale, int cap, int interlace, unsigned long long *__restrict__ GA,
-              unsigned long long *__restrict__ GB) {
-    extern __shared__ int lds[];
-    const int it = blockIdx.x;
-    if (it >= (int)items[ntiles]) return;
-    int lo = 0, hi = ntiles - 1;          // last tile with items[t] <= it
+// paint work item -> tile slot and particle range [p0, p1)
+__device__ __forceinline__ bool item_range(long it, const uint32_t *__restrict__ tilebase,
+                                           const uint32_t *__restrict__ items, long nslots, int cap,
+                                           long &t, long &p0, long &p1) {
+    if (it >= (long)items[nslots]) return false;
+    long lo = 0, hi = nslots - 1;          // last slot with items[t] <= it
     while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if ((int)items[mid] <= it) lo = mid; else hi = mid - 1;
+        const long mid = (lo + hi + 1) >> 1;
+        if ((long)items[mid] <= it) lo = mid; else hi = mid - 1;
     }
-    const int t = lo;
-    const long p0 = (long)base[t] + (long)(it - (int)items[t]) * cap;
-    const long pe = base[t + 1];
-    const long p1 = p0 + cap < pe ? p0 + cap : pe;
+    t = lo;
+    p0 = (long)tilebase[t] + (it - (long)items[t]) * cap;
+    const long pe = tilebase[t + 1];
+    p1 = p0 + cap < pe ? p0 + cap : pe;
+    return true;
+}
 
-    const int sa = T + 3, sb = T + 4;
-    const int na = sa * sa * sa, nb = interlace ? sb * sb * sb : 0;
-    int *fa = lds, *fb = lds + na;
-    for (int i = threadIdx.x; i < na + nb; i += blockDim.x) lds[i] = 0;
+// One particle's 4x4x4 stencil into an LDS footprint of side SIDE (a
+// compile-time constant, so every atomic uses an immediate LDS offset).
+// Rows a are visited in the lane-dependent order (a + ph) & 3, so lanes
+// holding particles of one dense cell are spread over 4 different rows.
+template <int SIDE>
+__device__ __forceinline__ void stencil_lds(int *__restrict__ fp, int bi, int bj, int bk,
+                                            const float wx[4], const float wy[4],
+                                            const float wzs[4], int ph) {
+    int *base = fp + (bi * SIDE + bj) * SIDE + bk;
+#pragma unroll
+    for (int aa = 0; aa < 4; ++aa) {
+        const int a = (aa + ph) & 3;
+        const float wa = sel4(wx, a);
+        int *pa = base + a * SIDE * SIDE;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float wab = wa * wy[b];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) atomicAdd(pa + b * SIDE + c, fixed_rne(wab, wzs[c]));
+        }
+    }
+}
+
+// Paint of one work item (<= cap particles of one tile), one lane per
+// particle with coalesced loads; both grids' stencils go into the LDS
+// footprints as int32 fixed point, then the footprints are flushed.
+template <int T>
+__global__ void __launch_bounds__(PAINT_THREADS)
+k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ tilebase,
+              const uint32_t *__restrict__ items, long nslots, KeyGeo g, int nb2, float wscale,
+              int cap, int interlace, unsigned long long *__restrict__ GA,
+              unsigned long long *__restrict__ GB) {
+    constexpr int SA = T + 3, SB = T + 4;
+    constexpr int NA = SA * SA * SA, NB = SB * SB * SB;
+    extern __shared__ int lds[];
+    long t, p0, p1;
+    if (!item_range(blockIdx.x, tilebase, items, nslots, cap, t, p0, p1)) return;
+    const int nb = interlace ? NB : 0;
+    int *fa = lds, *fb = lds + NA;
+    for (int i = threadIdx.x; i < NA + nb; i += blockDim.x) lds[i] = 0;
     __syncthreads();
 
-    const uint32_t fmask = (1u << F) - 1u;
-    const float fsc = 1.0f / (float)(1u << F);
-    for (long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-        uint64_t pk = sorted[p];
-        uint32_t q[3] = {(uint32_t)(pk >> 42) & 0x1fffffu, (uint32_t)(pk >> 21) & 0x1fffffu,
-                         (uint32_t)pk & 0x1fffffu};
+    const int F = g.F;
+    const float fsc = 1.0f / (float)(1 << F);
+    const int ph = (threadIdx.x >> 4) & 3;
+    long p = p0 + threadIdx.x;
+    uint64_t nextpk = p < p1 ? sorted[p] : 0ull;
+    for (; p < p1; p += PAINT_THREADS) {
+        const uint64_t pk = nextpk;
+        if (p + PAINT_THREADS < p1) nextpk = sorted[p + PAINT_THREADS];
+        const uint32_t q[3] = {(uint32_t)(pk >> (2 * POS_BITS)) & POS_MASK,
+                               (uint32_t)(pk >> POS_BITS) & POS_MASK, (uint32_t)pk & POS_MASK};
         int li[3];
         float d[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             li[a] = (int)(q[a] >> F);
-            d[a] = (float)(q[a] & fmask) * fsc;
+            d[a] = (float)(q[a] & ((1u << F) - 1u)) * fsc;
         }
-        float wx[4], wy[4], wz[4];
+        float wx[4], wy[4], wz[4], wzs[4];
         pcs_w_f32(d[0], wx);
         pcs_w_f32(d[1], wy);
         pcs_w_f32(d[2], wz);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * wscale;
         // plain grid: nodes li-1..li+2, footprint origin (tile origin - 1)
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                int *row = fa + ((li[0] + a) * sa + (li[1] + b)) * sa + li[2];
-                const float wab = wx[a] * wy[b];
-#pragma unroll
-                for (int c = 0; c < 4; ++c) atomicAdd(row + c, (int)rintf(wab * wz[c] * wscale));
-            }
+        stencil_lds<SA>(fa, li[0], li[1], li[2], wx, wy, wzs, ph);
         if (interlace) {
             // dual grid (offset 1/2): base li or li-1, footprint origin (tile origin - 2)
             int ob[3];
@@ -398,21 +627,19 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
             pcs_w_f32(d[1], wy);
             pcs_w_f32(d[2], wz);
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    int *row = fb + ((ob[0] + a) * sb + (ob[1] + b)) * sb + ob[2];
-                    const float wab = wx[a] * wy[b];
-#pragma unroll
-                    for (int c = 0; c < 4; ++c)
-                        atomicAdd(row + c, (int)rintf(wab * wz[c] * wscale));
-                }
+            for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * wscale;
+            stencil_lds<SB>(fb, ob[0], ob[1], ob[2], wx, wy, wzs, ph);
         }
     }
     __syncthreads();
-    const int ti = t / (nt * nt), tj = (t / nt) % nt, tk = t % nt;
-    flush_footprint(fa, sa, ti * T - 1, tj * T - 1, tk * T - 1, Ng, GA);
-    if (interlace) flush_footprint(fb, sb, ti * T - 2, tj * T - 2, tk * T - 2, Ng, GB);
+    const long b1 = t / nb2, f = t - b1 * nb2;
+    const int cx = (int)(b1 / ((long)g.nc1 * g.nc2)), cy = (int)((b1 / g.nc2) % g.nc1),
+              cz = (int)(b1 % g.nc2);
+    const int fx = (int)(f / (g.cg1 * g.cg2)), fy = (int)((f / g.cg2) % g.cg1),
+              fz = (int)(f % g.cg2);
+    const int ti = cx * g.cg0 + fx, tj = cy * g.cg1 + fy, tk = cz * g.cg2 + fz;
+    flush_footprint(fa, SA, ti * T - 1, tj * T - 1, tk * T - 1, g.Ng, GA);
+    if (interlace) flush_footprint(fb, SB, ti * T - 2, tj * T - 2, tk * T - 2, g.Ng, GB);
 }
 
 // ------------------------------------------------------------------ host
@@ -423,46 +650,94 @@ static int paint_lds_bytes(int T, int interlace) {
     return (a + b) * 4;
 }
 
+static int ceil_log2(long v) {
+    int b = 0;
+    while ((1l << b) < v) ++b;
+    return b;
+}
+
+static long round_up(long v, long m) { return (v + m - 1) / m * m; }
+
 extern "C" int fgk_paint_plan_create(fgk_paintplan **out, int Ng, int tile, long Np_max,
                                      int interlace) {
     *out = NULL;
-    if (Ng < 2) { fgk_set_error("paint: Ng must be >= 2"); return -2; }
+    if (Ng < 2 || Ng > (1 << 20)) { fgk_set_error("paint: Ng must be in [2, 2^20]"); return -2; }
     if (Np_max < 1) Np_max = 1;
     if (Np_max >= (1l << 32) - 1) { fgk_set_error("paint: at most 2^32-2 particles"); return -2; }
+    // tile sides with a compiled paint kernel: 16, 12, 8, 4, 2, 1
     int T = tile > 0 ? tile : 16;
     if (T > Ng) T = Ng;
-    if (T > 32) T = 32;
-    while (T > 1 && paint_lds_bytes(T, interlace) > 64 * 1024) --T;
+    static const int kSides[] = {16, 12, 8, 4, 2, 1};
+    for (int i = 0; i < 6; ++i)
+        if (kSides[i] <= T) {
+            T = kSides[i];
+            break;
+        }
     fgk_paintplan *pp = (fgk_paintplan *)calloc(1, sizeof(fgk_paintplan));
     if (!pp) { fgk_set_error("out of host memory"); return -1; }
     pp->Ng = Ng;
     pp->T = T;
     pp->nt = (Ng + T - 1) / T;
     pp->interlace = interlace ? 1 : 0;
-    int ib = 0;
-    while ((1 << ib) < T) ++ib;
-    pp->F = 21 - ib;
+    pp->F = POS_BITS - ceil_log2(T);
     pp->cap = 8192;
-    // largest S with cap * (2/3)^3 * 2^S < 2^31
+    // largest S with cap * (2/3)^3 * 2^S < 2^31 (int32 LDS sums cannot
+    // overflow) and every rounded weight below 2^22 (single-FMA rounding)
     pp->S = 0;
-    while ((double)pp->cap * (8.0 / 27.0) * ldexp(1.0, pp->S + 1) < 2147483647.0) ++pp->S;
-    pp->ntiles = (long)pp->nt * pp->nt * pp->nt;
+    while ((double)pp->cap * (8.0 / 27.0) * ldexp(1.0, pp->S + 1) < 2147483647.0 &&
+           (8.0 / 27.0) * ldexp(1.0, pp->S + 1) < 4194304.0)
+        ++pp->S;
     pp->Np_max = Np_max;
-    pp->lds_hist = pp->ntiles <= LDS_HIST_MAX_TILES;
-    pp->chunk = 65535;
-    pp->nwg = (Np_max + pp->chunk - 1) / pp->chunk;
-    int rc = 0;
-    if (pp->lds_hist) {
-        rc |= fgk_malloc((void **)&pp->H, (size_t)pp->nwg * pp->ntiles * sizeof(uint16_t));
-        rc |= fgk_malloc((void **)&pp->O, (size_t)pp->nwg * pp->ntiles * sizeof(uint32_t));
-        rc |= fgk_malloc(&pp->rank, (size_t)Np_max * sizeof(uint16_t));
-    } else {
-        rc |= fgk_malloc(&pp->rank, (size_t)Np_max * sizeof(uint32_t));
+    // coarse buckets: group tiles until at most MAX_D1 buckets remain
+    int cg[3] = {1, 1, 1}, nc[3] = {pp->nt, pp->nt, pp->nt};
+    while ((long)nc[0] * nc[1] * nc[2] > MAX_D1) {
+        int ax = 2;
+        if (nc[1] > nc[ax]) ax = 1;
+        if (nc[0] > nc[ax]) ax = 0;
+        cg[ax] *= 2;
+        nc[ax] = (pp->nt + cg[ax] - 1) / cg[ax];
     }
-    rc |= fgk_malloc((void **)&pp->tot, (size_t)pp->ntiles * sizeof(uint32_t));
-    rc |= fgk_malloc((void **)&pp->base, (size_t)(pp->ntiles + 1) * sizeof(uint32_t));
-    rc |= fgk_malloc((void **)&pp->items, (size_t)(pp->ntiles + 1) * sizeof(uint32_t));
-    rc |= fgk_malloc((void **)&pp->sorted, (size_t)Np_max * sizeof(uint64_t));
+    pp->two_level = (long)cg[0] * cg[1] * cg[2] <= MAX_D2;
+    if (!pp->two_level) {   // single level: every tile is its own bucket
+        cg[0] = cg[1] = cg[2] = 1;
+        nc[0] = nc[1] = nc[2] = pp->nt;
+    }
+    KeyGeo g = {0.0,   Ng,    T,     pp->nt, pp->F,
+                cg[0], cg[1], cg[2], nc[0],  nc[1],
+                nc[2], ceil_log2(cg[0]), ceil_log2(cg[1]), ceil_log2(cg[2]),
+                ((1ull << 32) + (uint64_t)T - 1) / (uint64_t)T};
+    pp->g = g;
+    pp->nb1 = nc[0] * nc[1] * nc[2];
+    pp->nb2 = cg[0] * cg[1] * cg[2];
+    pp->bits1 = ceil_log2(pp->nb1);
+    pp->bits2 = ceil_log2(pp->nb2);
+    pp->nslots = (long)pp->nb1 * pp->nb2;
+    pp->nw1max = round_up((Np_max + SORT_CHUNK - 1) / SORT_CHUNK, RB);
+    pp->nw2max = pp->nw1max + (long)pp->nb1 * RB;
+    int rc = 0;
+    const size_t u = sizeof(uint32_t);
+    if (pp->two_level) {
+        rc |= fgk_malloc((void **)&pp->H1, (size_t)pp->nw1max * pp->nb1 * u);
+        rc |= fgk_malloc((void **)&pp->O1, (size_t)pp->nw1max * pp->nb1 * u);
+        rc |= fgk_malloc((void **)&pp->S1, (size_t)(pp->nw1max / RB) * pp->nb1 * u);
+        rc |= fgk_malloc((void **)&pp->B1, (size_t)(pp->nw1max / RB) * pp->nb1 * u);
+        rc |= fgk_malloc((void **)&pp->T1, (size_t)pp->nb1 * u);
+        rc |= fgk_malloc((void **)&pp->H2, (size_t)pp->nw2max * pp->nb2 * u);
+        rc |= fgk_malloc((void **)&pp->O2, (size_t)pp->nw2max * pp->nb2 * u);
+        rc |= fgk_malloc((void **)&pp->S2, (size_t)(pp->nw2max / RB) * pp->nb2 * u);
+        rc |= fgk_malloc((void **)&pp->B2, (size_t)(pp->nw2max / RB) * pp->nb2 * u);
+        rc |= fgk_malloc((void **)&pp->seg1, 2 * u);
+        rc |= fgk_malloc((void **)&pp->base1, (size_t)(pp->nb1 + 1) * u);
+        rc |= fgk_malloc((void **)&pp->chunk0, (size_t)(pp->nb1 + 1) * u);
+        rc |= fgk_malloc((void **)&pp->blk0, (size_t)(pp->nb1 + 1) * u);
+    } else {
+        rc |= fgk_malloc((void **)&pp->rank32, (size_t)Np_max * u);
+    }
+    rc |= fgk_malloc((void **)&pp->T2, (size_t)pp->nslots * u);
+    rc |= fgk_malloc((void **)&pp->tilebase, (size_t)(pp->nslots + 1) * u);
+    rc |= fgk_malloc((void **)&pp->items, (size_t)(pp->nslots + 1) * u);
+    rc |= fgk_malloc((void **)&pp->sorted1, (size_t)Np_max * sizeof(uint64_t));
+    rc |= fgk_malloc((void **)&pp->sorted2, (size_t)Np_max * sizeof(uint64_t));
     if (rc) {
         fgk_paint_plan_destroy(pp);
         return -1;
@@ -473,20 +748,50 @@ extern "C" int fgk_paint_plan_create(fgk_paintplan **out, int Ng, int tile, long
 
 extern "C" void fgk_paint_plan_destroy(fgk_paintplan *pp) {
     if (!pp) return;
-    fgk_free(pp->H);
-    fgk_free(pp->O);
-    fgk_free(pp->rank);
-    fgk_free(pp->tot);
-    fgk_free(pp->base);
-    fgk_free(pp->items);
-    fgk_free(pp->sorted);
+    uint32_t *bufs[] = {pp->H1, pp->O1, pp->S1, pp->B1, pp->T1, pp->H2, pp->O2, pp->S2, pp->B2,
+                        pp->seg1, pp->base1, pp->chunk0, pp->blk0, pp->T2, pp->tilebase,
+                        pp->items, pp->rank32};
+    for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; ++i) fgk_free(bufs[i]);
+    fgk_free(pp->sorted1);
+    fgk_free(pp->sorted2);
     free(pp);
 }
 
 extern "C" double fgk_paint_fixed_scale(const fgk_paintplan *pp) { return ldexp(1.0, -pp->S); }
 extern "C" int fgk_paint_plan_tile(const fgk_paintplan *pp) { return pp->T; }
 
-// stage 0: count, 1: scan, 2: scatter, 3: paint
+static const char *const kPaintStages[] = {"sort1_count", "sort1_scan", "sort1_scatter",
+                                           "sort2_count", "sort2_scan", "sort2_scatter",
+                                           "paint_tiles"};
+
+extern "C" int fgk_paint_nstages(void) { return (int)(sizeof kPaintStages / sizeof kPaintStages[0]); }
+extern "C" const char *fgk_paint_stage_name(int i) {
+    return (i >= 0 && i < fgk_paint_nstages()) ? kPaintStages[i] : "";
+}
+
+static unsigned blocks_for(long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// scans of one level: H[nw][nd] histograms of nw chunks in nseg segments
+// (segment s = blocks [blk0[s], blk0[s+1])) -> O[j][d] output offsets
+static int level_scans(hipStream_t s, const uint32_t *H, long nw, int nd, uint32_t *S, uint32_t *B,
+                       const uint32_t *blk0, int nseg, uint32_t *T, uint32_t *base, int mode,
+                       int cap, uint32_t *aux, uint32_t *aux_blk, uint32_t *O, long nslots) {
+    const long nblk = nw / RB;
+    hipLaunchKernelGGL(k_blocksum, dim3(blocks_for(nblk * nd, 256)), dim3(256), 0, s, H, nblk, nd,
+                       S);
+    FGK_LAUNCHED();
+    hipLaunchKernelGGL(k_segscan, dim3(blocks_for((long)nseg * nd, 256)), dim3(256), 0, s, S,
+                       blk0, nseg, nd, B, T);
+    FGK_LAUNCHED();
+    hipLaunchKernelGGL(k_digitscan, dim3(1), dim3(SCAN_THREADS), 0, s, T, nslots, mode, cap, base,
+                       aux, aux_blk);
+    FGK_LAUNCHED();
+    hipLaunchKernelGGL(k_colscan, dim3(blocks_for(nblk * nd, 256)), dim3(256), 0, s, H, blk0, nseg,
+                       nblk, nd, base, B, O);
+    FGK_LAUNCHED();
+    return 0;
+}
+
 extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, const float *y,
                                const float *z, long Np, double Hinv, int64_t *gridA,
                                int64_t *gridB, fgk_stream_t st) {
@@ -495,75 +800,123 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
         fgk_set_error("paint: Np=%ld exceeds the plan's Np_max=%ld", Np, pp->Np_max);
         return -2;
     }
-    TileGeo g = {pp->Ng, pp->T, pp->nt, pp->F, Hinv};
-    const int ntiles = (int)pp->ntiles;
-    const long nwg = (Np + pp->chunk - 1) / pp->chunk;
-    switch (stage) {
-    case 0:
-        if (pp->lds_hist) {
-            if (nwg > 0) {
-                size_t lds = (size_t)((ntiles + 1) / 2) * 4;
-                hipLaunchKernelGGL(k_count_lds, dim3((unsigned)nwg), dim3(COUNT_THREADS), lds, s,
-                                   x, y, z, Np, g, ntiles, pp->chunk, pp->H,
-                                   (uint16_t *)pp->rank);
-                FGK_LAUNCHED();
-            }
-        } else {
-            if (fgk_fill_zero(pp->tot, (size_t)ntiles * 4, st)) return -1;
+    KeyGeo g = pp->g;
+    g.Hinv = Hinv;
+    const long nw1 = round_up((Np + SORT_CHUNK - 1) / SORT_CHUNK, RB);
+    if (!pp->two_level) {
+        switch (stage) {
+        case 0:
+            if (fgk_fill_zero(pp->T2, (size_t)pp->nslots * sizeof(uint32_t), st)) return -1;
             if (Np > 0) {
-                hipLaunchKernelGGL(k_count_global, dim3(fgk_ceil_div(Np, 256)), dim3(256), 0, s,
-                                   x, y, z, Np, g, pp->tot, (uint32_t *)pp->rank);
+                hipLaunchKernelGGL(k_count_global, dim3(blocks_for(Np, 256)), dim3(256), 0, s, x,
+                                   y, z, Np, g, pp->T2, pp->rank32);
                 FGK_LAUNCHED();
             }
-        }
-        return 0;
-    case 1:
-        if (pp->lds_hist) {
-            hipLaunchKernelGGL(k_colsum, dim3(fgk_ceil_div(ntiles, 256)), dim3(256), 0, s, pp->H,
-                               nwg, ntiles, pp->tot);
+            return 0;
+        case 1:
+            hipLaunchKernelGGL(k_digitscan, dim3(1), dim3(SCAN_THREADS), 0, s, pp->T2, pp->nslots,
+                               2, pp->cap, pp->tilebase, pp->items, (uint32_t *)nullptr);
             FGK_LAUNCHED();
+            return 0;
+        case 2:
+            if (Np > 0) {
+                hipLaunchKernelGGL(k_scatter_global, dim3(blocks_for(Np, 256)), dim3(256), 0, s,
+                                   x, y, z, Np, g, pp->tilebase, pp->rank32, pp->sorted2);
+                FGK_LAUNCHED();
+            }
+            return 0;
+        case 3:
+        case 4:
+        case 5:
+            return 0;
+        default:
+            break;
         }
-        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, s, pp->tot,
-                           (long)ntiles, pp->cap, pp->base, pp->items);
-        FGK_LAUNCHED();
-        if (pp->lds_hist) {
-            hipLaunchKernelGGL(k_colscan, dim3(fgk_ceil_div(ntiles, 256)), dim3(256), 0, s, pp->H,
-                               nwg, ntiles, pp->base, pp->O);
+    } else {
+        switch (stage) {
+        case 0:   // level 1 histograms
+            hipLaunchKernelGGL((k_sort<1, 1>), dim3((unsigned)nw1), dim3(SORT_THREADS), 0, s, x, y,
+                               z, Np, g, (const uint64_t *)nullptr, (const uint32_t *)nullptr,
+                               (const uint32_t *)nullptr, (const uint32_t *)nullptr, pp->nb1,
+                               pp->nb1, pp->bits1, pp->H1, (const uint32_t *)nullptr,
+                               (uint64_t *)nullptr);
             FGK_LAUNCHED();
+            return 0;
+        case 1:   // level 1 offsets and the level 2 chunk layout
+            hipLaunchKernelGGL(k_set_seg1, dim3(1), dim3(64), 0, s, pp->seg1,
+                               (uint32_t)(nw1 / RB));
+            FGK_LAUNCHED();
+            return level_scans(s, pp->H1, nw1, pp->nb1, pp->S1, pp->B1, pp->seg1, 1, pp->T1,
+                               pp->base1, 1, pp->cap, pp->chunk0, pp->blk0, pp->O1, pp->nb1);
+        case 2:
+            hipLaunchKernelGGL((k_sort<1, 0>), dim3((unsigned)nw1), dim3(SORT_THREADS), 0, s, x, y,
+                               z, Np, g, (const uint64_t *)nullptr, (const uint32_t *)nullptr,
+                               (const uint32_t *)nullptr, (const uint32_t *)nullptr, pp->nb1,
+                               pp->nb1, pp->bits1, (uint32_t *)nullptr, pp->O1, pp->sorted1);
+            FGK_LAUNCHED();
+            return 0;
+        case 3: { // level 2 histograms (every capacity chunk; empty ones write zeros)
+            const long nw2 = nw1 + (long)pp->nb1 * RB;
+            hipLaunchKernelGGL((k_sort<2, 1>), dim3((unsigned)nw2), dim3(SORT_THREADS), 0, s,
+                               (const float *)nullptr, (const float *)nullptr,
+                               (const float *)nullptr, Np, g, pp->sorted1, pp->chunk0, pp->base1,
+                               pp->T1, pp->nb1, pp->nb2, pp->bits2, pp->H2,
+                               (const uint32_t *)nullptr, (uint64_t *)nullptr);
+            FGK_LAUNCHED();
+            return 0;
         }
-        return 0;
-    case 2:
-        if (Np <= 0) return 0;
-        if (pp->lds_hist) {
-            hipLaunchKernelGGL(k_scatter<1>, dim3((unsigned)nwg), dim3(COUNT_THREADS), 0, s, x, y,
-                               z, Np, g, ntiles, pp->chunk, pp->O, pp->rank, pp->sorted);
-        } else {
-            hipLaunchKernelGGL(k_scatter<0>, dim3(fgk_ceil_div(Np, COUNT_THREADS)),
-                               dim3(COUNT_THREADS), 0, s, x, y, z, Np, g, ntiles,
-                               (long)COUNT_THREADS, pp->base, pp->rank, pp->sorted);
+        case 4: {
+            const long nw2 = nw1 + (long)pp->nb1 * RB;
+            return level_scans(s, pp->H2, nw2, pp->nb2, pp->S2, pp->B2, pp->blk0, pp->nb1, pp->T2,
+                               pp->tilebase, 2, pp->cap, pp->items, (uint32_t *)nullptr, pp->O2,
+                               pp->nslots);
         }
-        FGK_LAUNCHED();
-        return 0;
-    case 3: {
-        long max_items = pp->ntiles + (Np + pp->cap - 1) / pp->cap;
-        size_t lds = (size_t)paint_lds_bytes(pp->T, pp->interlace);
-        hipLaunchKernelGGL(k_paint_tiles, dim3((unsigned)max_items), dim3(PAINT_THREADS), lds, s,
-                           pp->sorted, pp->base, pp->items, ntiles, pp->nt, pp->T, pp->Ng, pp->F,
-                           (float)ldexp(1.0, pp->S), pp->cap, pp->interlace,
-                           (unsigned long long *)gridA, (unsigned long long *)gridB);
+        case 5: {
+            const long nw2 = nw1 + (long)pp->nb1 * RB;
+            hipLaunchKernelGGL((k_sort<2, 0>), dim3((unsigned)nw2), dim3(SORT_THREADS), 0, s,
+                               (const float *)nullptr, (const float *)nullptr,
+                               (const float *)nullptr, Np, g, pp->sorted1, pp->chunk0, pp->base1,
+                               pp->T1, pp->nb1, pp->nb2, pp->bits2, (uint32_t *)nullptr, pp->O2,
+                               pp->sorted2);
+            FGK_LAUNCHED();
+            return 0;
+        }
+        default:
+            break;
+        }
+    }
+    if (stage == 6) {
+        const long max_items = pp->nslots + (Np + pp->cap - 1) / pp->cap;
+        const size_t lds = (size_t)paint_lds_bytes(pp->T, pp->interlace);
+        const dim3 grid((unsigned)max_items), block(PAINT_THREADS);
+        const float ws = (float)ldexp(1.0, pp->S);
+        unsigned long long *A = (unsigned long long *)gridA, *B = (unsigned long long *)gridB;
+#define PAINT_T(TT)                                                                           \
+    hipLaunchKernelGGL(k_paint_tiles<TT>, grid, block, lds, s, pp->sorted2, pp->tilebase,     \
+                       pp->items, pp->nslots, g, pp->nb2, ws, pp->cap, pp->interlace, A, B)
+        switch (pp->T) {
+        case 16: PAINT_T(16); break;
+        case 12: PAINT_T(12); break;
+        case 8: PAINT_T(8); break;
+        case 4: PAINT_T(4); break;
+        case 2: PAINT_T(2); break;
+        case 1: PAINT_T(1); break;
+        default:
+            fgk_set_error("paint: no kernel instance for tile side %d", pp->T);
+            return -2;
+        }
+#undef PAINT_T
         FGK_LAUNCHED();
         return 0;
     }
-    default:
-        fgk_set_error("paint: unknown stage %d", stage);
-        return -2;
-    }
+    fgk_set_error("paint: unknown stage %d", stage);
+    return -2;
 }
 
 extern "C" int fgk_paint_fixed(fgk_paintplan *pp, const float *x, const float *y, const float *z,
                                long Np, double Hinv, int64_t *gridA, int64_t *gridB,
                                fgk_stream_t s) {
-    for (int stage = 0; stage < 4; ++stage)
+    for (int stage = 0; stage < fgk_paint_nstages(); ++stage)
         if (fgk_paint_stage(pp, stage, x, y, z, Np, Hinv, gridA, gridB, s)) return -1;
     return 0;
 }

@@ -22,6 +22,9 @@
 #include "../../../include/fftgal_batch.h"
 #include "tune.h"
 
+/* paint stages of the fgk layer (fgk_paint_nstages) */
+#define NPAINT 7
+
 int fftgal_nk(int Ng, double L, double dK);
 void fftgal_winv_table(int Ng, double *winv);
 void fftgal_phase_table(int Ng, const double offset[3], int onoff, double *phase);
@@ -170,6 +173,10 @@ int fftgal_power_create(fftgal_power_t **out, const fftgal_power_desc *desc) {
     STEP(fgk_stream_create(&p->s));
     STEP(fgk_fft_plan_create(&p->plan, Ng, 4));
     if (!rc) STEP(fgk_fft_set_tuning(p->plan, p->tune.fft_cols, p->tune.fft_rows));
+    if (!rc && fgk_paint_nstages() != NPAINT) {
+        fgk_set_error("internal: paint stage count mismatch");
+        rc = -3;
+    }
     STEP(fgk_paint_plan_create(&p->paint, Ng, p->tune.paint_tile, desc->Np_max, desc->interlace));
     const size_t n64 = (size_t)Ng * Ng * Ng * sizeof(int64_t);
     const size_t nf = (size_t)Ng * Ng * (2 * (Ng / 2 + 1)) * sizeof(float);
@@ -238,12 +245,13 @@ int fftgal_power_create(fftgal_power_t **out, const fftgal_power_desc *desc) {
 
 int fftgal_power_NK(const fftgal_power_t *p) { return p->NK; }
 
-/* pipeline stages, in order (fftgal_power_profile reports one time each) */
-static const char *const kStageNames[] = {
-    "zero_bins",  "paint_count", "paint_scan", "paint_scatter", "paint_tiles",
-    "fft_rows_A", "fft_cols_j_A", "fft_cols_i_A", "fft_rows_B", "fft_cols_j_B",
-    "fft_cols_i_B", "fused_bin",  "copy_bins"};
-#define NSTAGES ((int)(sizeof kStageNames / sizeof kStageNames[0]))
+/* pipeline stages, in order (fftgal_power_profile reports one time each):
+ * zero_bins, the paint stages of fgk_paint_stage_name(), the FFT passes of
+ * both grids, the fused bin kernel and the copy of the sums to the host */
+static const char *const kTailStages[] = {"fft_rows_A", "fft_cols_j_A", "fft_cols_i_A",
+                                          "fft_rows_B", "fft_cols_j_B", "fft_cols_i_B",
+                                          "fused_bin",  "copy_bins"};
+#define NSTAGES (1 + NPAINT + (int)(sizeof kTailStages / sizeof kTailStages[0]))
 
 /* enqueue the whole pipeline; ev (NSTAGES+1 events or NULL) brackets the
  * stages for profiling */
@@ -261,7 +269,7 @@ static int enqueue_work(fftgal_power_t *p, const float *x, const float *y, const
 #define MARK() do { if (ev) TRY(fgk_event_record(ev[st], p->s)); ++st; } while (0)
     MARK();
     TRY(fgk_fill_zero(p->d_bins, p->nbins * sizeof(double), p->s));
-    for (int stage = 0; stage < 4; ++stage) {
+    for (int stage = 0; stage < NPAINT; ++stage) {
         MARK();
         TRY(fgk_paint_stage(p->paint, stage, x, y, z, Np, Hinv, p->gA, p->gB, p->s));
     }
@@ -289,7 +297,10 @@ static int enqueue_work(fftgal_power_t *p, const float *x, const float *y, const
 int fftgal_power_stages(void) { return NSTAGES; }
 
 const char *fftgal_power_stage_name(int i) {
-    return (i >= 0 && i < NSTAGES) ? kStageNames[i] : NULL;
+    if (i == 0) return "zero_bins";
+    if (i >= 1 && i <= NPAINT) return fgk_paint_stage_name(i - 1);
+    if (i > NPAINT && i < NSTAGES) return kTailStages[i - 1 - NPAINT];
+    return NULL;
 }
 
 int fftgal_power_profile(fftgal_power_t *p, const float *x, const float *y, const float *z,
