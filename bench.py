@@ -53,17 +53,19 @@ def parse():
 def stage_bytes(name, Np, Ng):
     """Algorithmic (compulsory) bytes of one stage of one catalog."""
     n3 = Ng ** 3
-    nc = Ng * Ng * (Ng + 2)
+    spec = 4 * n3                                # FP32 packed half spectrum: Ng^3/2 complex
     table = {
-        "paint_count": 12 * Np,                 # read float32 xyz
-        "paint_scan": 0,
-        "paint_scatter": 12 * Np + 8 * Np,      # read xyz, write packed 8 B
+        "sort1_count": 12 * Np,                 # read float32 xyz
+        "sort1_scatter": 12 * Np + 8 * Np,      # read xyz, write packed 8-byte word
+        "sort2_count": 8 * Np,                  # read packed words
+        "sort2_scatter": 8 * Np + 8 * Np,       # read, write packed words
         "paint_tiles": 8 * Np + 2 * 16 * n3,    # read packed, RMW both int64 grids once
-        "fft_rows_A": 8 * n3 + 4 * nc,          # read int64 grid, write FP32 half spectrum
-        "fft_rows_B": 8 * n3 + 4 * nc,
-        "fft_cols_j_A": 8 * nc, "fft_cols_i_A": 8 * nc,   # read + write FP32
-        "fft_cols_j_B": 8 * nc, "fft_cols_i_B": 8 * nc,
-        "fused_bin": 2 * 4 * nc,                # read both spectra
+        # read the int64 grid, clear it for the next paint, write the spectrum
+        "fft_rows_A": 8 * n3 + 8 * n3 + spec,
+        "fft_rows_B": 8 * n3 + 8 * n3 + spec,
+        "fft_cols_j_A": 2 * spec, "fft_cols_i_A": 2 * spec,   # read + write FP32
+        "fft_cols_j_B": 2 * spec, "fft_cols_i_B": 2 * spec,
+        "fused_bin": 2 * spec,                  # read both spectra
     }
     return table.get(name, 0)
 

@@ -99,6 +99,7 @@ API = {
     "fftgal_power_tuning": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
     "fftgal_power_use_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "fftgal_power_fft_generic": (C.c_int, [C.c_void_p, C.c_int]),
     "fftgal_power_stream": (C.c_void_p, [C.c_void_p]),
     "fftgal_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_ulong]),
     "fftgal_power_stages": (C.c_int, []),
@@ -370,6 +371,27 @@ class PowerPlan:
 
     def use_graph(self, on):
         self.lib.fftgal_power_use_graph(self.h, 1 if on else 0)
+
+    def fft_generic(self, on):
+        """Force the generic FFT kernels (True) or restore the default;
+        returns True if the generic kernels are in use."""
+        r = self.lib.fftgal_power_fft_generic(self.h, 1 if on else 0)
+        self._check(min(r, 0), "fftgal_power_fft_generic")
+        return bool(r)
+
+    def spectra_host(self):
+        """The FP32 spectra of the last run copied to host as complex64
+        arrays [Ng, Ng, Ng//2] in the packed half-spectrum layout."""
+        out = []
+        for ptr in self.spectra():
+            if not ptr:
+                out.append(None)
+                continue
+            h = np.empty((self.Ng, self.Ng, self.Ng // 2), np.complex64)
+            self._check(self.lib.fftgal_memcpy_d2h(h.ctypes.data, ptr, h.nbytes),
+                        "fftgal_memcpy_d2h")
+            out.append(h)
+        return tuple(out)
 
     def stream(self):
         return self.lib.fftgal_power_stream(self.h)

@@ -61,15 +61,24 @@ int  fgk_fft_r2c_f64(fgk_fftplan *p, double *f, double scale, fgk_stream_t s);
 int  fgk_fft_c2r_f64(fgk_fftplan *p, double *f, double scale, fgk_stream_t s);
 /* FP32 forward r2c whose real input is an int64 fixed-point grid
  * src[(i*Ng+j)*Ng+k] times in_scale; src rows are cleared to zero after they
- * are read when clear_src != 0; output in the padded float layout */
+ * are read when clear_src != 0.  Output in the packed half-spectrum layout
+ * c[(i*Ng+j)*(Ng/2) + k], k < Ng/2 (complex float): the real k = 0 and
+ * k = Ng/2 values of every row transform share column 0 as (X0, XN), so
+ * after the column passes column 0 holds Z = F(X0) + i F(XN) and
+ *   F(X0)[m] = (Z[m] + conj Z[-m]) / 2,  F(XN)[m] = (Z[m] - conj Z[-m]) / 2i */
 int  fgk_fft_r2c_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *src,
                                 double in_scale, int clear_src, fgk_stream_t s);
 /* the three passes separately (for the profiler and the tuner) */
 int  fgk_fft_rows_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *src,
                                  double in_scale, int clear_src, fgk_stream_t s);
 int  fgk_fft_cols_f32(fgk_fftplan *p, float *f, int axis, int sign, fgk_stream_t s);
+/* 1 if the FP32 passes of this Ng run compile-time specialised kernels
+ * (otherwise the generic LDS Stockham kernels) */
+int  fgk_fft_f32_specialised(int Ng);
+/* force the generic FP32 kernels (1) or the default choice (0); for tests */
+int  fgk_fft_force_generic(fgk_fftplan *p, int on);
 /* tuning knobs: columns per workgroup for the strided passes, rows per
- * workgroup for the row pass (0 = default) */
+ * workgroup for the row pass (0 = default); generic kernels only */
 int  fgk_fft_set_tuning(fgk_fftplan *p, int cols_per_wg, int rows_per_wg);
 
 /* ---- painting (fgk_paint.hip) ---- */
@@ -134,12 +143,18 @@ int  fgk_bin_geometry(const fgk_binspec *b, double *bins_dev, int64_t *counts_de
 int  fgk_bin_pgrid_f64(const double *f, const fgk_binspec *b, double *bins_dev,
                        int64_t *counts_dev, fgk_stream_t s);
 /* fused: phase (dual grid), interlace average, deconvolution, |delta|^2*norm
- * - Pshot, binning.  A = plain grid, B = dual grid (NULL: no interlacing).
- * tables: phase (3*Ng complex double), winv (Ng double) on the device */
+ * - Pshot, binning.  A = plain grid, B = dual grid (NULL: no interlacing),
+ * both in the packed half-spectrum layout of fgk_fft_r2c_f32_from_fixed.
+ * tables: phase (3*Ng complex double), winv (Ng double) on the device.
+ * Each spectrum is normalised by norm * npart / (its DC mode, the painted
+ * mass in particle units).
+ * Power sums only (b->geometry must be 0; fgk_bin_geometry gives K and
+ * the counts).  Two launches: modes with 0 < k < Ng/2, then the k = 0 and
+ * k = Ng/2 planes. */
 int  fgk_fused_bin_f32(const float *A, const float *B, const fgk_binspec *b,
                        const double *phase_dev, const double *winv_dev,
-                       double norm, double Vinv, double Pshot,
-                       double *bins_dev, int64_t *counts_dev, fgk_stream_t s);
+                       double norm, double Vinv, double Pshot, double npart,
+                       double *bins_dev, fgk_stream_t s);
 
 #ifdef __cplusplus
 }

@@ -36,6 +36,9 @@ struct fgk_fftplan {
     void *tw;            // Ng complex, exp(-2 pi i t / Ng)
     int cols_per_wg;     // 0 = automatic
     int rows_per_wg;     // 0 = automatic
+    int spec;            // FP32 passes use the specialised kernels
+    void *stw_cols;      // stage twiddles, length Ng (specialised FP32 only)
+    void *stw_rows;      // stage twiddles, length Ng/2
 };
 
 static int factor_stages(int n, Stages *s) {
@@ -216,7 +219,9 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
 
 // ------------------------------------------------------------ row kernels
 
-// MODE 0: real input in place in f; MODE 1: int64 fixed-point input src
+// MODE 0: real input in place in f; MODE 1: int64 fixed-point input src;
+// MODE 2: int64 input, output in the packed half-spectrum layout (Ng/2
+// complex per row, column 0 = (X[0], X[Ng/2]), both real)
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256)
 k_rows_r2c(T *f, int64_t *src, double in_scale, int clear_src, T out_scale, int N, int Ng_pad,
@@ -246,12 +251,29 @@ k_rows_r2c(T *f, int64_t *src, double in_scale, int clear_src, T out_scale, int 
     }
     __syncthreads();
     cpx<T> *res = lds_fft<T>(a, b, S, P, 0, 0, 1, M, tw, 2, -1);
-    const int K1 = M + 1;
+    const int K1 = MODE == 2 ? M : M + 1;
     for (int idx = threadIdx.x; idx < P * K1; idx += blockDim.x) {
         const int p = idx / K1;
         const int k = idx - p * K1;
         const long row = row0 + p;
         if (row >= nrows) continue;
+        if (MODE == 2) {
+            cpx<T> X;
+            if (k == 0) {
+                const cpx<T> Z = res[p * M];
+                X = cmk((Z.x + Z.y) * out_scale, (Z.x - Z.y) * out_scale);
+            } else {
+                cpx<T> Z = res[p * M + k];
+                cpx<T> Zc = res[p * M + M - k];
+                Zc.y = -Zc.y;
+                cpx<T> E = cscale(cadd(Z, Zc), (T)0.5);
+                cpx<T> D = csub(Z, Zc);
+                cpx<T> O = cmk((T)0.5 * D.y, (T)-0.5 * D.x);
+                X = cscale(cadd(E, cmul(O, tw[k])), out_scale);
+            }
+            ((cpx<T> *)f)[row * (long)M + k] = X;
+            continue;
+        }
         const int k1 = (k == M) ? 0 : k;
         const int k2 = (k == 0) ? 0 : M - k;
         cpx<T> Z = res[p * M + k1];
@@ -333,6 +355,313 @@ k_cols(cpx<T> *g, int N, int Nk, long stride_t, long stride_o, int C, int lgC, S
     }
 }
 
+// ------------------------------------- specialised FP32 kernels (fast path)
+//
+// For the pencil lengths of the common grid sizes the whole transform is
+// unrolled at compile time: the radix sequence, every stage's span and the
+// butterflies per thread are constants, so index arithmetic folds away.
+// Each thread owns whole butterflies; a stage loads its R inputs per
+// butterfly into registers (from global memory in the first stage), applies
+// the stage twiddles (a per-stage table [k][r-1], FP64-built), runs the
+// radix-R butterfly and stores to LDS (to global memory in the last stage of
+// a column pass).  The layout is the packed half spectrum: Ng/2 complex
+// columns per row, so rows are 2^k-byte aligned and a 16-column workgroup
+// moves whole 128-byte lines.
+
+struct RSeq {
+    int n;
+    int r[16];
+};
+
+// radix-8 stages, one or two radix-4 stages for the remaining power of two,
+// then the odd prime factors
+constexpr RSeq radix_seq(int L) {
+    RSeq s{0, {0}};
+    int m = L, a = 0;
+    while (m % 2 == 0) { m /= 2; ++a; }
+    int n8 = a / 3;
+    const int rem = a % 3;
+    if (rem == 1 && n8 > 0) --n8;
+    for (int i = 0; i < n8; ++i) s.r[s.n++] = 8;
+    if (rem == 2) s.r[s.n++] = 4;
+    if (rem == 1) {
+        if (a >= 4) { s.r[s.n++] = 4; s.r[s.n++] = 4; }
+        else s.r[s.n++] = 2;
+    }
+    const int odd[5] = {3, 5, 7, 11, 13};
+    for (int q = 0; q < 5; ++q)
+        while (m % odd[q] == 0) { s.r[s.n++] = odd[q]; m /= odd[q]; }
+    return s;
+}
+
+constexpr int seq_rmax(int L) {
+    RSeq s = radix_seq(L);
+    int r = 1;
+    for (int i = 0; i < s.n; ++i) r = s.r[i] > r ? s.r[i] : r;
+    return r;
+}
+
+// offset of stage S's twiddles in the concatenated table: stage s holds
+// Ns_s * (R_s - 1) entries
+constexpr int stw_offset(int L, int S) {
+    RSeq s = radix_seq(L);
+    int off = 0, ns = 1;
+    for (int i = 0; i < S; ++i) {
+        off += ns * (s.r[i] - 1);
+        ns *= s.r[i];
+    }
+    return off;
+}
+
+constexpr int stw_size(int L) { return stw_offset(L, radix_seq(L).n); }
+
+__device__ __forceinline__ cpx<float> twmul(cpx<float> v, cpx<float> w, int sign) {
+    return sign < 0 ? cmul(v, w) : cmulc(v, w);
+}
+
+// stage S (span NS) of a length-L transform; TPP threads per pencil, q = this
+// thread's index within its pencil.  in(t): stage input (global in stage 0,
+// LDS after), ld/st: LDS, gout: global output of the last stage when GOUT.
+template <int L, int TPP, int S, int NS, bool GOUT, class In, class Ld, class St, class GOut>
+__device__ __forceinline__ void spencil(int q, In in, Ld ld, St st, GOut gout,
+                                        const cpx<float> *__restrict__ stw,
+                                        const cpx<float> *__restrict__ tw, int tws, int sign) {
+    constexpr RSeq seq = radix_seq(L);
+    constexpr int R = seq.r[S];
+    constexpr bool last = S + 1 == seq.n;
+    constexpr int NB = L / R;
+    constexpr int B = (NB + TPP - 1) / TPP;
+    cpx<float> v[B][R];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = q + b * TPP;
+        if (NB % TPP == 0 || j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = S == 0 ? in(j + r * NB) : ld(j + r * NB);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = q + b * TPP;
+        if (NB % TPP == 0 || j < NB) {
+            if (NS > 1) {
+                const int k = j % NS;
+                const cpx<float> *w = stw + stw_offset(L, S) + k * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = twmul(v[b][r], w[r - 1], sign);
+            }
+            Bfly<float, R>::run(v[b], sign, tw, L, tws);
+        }
+    }
+    if (last && GOUT) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = q + b * TPP;
+            if (NB % TPP == 0 || j < NB) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) gout(j + r * NB, v[b][r]);
+            }
+        }
+        return;
+    }
+    if (S > 0) __syncthreads();          // every thread has read this stage's inputs
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = q + b * TPP;
+        if (NB % TPP == 0 || j < NB) {
+            const int k = j % NS;
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) st(base + r * NS, v[b][r]);
+        }
+    }
+    __syncthreads();
+    if constexpr (!last)
+        spencil<L, TPP, S + 1, NS * R, GOUT>(q, in, ld, st, gout, stw, tw, tws, sign);
+}
+
+// column pass: C consecutive columns of one row/plane per workgroup, the
+// pencil index t at stride stride_t; LDS row stride C+1 complex
+template <int L, int C, int TPP>
+__global__ void __launch_bounds__(C * TPP)
+k_cols32(cpx<float> *g, int ncols, long stride_t, long stride_o, const cpx<float> *stw,
+         const cpx<float> *tw, int tws, int sign) {
+    constexpr int LS = C + 1;
+    __shared__ cpx<float> buf[L * LS];
+    const int c = threadIdx.x % C, q = threadIdx.x / C;
+    const int ncb = (ncols + C - 1) / C;
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int o = (int)(lb / (unsigned)ncb);
+    const int c0 = (int)(lb - (unsigned)o * ncb) * C;
+    const bool live = c0 + c < ncols;
+    cpx<float> *base = g + (long)o * stride_o + c0 + c;
+    auto gin = [&](int t) -> cpx<float> {
+        return live ? base[(long)t * stride_t] : cmk(0.0f, 0.0f);
+    };
+    auto gout = [&](int t, cpx<float> v) {
+        if (live) base[(long)t * stride_t] = v;
+    };
+    auto ld = [&](int t) -> cpx<float> { return buf[t * LS + c]; };
+    auto st = [&](int t, cpx<float> v) { buf[t * LS + c] = v; };
+    spencil<L, TPP, 0, 1, true>(q, gin, ld, st, gout, stw, tw, tws, sign);
+}
+
+// row pass: P rows of 2M int64 fixed-point values per workgroup, complex
+// length-M transform of z[t] = x[2t] + i x[2t+1], then the real-input split
+// X[k] = A[k] + W^k B[k], X[M-k] = conj(A[k] - W^k B[k]) with
+// A = (Z[k] + conj Z[M-k]) / 2, B = (Z[k] - conj Z[M-k]) / 2i, W = e^{-2 pi i/2M},
+// written to the packed layout (column 0 = (X[0], X[M])).  LDS holds each row
+// with one pad slot per 8 elements.
+template <int M, int P, int TPP>
+__global__ void __launch_bounds__(P * TPP)
+k_rows32(cpx<float> *out, int64_t *src, double in_scale, int clear_src, long nrows,
+         const cpx<float> *stw, const cpx<float> *tw) {
+    constexpr int RS = M + M / 8 + 1;
+    __shared__ cpx<float> buf[P * RS];
+    const int q = threadIdx.x % TPP, p = threadIdx.x / TPP;
+    const long row = (long)xcd_block(blockIdx.x, gridDim.x) * P + p;
+    const bool live = row < nrows;
+    cpx<float> *mine = buf + p * RS;
+    longlong2 *srow = (longlong2 *)(src + row * (2L * M));
+    auto gin = [&](int t) -> cpx<float> {
+        if (!live) return cmk(0.0f, 0.0f);
+        const longlong2 v = srow[t];
+        if (clear_src) srow[t] = make_longlong2(0, 0);
+        return cmk((float)((double)v.x * in_scale), (float)((double)v.y * in_scale));
+    };
+    auto ld = [&](int t) -> cpx<float> { return mine[t + (t >> 3)]; };
+    auto st = [&](int t, cpx<float> v) { mine[t + (t >> 3)] = v; };
+    auto none = [&](int, cpx<float>) {};
+    spencil<M, TPP, 0, 1, false>(q, gin, ld, st, none, stw, tw, 2, -1);
+    if (!live) return;
+    cpx<float> *orow = out + row * (long)M;
+    for (int k = q; k <= M / 2; k += TPP) {
+        const cpx<float> Z = ld(k);
+        if (k == 0) {
+            orow[0] = cmk(Z.x + Z.y, Z.x - Z.y);
+            continue;
+        }
+        cpx<float> Zc = ld(M - k);
+        Zc.y = -Zc.y;
+        const cpx<float> A = cscale(cadd(Z, Zc), 0.5f);
+        const cpx<float> D = csub(Z, Zc);
+        const cpx<float> Bk = cmk(0.5f * D.y, -0.5f * D.x);
+        const cpx<float> Tk = cmul(Bk, tw[k]);
+        orow[k] = cadd(A, Tk);
+        const cpx<float> Dm = csub(A, Tk);
+        orow[M - k] = cmk(Dm.x, -Dm.y);
+    }
+}
+
+// launch shapes: columns per workgroup C (<= 16) and rows per workgroup P
+// with at most 1024 threads and about 80 KB of LDS per workgroup
+constexpr int cols_C(int L) {
+    int C = 16;
+    while (C > 1 && (C * (L / seq_rmax(L)) > 1024 || L * (C + 1) * 8 > 80 * 1024)) C /= 2;
+    return C;
+}
+constexpr int cols_TPP(int L) {
+    int t = L / seq_rmax(L);
+    while (cols_C(L) * t > 1024) t /= 2;
+    return t;
+}
+constexpr int rows_TPP(int M) {
+    int t = M / seq_rmax(M);
+    return t > 1024 ? 1024 : t;
+}
+constexpr int rows_P(int M) {
+    int P = 1024 / rows_TPP(M);
+    while (P > 1 && P * (M + M / 8 + 1) * 8 > 80 * 1024) P /= 2;
+    return P;
+}
+
+// grid sides with specialised kernels (columns of length Ng, rows of Ng/2)
+#define FGK_SPECIALISED_NG(X) \
+    X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(2048)
+
+extern "C" int fgk_fft_f32_specialised(int Ng) {
+    switch (Ng) {
+#define CASE_NG(n) case n:
+        FGK_SPECIALISED_NG(CASE_NG)
+#undef CASE_NG
+        return 1;
+    default:
+        return 0;
+    }
+}
+
+static int launch_cols32(int N, cpx<float> *g, long stride_t, long stride_o,
+                         const cpx<float> *stw, const cpx<float> *tw, int sign, hipStream_t s) {
+    const int ncols = N / 2;
+    switch (N) {
+#define CASE_NG(n)                                                                            \
+    case n: {                                                                                 \
+        constexpr int C = cols_C(n), TPP = cols_TPP(n);                                       \
+        const long nb = (long)N * ((ncols + C - 1) / C);                                      \
+        hipLaunchKernelGGL((k_cols32<n, C, TPP>), dim3((unsigned)nb), dim3(C * TPP), 0, s, g, \
+                           ncols, stride_t, stride_o, stw, tw, 1, sign);                      \
+        break;                                                                                \
+    }
+        FGK_SPECIALISED_NG(CASE_NG)
+#undef CASE_NG
+    default:
+        fgk_set_error("no specialised FFT kernel for Ng=%d", N);
+        return -2;
+    }
+    FGK_LAUNCHED();
+    return 0;
+}
+
+static int launch_rows32(int N, cpx<float> *out, int64_t *src, double in_scale, int clear_src,
+                         const cpx<float> *stw, const cpx<float> *tw, hipStream_t s) {
+    const long nrows = (long)N * N;
+    switch (N) {
+#define CASE_NG(n)                                                                            \
+    case n: {                                                                                 \
+        constexpr int M = n / 2, P = rows_P(M), TPP = rows_TPP(M);                            \
+        hipLaunchKernelGGL((k_rows32<M, P, TPP>), dim3((unsigned)((nrows + P - 1) / P)),      \
+                           dim3(P * TPP), 0, s, out, src, in_scale, clear_src, nrows, stw, tw); \
+        break;                                                                                \
+    }
+        FGK_SPECIALISED_NG(CASE_NG)
+#undef CASE_NG
+    default:
+        fgk_set_error("no specialised FFT kernel for Ng=%d", N);
+        return -2;
+    }
+    FGK_LAUNCHED();
+    return 0;
+}
+
+// concatenated per-stage twiddles of a length-L transform (FP64-built):
+// stage s, k < Ns_s, r = 1..R_s-1 -> exp(-2 pi i r k / (Ns_s R_s))
+static int stage_twiddles(int L, void **dev) {
+    *dev = NULL;
+    const RSeq seq = radix_seq(L);
+    const int n = stw_size(L);
+    float *h = (float *)malloc((size_t)(n > 0 ? n : 1) * 2 * sizeof(float));
+    if (!h) { fgk_set_error("out of host memory"); return -1; }
+    int off = 0, ns = 1;
+    for (int st = 0; st < seq.n; ++st) {
+        const int R = seq.r[st];
+        for (int k = 0; k < ns; ++k)
+            for (int r = 1; r < R; ++r) {
+                const double ang = -2.0 * M_PI * (double)r * k / ((double)ns * R);
+                h[2 * off] = (float)cos(ang);
+                h[2 * off + 1] = (float)sin(ang);
+                ++off;
+            }
+        ns *= R;
+    }
+    int rc = fgk_malloc(dev, (size_t)(n > 0 ? n : 1) * 2 * sizeof(float));
+    if (!rc) {
+        hipError_t e = hipMemcpy(*dev, h, (size_t)n * 2 * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = fgk_hip_fail(e, "hipMemcpy stage twiddles", __FILE__, __LINE__);
+    }
+    free(h);
+    return rc;
+}
+
 // ------------------------------------------------------------------ host
 
 extern "C" int fgk_fft_plan_create(fgk_fftplan **out, int Ng, int precision) {
@@ -373,13 +702,27 @@ extern "C" int fgk_fft_plan_create(fgk_fftplan **out, int Ng, int precision) {
         free(p);
         return fgk_hip_fail(e, "hipMemcpy twiddles", __FILE__, __LINE__);
     }
+    if (precision == 4 && fgk_fft_f32_specialised(Ng)) {
+        p->spec = 1;
+        if (stage_twiddles(Ng, &p->stw_cols) || stage_twiddles(Ng / 2, &p->stw_rows)) {
+            fgk_fft_plan_destroy(p);
+            return -1;
+        }
+    }
     *out = p;
+    return 0;
+}
+
+extern "C" int fgk_fft_force_generic(fgk_fftplan *p, int on) {
+    p->spec = (!on && p->stw_cols && p->stw_rows) ? 1 : 0;
     return 0;
 }
 
 extern "C" void fgk_fft_plan_destroy(fgk_fftplan *p) {
     if (!p) return;
     fgk_free(p->tw);
+    fgk_free(p->stw_cols);
+    fgk_free(p->stw_rows);
     free(p);
 }
 
@@ -422,9 +765,12 @@ static int ilog2(int c) {
     return l;
 }
 
+// packed: the FP32 packed half-spectrum layout (Ng/2 columns), else the
+// padded layout (Ng/2+1 columns in rows of Ng/2+1 complex)
 template <typename T>
-static int launch_cols(fgk_fftplan *p, T *f, int axis, int sign, T scale, hipStream_t s) {
-    const int N = p->Ng, Nk = N / 2 + 1;
+static int launch_cols(fgk_fftplan *p, T *f, int axis, int sign, T scale, hipStream_t s,
+                       int packed = 0) {
+    const int N = p->Ng, Nk = packed ? N / 2 : N / 2 + 1;
     const int C = cols_per_wg(p, sizeof(cpx<T>));
     const long rowc = Nk;                  // complex elements between j rows
     const long planec = (long)N * Nk;      // complex elements between i planes
@@ -440,7 +786,13 @@ static int launch_cols(fgk_fftplan *p, T *f, int axis, int sign, T scale, hipStr
 }
 
 extern "C" int fgk_fft_cols_f32(fgk_fftplan *p, float *f, int axis, int sign, fgk_stream_t s) {
-    return launch_cols<float>(p, f, axis, sign, 1.0f, (hipStream_t)s);
+    if (p->prec != 4) { fgk_set_error("plan precision is not FP32"); return -2; }
+    const int N = p->Ng, M = N / 2;
+    if (!p->spec) return launch_cols<float>(p, f, axis, sign, 1.0f, (hipStream_t)s, 1);
+    const long rowc = M, planec = (long)N * M;
+    return launch_cols32(N, (cpx<float> *)f, axis == 1 ? rowc : planec,
+                         axis == 1 ? planec : rowc, (const cpx<float> *)p->stw_cols,
+                         (const cpx<float> *)p->tw, sign, (hipStream_t)s);
 }
 
 extern "C" int fgk_fft_r2c_f64(fgk_fftplan *p, double *f, double scale, fgk_stream_t st) {
@@ -479,9 +831,12 @@ extern "C" int fgk_fft_rows_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *sr
     hipStream_t s = (hipStream_t)st;
     const int N = p->Ng, M = N / 2;
     const long nrows = (long)N * N;
+    if (p->spec)
+        return launch_rows32(N, (cpx<float> *)f, src, in_scale, clear_src,
+                             (const cpx<float> *)p->stw_rows, (const cpx<float> *)p->tw, s);
     const int P = rows_per_wg(p, sizeof(cpx<float>), 0);
     size_t lds = (size_t)2 * P * M * sizeof(cpx<float>);
-    hipLaunchKernelGGL((k_rows_r2c<float, 1>), dim3(fgk_ceil_div(nrows, P)), dim3(256), lds, s,
+    hipLaunchKernelGGL((k_rows_r2c<float, 2>), dim3(fgk_ceil_div(nrows, P)), dim3(256), lds, s,
                        f, src, in_scale, clear_src, 1.0f, N, 2 * (M + 1), nrows, p->sH,
                        (const cpx<float> *)p->tw, P);
     FGK_LAUNCHED();
@@ -491,6 +846,6 @@ extern "C" int fgk_fft_rows_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *sr
 extern "C" int fgk_fft_r2c_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *src,
                                           double in_scale, int clear_src, fgk_stream_t s) {
     if (fgk_fft_rows_f32_from_fixed(p, f, src, in_scale, clear_src, s)) return -1;
-    if (launch_cols<float>(p, f, 1, -1, 1.0f, (hipStream_t)s)) return -1;
-    return launch_cols<float>(p, f, 0, -1, 1.0f, (hipStream_t)s);
+    if (fgk_fft_cols_f32(p, f, 1, -1, s)) return -1;
+    return fgk_fft_cols_f32(p, f, 0, -1, s);
 }

@@ -4,12 +4,15 @@
 //   k_offset_phase_f64  src/fft.c:196-233 (separable table, 4-term product)
 //   k_deconv_f64        src/fft.c:236-254 (PCS window W^-1 per axis)
 //   k_P_f64             src/power.c:12-27 (|delta|^2/V - Pshot, DC = 0)
-// and one binning kernel used two ways:
+// and one binning kernel used three ways:
 //   MODE 0: bins P(k) held in the FP64 grid (src/power.c:52-67, 125-138)
 //   MODE 1: the fused fast-path kernel -- reads the FP32 spectra of the plain
-//           and dual grids, applies the dual grid's offset phase, the
+//           and dual grids (packed half-spectrum layout: Ng/2 columns, the
+//           k = 0 column holding F(X0) + i F(XN), separated here with the
+//           mirror mode), applies the dual grid's offset phase, the
 //           interlace average (src/fft.c:270-271), the deconvolution, forms
 //           |delta|^2/V - Pshot in FP64 and bins it, without writing a grid.
+//   MODE 2: the bin geometry alone (K sums and mode counts).
 // Binning: each workgroup owns a 4 x 4 x 64 block of modes, whose |k| spans
 // a short window of k bins; the Legendre (l = 0,2,4,6) and mu-wedge sums of
 // that window are accumulated in LDS (FP64) and flushed with one global
@@ -155,15 +158,223 @@ __device__ __forceinline__ void axis_range(int a, int b, int Ng, int &lo, int &h
     hi = am > ax ? am : ax;
 }
 
+// where one workgroup's modes are binned: LDS windows of bins (sv:
+// [wtot][nvals] sums, sc: [wtot][ncnt] counts) -- window 0 holds bins
+// [blo, blo + wn), window 1 (used by the k = 0 / k = Ng/2 plane kernel for
+// its Nyquist modes) bins [blo2, blo2 + wtot - wn) -- or, when the windows
+// do not fit, straight into the global bins
+struct BinOut {
+    double *sv;
+    unsigned int *sc;
+    int blo, wn, blo2, wtot;
+    bool windowed;
+    double *bins;
+    unsigned long long *cnts;
+};
+
+// one mode of power P into the bins, given its count (1 on the k = 0 and
+// k = Ng/2 planes, else 2), |k| in units of KF (Kamp) and los . k (dot):
+// bin, mu wedge and Legendre weights as src/power.c:52-67 (Pl) and
+// 125-143 (Pmu) compute them, in FP64 without contraction.  SUMS adds the
+// power sums, a.geometry the K sums and mode counts.  win selects the LDS
+// window (0 or 1).
+template <bool SUMS>
+__device__ __forceinline__ void bin_mode(const BinArgs &a, const BinOut &o, int count,
+                                         double Kamp, double dot, double P, int win) {
+#pragma clang fp contract(off)
+    const int Wm = a.Nmu > 0 ? a.Nmu + 1 : 0;
+    const double d2 = P * count;
+    const int iK = (int)floor(Kamp * a.dKinv);
+    if (iK >= a.NK) return;   // cannot happen for NK of src/power.c:36; keeps LDS in bounds
+    const double mu = dot / Kamp;
+    int imu = 0;
+    if (a.Nmu > 0) imu = (int)floor(fabs(mu * a.Nmu));
+    double pl[4];
+    if (a.do_pl && SUMS) {
+        const double mu2 = mu * mu;
+        pl[0] = d2;
+        pl[1] = d2 * (1.5 * mu2 - 0.5);
+        pl[2] = d2 * ((4.375 * mu2 - 3.75) * mu2 + 0.375);
+        pl[3] = d2 * (((14.4375 * mu2 - 19.6875) * mu2 + 6.5625) * mu2 - 0.3125);
+    }
+    const int slot = win ? o.wn + iK - o.blo2 : iK - o.blo;
+    if (o.windowed && slot >= (win ? o.wn : 0) && slot < (win ? o.wtot : o.wn)) {
+        double *v = o.sv + (size_t)slot * a.nvals;
+        int q = 0;
+        if (a.do_pl) {
+            if (SUMS)
+                for (int l = 0; l < 4; ++l) atomicAdd(v + l, pl[l]);
+            q = 4;
+        }
+        if (Wm && SUMS) atomicAdd(v + q + imu, d2);
+        if (a.geometry) {
+            const double kc = count * Kamp;
+            atomicAdd(v + q + Wm, kc);
+            if (Wm) atomicAdd(v + q + Wm + 1 + imu, kc);
+            unsigned int *c = o.sc + (size_t)slot * a.ncnt;
+            atomicAdd(c, (unsigned)count);
+            if (Wm) atomicAdd(c + 1 + imu, (unsigned)count);
+        }
+        return;
+    }
+    const size_t offPmu = (size_t)a.NK * 4;
+    const size_t offK = offPmu + (size_t)a.NK * Wm;
+    const size_t offKmu = offK + (size_t)a.NK;
+    if (a.do_pl && SUMS)
+        for (int l = 0; l < 4; ++l) atomicAdd(o.bins + (size_t)iK * 4 + l, pl[l]);
+    if (Wm && SUMS) atomicAdd(o.bins + offPmu + (size_t)iK * Wm + imu, d2);
+    if (a.geometry) {
+        const double kc = count * Kamp;
+        atomicAdd(o.bins + offK + iK, kc);
+        if (Wm) atomicAdd(o.bins + offKmu + (size_t)iK * Wm + imu, kc);
+        atomicAdd(o.cnts + iK, (unsigned long long)count);
+        if (Wm) atomicAdd(o.cnts + a.NK + (size_t)iK * Wm + imu, (unsigned long long)count);
+    }
+}
+
+// |k| of a mode in units of KF, sqrt(fi^2 + fj^2 + fk^2) as src/power.c:55
+__device__ __forceinline__ double kamp(int fi, int fj, int fk) {
+#pragma clang fp contract(off)
+    const double a = fi, b = fj, c = fk;
+    return sqrt(a * a + b * b + c * c);
+}
+
+// los . k in the reference's order of operations (src/power.c:56)
+__device__ __forceinline__ double los_dot(const BinArgs &a, int fi, int fj, int fk) {
+#pragma clang fp contract(off)
+    return (double)fi * a.l0 + (double)fj * a.l1 + (double)fk * a.l2;
+}
+
+// per-k factors of the fused power: dual-grid phase along k and the k
+// deconvolution weight
+struct KFac {
+    double pr, pi, w;
+};
+
+// power of one mode of the fast path from the plain (a) and dual (b)
+// spectra: dual times the offset phase (src/fft.c:217-232, here as the
+// product of the (i, j) phase q and the k phase), interlace average
+// (src/fft.c:270-271), deconvolution (src/fft.c:249-253),
+// |delta|^2 / V - Pshot (src/power.c:18-24)
+__device__ __forceinline__ double fused_power(double ar, double ai, double br, double bi, bool dual,
+                                              double qr, double qi, double wij, const KFac &kf,
+                                              double na, double nb, double Vinv, double Pshot) {
+    double dr = ar * na, di = ai * na;
+    if (dual) {
+        const double pr = qr * kf.pr - qi * kf.pi, pim = qr * kf.pi + qi * kf.pr;
+        const double xr = br * nb, xi = bi * nb;
+        const double er = xr * pr - xi * pim, ei = xr * pim + xi * pr;
+        dr = 0.5 * (er + dr);
+        di = 0.5 * (ei + di);
+    }
+    const double w3 = wij * kf.w;
+    dr *= w3;
+    di *= w3;
+    return (dr * dr + di * di) * Vinv - Pshot;
+}
+
+// normalisations of the plain (na) and dual (nb) spectra: norm = V / Np
+// times Np over the grid's painted mass, Re of its packed (0, 0, 0) entry
+// (the DC mode).  The fixed-point paint rounds each of a particle's 64
+// weights to nearest, which paints slightly less than unit mass on average;
+// dividing by the painted mass removes that bias from delta.
+__device__ __forceinline__ void grid_norms(const float *A, const float *B, double norm,
+                                           double npart, double &na, double &nb) {
+    na = nb = norm;
+    const double ma = A[0];
+    if (ma > 0.0) na = norm * (npart / ma);
+    if (B) {
+        const double mb = B[0];
+        if (mb > 0.0) nb = norm * (npart / mb);
+    }
+}
+
+// k-bin range [lo, hi] of modes with |frequency| per axis in [lo_d, hi_d]
+__device__ __forceinline__ void bin_window(const BinArgs &a, int lo0, int hi0, int lo1, int hi1,
+                                           int lo2, int hi2, int &lo, int &hi) {
+    const double kmin = sqrt((double)lo0 * lo0 + (double)lo1 * lo1 + (double)lo2 * lo2);
+    const double kmax = sqrt((double)hi0 * hi0 + (double)hi1 * hi1 + (double)hi2 * hi2);
+    lo = (int)floor(kmin * a.dKinv);
+    hi = (int)floor(kmax * a.dKinv);
+    if (lo > a.NK - 1) lo = a.NK - 1;
+    if (hi > a.NK - 1) hi = a.NK - 1;
+}
+
+__device__ __forceinline__ void window_clear(const BinArgs &a, const BinOut &o) {
+    if (o.windowed) {
+        for (int t = threadIdx.x; t < o.wtot * a.nvals; t += blockDim.x) o.sv[t] = 0.0;
+        for (int t = threadIdx.x; t < o.wtot * a.ncnt; t += blockDim.x) o.sc[t] = 0u;
+    }
+    __syncthreads();
+}
+
+// one global atomic per non-zero LDS window entry
+__device__ __forceinline__ void window_flush(const BinArgs &a, const BinOut &o) {
+    if (!o.windowed) return;
+    __syncthreads();
+    const int Wm = a.Nmu > 0 ? a.Nmu + 1 : 0;
+    const size_t offPmu = (size_t)a.NK * 4;
+    const size_t offK = offPmu + (size_t)a.NK * Wm;
+    const size_t offKmu = offK + (size_t)a.NK;
+    for (int t = threadIdx.x; t < o.wtot * a.nvals; t += blockDim.x) {
+        const double v = o.sv[t];
+        if (v == 0.0) continue;
+        const int w = t / a.nvals, q = t - w * a.nvals;
+        const int iK = w < o.wn ? o.blo + w : o.blo2 + w - o.wn;
+        int qq = q;
+        if (a.do_pl) {
+            if (qq < 4) { atomicAdd(o.bins + (size_t)iK * 4 + qq, v); continue; }
+            qq -= 4;
+        }
+        if (qq < Wm) { atomicAdd(o.bins + offPmu + (size_t)iK * Wm + qq, v); continue; }
+        qq -= Wm;
+        if (qq == 0) { atomicAdd(o.bins + offK + iK, v); continue; }
+        atomicAdd(o.bins + offKmu + (size_t)iK * Wm + (qq - 1), v);
+    }
+    if (a.geometry) {
+        for (int t = threadIdx.x; t < o.wtot * a.ncnt; t += blockDim.x) {
+            const unsigned v = o.sc[t];
+            if (!v) continue;
+            const int w = t / a.ncnt, q = t - w * a.ncnt;
+            const int iK = w < o.wn ? o.blo + w : o.blo2 + w - o.wn;
+            if (q == 0) atomicAdd(o.cnts + iK, (unsigned long long)v);
+            else atomicAdd(o.cnts + a.NK + (size_t)iK * Wm + (q - 1), (unsigned long long)v);
+        }
+    }
+}
+
+__device__ __forceinline__ BinOut make_out(const BinArgs &a, double *sh, int blo, int bhi,
+                                           int blo2, int bhi2, double *bins,
+                                           unsigned long long *cnts) {
+    BinOut o;
+    o.blo = blo;
+    o.wn = bhi - blo + 1;
+    o.blo2 = blo2;
+    o.wtot = o.wn + (bhi2 >= blo2 ? bhi2 - blo2 + 1 : 0);
+    o.windowed = o.wtot <= a.wmax;
+    o.sv = sh;
+    o.sc = (unsigned int *)(sh + (size_t)a.wmax * a.nvals);
+    o.bins = bins;
+    o.cnts = cnts;
+    return o;
+}
+
+// MODE 0: P(k) held in the FP64 padded grid fP (k <= Ng/2);
+// MODE 1: fused fast path over the packed FP32 spectra A, B, modes with
+//         0 < k < Ng/2 (the k = 0 and k = Ng/2 planes: k_bin_plane);
+// MODE 2: geometry only (K sums and mode counts), padded k range.
+// Each workgroup owns a BI x BJ x BK block of modes, one thread per k.
 template <int MODE>
 __global__ void __launch_bounds__(BIN_THREADS)
 k_bin(const double *__restrict__ fP, const float *__restrict__ A, const float *__restrict__ B,
       BinArgs a, const double *__restrict__ ph, const double *__restrict__ winv, double norm,
-      double Vinv, double Pshot, double *__restrict__ bins, unsigned long long *__restrict__ cnts) {
+      double Vinv, double Pshot, double npart, double *__restrict__ bins,
+      unsigned long long *__restrict__ cnts) {
 #pragma clang fp contract(off)
     extern __shared__ double sh[];
     const int Ng = a.Ng;
-    const int Nk = Ng / 2 + 1;
+    const int Mh = Ng / 2;
+    const int Nk = MODE == 1 ? Mh : Mh + 1;      // k columns walked
     const int nbk = (Nk + BK - 1) / BK;
     const int nbj = (Ng + BJ - 1) / BJ;
     int blk = blockIdx.x;
@@ -174,141 +385,105 @@ k_bin(const double *__restrict__ fP, const float *__restrict__ A, const float *_
     const int i0 = bi * BI, j0 = bj * BJ, k0 = bk * BK;
     const int i1 = min(i0 + BI, Ng), j1 = min(j0 + BJ, Ng), k1 = min(k0 + BK, Nk);
 
-    // k-bin window of this block
-    int lo0, hi0, lo1, hi1, lo2, hi2;
+    int lo0, hi0, lo1, hi1, lo2, hi2, blo, bhi;
     axis_range(i0, i1, Ng, lo0, hi0);
     axis_range(j0, j1, Ng, lo1, hi1);
     axis_range(k0, k1, Ng, lo2, hi2);
-    double kmin = sqrt((double)lo0 * lo0 + (double)lo1 * lo1 + (double)lo2 * lo2);
-    double kmax = sqrt((double)hi0 * hi0 + (double)hi1 * hi1 + (double)hi2 * hi2);
-    int blo = (int)floor(kmin * a.dKinv);
-    int bhi = (int)floor(kmax * a.dKinv);
-    if (blo > a.NK - 1) blo = a.NK - 1;
-    if (bhi > a.NK - 1) bhi = a.NK - 1;
-    const int wn = bhi - blo + 1;
-    const bool windowed = wn <= a.wmax;
-    const int Wm = a.Nmu > 0 ? a.Nmu + 1 : 0;
+    bin_window(a, lo0, hi0, lo1, hi1, lo2, hi2, blo, bhi);
+    const BinOut o = make_out(a, sh, blo, bhi, 0, -1, bins, cnts);
+    window_clear(a, o);
 
-    double *sv = sh;                                         // [wmax][nvals]
-    unsigned int *sc = (unsigned int *)(sh + (size_t)a.wmax * a.nvals);  // [wmax][ncnt]
-    if (windowed) {
-        for (int t = threadIdx.x; t < wn * a.nvals; t += blockDim.x) sv[t] = 0.0;
-        for (int t = threadIdx.x; t < wn * a.ncnt; t += blockDim.x) sc[t] = 0u;
+    const int k = k0 + threadIdx.x % BK;
+    const int fk = kval(k, Ng);
+    const int count = (k == 0 || 2 * k == Ng) ? 1 : 2;
+    KFac kf = {1.0, 0.0, 1.0};
+    double na = norm, nb = norm;
+    if (MODE == 1) grid_norms(A, B, norm, npart, na, nb);
+    if (MODE == 1 && k < k1) {
+        kf.pr = ph[2 * (2 * Ng + k)];
+        kf.pi = ph[2 * (2 * Ng + k) + 1];
+        kf.w = winv[k];
     }
-    __syncthreads();
-
-    // global layout offsets
-    const size_t offPmu = (size_t)a.NK * 4;
-    const size_t offK = offPmu + (size_t)a.NK * Wm;
-    const size_t offKmu = offK + (size_t)a.NK;
-
-    const int kk = threadIdx.x % BK;
-    const int k = k0 + kk;
+    const bool live = k < k1 && !(MODE == 1 && k == 0);
     for (int r = threadIdx.x / BK; r < BI * BJ; r += BIN_THREADS / BK) {
         const int i = i0 + r / BJ, j = j0 + r % BJ;
-        if (i >= i1 || j >= j1 || k >= k1) continue;
+        if (!live || i >= i1 || j >= j1) continue;
         if (i == 0 && j == 0 && k == 0) continue;
-        const long m = ((long)i * Ng + j) * Nk + k;
+        const int fi = kval(i, Ng), fj = kval(j, Ng);
         double P = 0.0;
         if (MODE == 0) {
-            P = fP[2 * m];
+            P = fP[2 * (((long)i * Ng + j) * Nk + k)];
         } else if (MODE == 1) {
-            // fused: plain + phase * dual, averaged, deconvolved
+            const long m = ((long)i * Ng + j) * Mh + k;
             const float2 av = ((const float2 *)A)[m];
-            double dr = (double)av.x * norm, di = (double)av.y * norm;
-            if (B) {
-                const double *p0 = ph + 2 * i, *p1 = ph + 2 * (Ng + j), *p2 = ph + 2 * (2 * Ng + k);
-                double pr = p0[0] * p1[0] * p2[0] - p0[0] * p1[1] * p2[1]
-                          - p0[1] * p1[0] * p2[1] - p0[1] * p1[1] * p2[0];
-                double pim = p0[1] * p1[0] * p2[0] + p0[0] * p1[1] * p2[0]
-                           + p0[0] * p1[0] * p2[1] - p0[1] * p1[1] * p2[1];
-                const float2 bv = ((const float2 *)B)[m];
-                double br = (double)bv.x * norm, bim = (double)bv.y * norm;
-                double er = br * pr - bim * pim, ei = br * pim + bim * pr;
-                dr = 0.5 * (er + dr);
-                di = 0.5 * (ei + di);
-            }
-            double w3 = winv[i] * winv[j] * winv[k];
-            dr *= w3;
-            di *= w3;
-            P = (dr * dr + di * di) * Vinv - Pshot;
+            float2 bv = make_float2(0.0f, 0.0f);
+            if (B) bv = ((const float2 *)B)[m];
+            // (i, j) parts of the dual phase and the deconvolution
+            const double *p0 = ph + 2 * i, *p1 = ph + 2 * (Ng + j);
+            const double qr = p0[0] * p1[0] - p0[1] * p1[1], qi = p0[0] * p1[1] + p0[1] * p1[0];
+            P = fused_power(av.x, av.y, bv.x, bv.y, B != nullptr, qr, qi, winv[i] * winv[j], kf,
+                            na, nb, Vinv, Pshot);
         }
-        const int count = 1 + ((2 * k) % Ng > 0);
-        const double d2 = P * count;
-        const double fi = (double)kval(i, Ng), fj = (double)kval(j, Ng), fk = (double)kval(k, Ng);
-        const double Kamp = sqrt(fi * fi + fj * fj + fk * fk);
-        const int iK = (int)floor(Kamp * a.dKinv);
-        if (iK >= a.NK) continue;   // cannot happen for NK of src/power.c:36; keeps LDS in bounds
-        const double dot = fi * a.l0 + fj * a.l1 + fk * a.l2;
-        const double mu = dot / Kamp;
-        int imu = 0;
-        if (a.Nmu > 0) imu = (int)floor(fabs(mu * a.Nmu));
-        double pl[4];
-        const bool sums = MODE != 2;
-        if (a.do_pl) {
-            const double mu2 = mu * mu;
-            pl[0] = d2;
-            pl[1] = d2 * (1.5 * mu2 - 0.5);
-            pl[2] = d2 * ((4.375 * mu2 - 3.75) * mu2 + 0.375);
-            pl[3] = d2 * (((14.4375 * mu2 - 19.6875) * mu2 + 6.5625) * mu2 - 0.3125);
-        }
-        if (windowed) {
-            double *v = sv + (size_t)(iK - blo) * a.nvals;
-            int o = 0;
-            if (a.do_pl) {
-                if (sums)
-                    for (int l = 0; l < 4; ++l) atomicAdd(v + l, pl[l]);
-                o = 4;
-            }
-            if (Wm && sums) atomicAdd(v + o + imu, d2);
-            if (a.geometry) {
-                const double kc = count * Kamp;
-                atomicAdd(v + o + Wm, kc);
-                if (Wm) atomicAdd(v + o + Wm + 1 + imu, kc);
-                unsigned int *c = sc + (size_t)(iK - blo) * a.ncnt;
-                atomicAdd(c, (unsigned)count);
-                if (Wm) atomicAdd(c + 1 + imu, (unsigned)count);
-            }
-        } else {
-            if (a.do_pl && sums)
-                for (int l = 0; l < 4; ++l) atomicAdd(bins + (size_t)iK * 4 + l, pl[l]);
-            if (Wm && sums) atomicAdd(bins + offPmu + (size_t)iK * Wm + imu, d2);
-            if (a.geometry) {
-                const double kc = count * Kamp;
-                atomicAdd(bins + offK + iK, kc);
-                if (Wm) atomicAdd(bins + offKmu + (size_t)iK * Wm + imu, kc);
-                atomicAdd(cnts + iK, (unsigned long long)count);
-                if (Wm) atomicAdd(cnts + a.NK + (size_t)iK * Wm + imu, (unsigned long long)count);
-            }
-        }
+        bin_mode<MODE != 2>(a, o, count, kamp(fi, fj, fk), los_dot(a, fi, fj, fk), P, 0);
     }
-    if (!windowed) return;
-    __syncthreads();
-    for (int t = threadIdx.x; t < wn * a.nvals; t += blockDim.x) {
-        const double v = sv[t];
-        if (v == 0.0) continue;
-        const int w = t / a.nvals, q = t - w * a.nvals;
-        const int iK = blo + w;
-        int qq = q;
-        if (a.do_pl) {
-            if (qq < 4) { atomicAdd(bins + (size_t)iK * 4 + qq, v); continue; }
-            qq -= 4;
+    window_flush(a, o);
+}
+
+// The k = 0 column of the packed spectra holds Z = F(X0) + i F(XN); this
+// kernel separates the k = 0 and k = Ng/2 planes with the mirror mode,
+// X0 = (Z + conj Z(-i,-j)) / 2, XN = (Z - conj Z(-i,-j)) / 2i, and bins both
+// (window 0: k = 0 modes, window 1: k = Ng/2 modes).  One thread per (i, j)
+// of a PI x PJ block.
+#define PI 16
+#define PJ 16
+__global__ void __launch_bounds__(PI * PJ)
+k_bin_plane(const float *__restrict__ A, const float *__restrict__ B, BinArgs a,
+            const double *__restrict__ ph, const double *__restrict__ winv, double norm,
+            double Vinv, double Pshot, double npart, double *__restrict__ bins) {
+#pragma clang fp contract(off)
+    extern __shared__ double sh[];
+    const int Ng = a.Ng;
+    const int Mh = Ng / 2;
+    const int nbj = (Ng + PJ - 1) / PJ;
+    const int i0 = (blockIdx.x / nbj) * PI, j0 = (blockIdx.x % nbj) * PJ;
+    const int i1 = min(i0 + PI, Ng), j1 = min(j0 + PJ, Ng);
+    int lo0, hi0, lo1, hi1, blo, bhi, blo2, bhi2;
+    axis_range(i0, i1, Ng, lo0, hi0);
+    axis_range(j0, j1, Ng, lo1, hi1);
+    bin_window(a, lo0, hi0, lo1, hi1, 0, 0, blo, bhi);
+    bin_window(a, lo0, hi0, lo1, hi1, Mh, Mh, blo2, bhi2);
+    const BinOut o = make_out(a, sh, blo, bhi, blo2, bhi2, bins, nullptr);
+    window_clear(a, o);
+    const int i = i0 + threadIdx.x / PJ, j = j0 + threadIdx.x % PJ;
+    double na, nb;
+    grid_norms(A, B, norm, npart, na, nb);
+    if (i < i1 && j < j1) {
+        const int im = i ? Ng - i : 0, jm = j ? Ng - j : 0;
+        const long m = ((long)i * Ng + j) * Mh, mm = ((long)im * Ng + jm) * Mh;
+        const float2 av = ((const float2 *)A)[m], am = ((const float2 *)A)[mm];
+        float2 bv = make_float2(0.0f, 0.0f), bm = bv;
+        if (B) {
+            bv = ((const float2 *)B)[m];
+            bm = ((const float2 *)B)[mm];
         }
-        if (qq < Wm) { atomicAdd(bins + offPmu + (size_t)iK * Wm + qq, v); continue; }
-        qq -= Wm;
-        if (qq == 0) { atomicAdd(bins + offK + iK, v); continue; }
-        atomicAdd(bins + offKmu + (size_t)iK * Wm + (qq - 1), v);
-    }
-    if (a.geometry) {
-        for (int t = threadIdx.x; t < wn * a.ncnt; t += blockDim.x) {
-            const unsigned v = sc[t];
-            if (!v) continue;
-            const int w = t / a.ncnt, q = t - w * a.ncnt;
-            const int iK = blo + w;
-            if (q == 0) atomicAdd(cnts + iK, (unsigned long long)v);
-            else atomicAdd(cnts + a.NK + (size_t)iK * Wm + (q - 1), (unsigned long long)v);
+        const double *p0 = ph + 2 * i, *p1 = ph + 2 * (Ng + j);
+        const double qr = p0[0] * p1[0] - p0[1] * p1[1], qi = p0[0] * p1[1] + p0[1] * p1[0];
+        const double wij = winv[i] * winv[j];
+        const int fi = kval(i, Ng), fj = kval(j, Ng), fn = kval(Mh, Ng);
+        const KFac k0f = {ph[2 * (2 * Ng)], ph[2 * (2 * Ng) + 1], winv[0]};
+        const KFac knf = {ph[2 * (2 * Ng + Mh)], ph[2 * (2 * Ng + Mh) + 1], winv[Mh]};
+        if (i != 0 || j != 0) {   // (0, 0, 0) is the DC mode
+            const double P = fused_power(0.5 * ((double)av.x + am.x), 0.5 * ((double)av.y - am.y),
+                                         0.5 * ((double)bv.x + bm.x), 0.5 * ((double)bv.y - bm.y),
+                                         B != nullptr, qr, qi, wij, k0f, na, nb, Vinv, Pshot);
+            bin_mode<true>(a, o, 1, kamp(fi, fj, 0), los_dot(a, fi, fj, 0), P, 0);
         }
+        const double P = fused_power(0.5 * ((double)av.y + am.y), -0.5 * ((double)av.x - am.x),
+                                     0.5 * ((double)bv.y + bm.y), -0.5 * ((double)bv.x - bm.x),
+                                     B != nullptr, qr, qi, wij, knf, na, nb, Vinv, Pshot);
+        bin_mode<true>(a, o, 1, kamp(fi, fj, fn), los_dot(a, fi, fj, fn), P, 1);
     }
+    window_flush(a, o);
 }
 
 static BinArgs make_args(const fgk_binspec *b) {
@@ -331,8 +506,8 @@ static BinArgs make_args(const fgk_binspec *b) {
     return a;
 }
 
-static unsigned bin_blocks(int Ng) {
-    const int Nk = Ng / 2 + 1;
+// Nk: k columns walked (Ng/2 + 1 padded layout, Ng/2 packed)
+static unsigned bin_blocks(int Ng, int Nk) {
     long n = (long)((Ng + BI - 1) / BI) * ((Ng + BJ - 1) / BJ) * ((Nk + BK - 1) / BK);
     return (unsigned)n;
 }
@@ -344,9 +519,10 @@ static size_t bin_lds(const BinArgs &a) {
 extern "C" int fgk_bin_pgrid_f64(const double *f, const fgk_binspec *b, double *bins_dev,
                                  int64_t *counts_dev, fgk_stream_t s) {
     BinArgs a = make_args(b);
-    hipLaunchKernelGGL(k_bin<0>, dim3(bin_blocks(b->Ng)), dim3(BIN_THREADS), bin_lds(a),
-                       (hipStream_t)s, f, (const float *)nullptr, (const float *)nullptr, a,
-                       (const double *)nullptr, (const double *)nullptr, 0.0, 0.0, 0.0, bins_dev,
+    hipLaunchKernelGGL(k_bin<0>, dim3(bin_blocks(b->Ng, b->Ng / 2 + 1)), dim3(BIN_THREADS),
+                       bin_lds(a), (hipStream_t)s, f, (const float *)nullptr,
+                       (const float *)nullptr, a, (const double *)nullptr,
+                       (const double *)nullptr, 0.0, 0.0, 0.0, 0.0, bins_dev,
                        (unsigned long long *)counts_dev);
     FGK_LAUNCHED();
     return 0;
@@ -359,23 +535,32 @@ extern "C" int fgk_bin_geometry(const fgk_binspec *b, double *bins_dev, int64_t 
         return -2;
     }
     BinArgs a = make_args(b);
-    hipLaunchKernelGGL(k_bin<2>, dim3(bin_blocks(b->Ng)), dim3(BIN_THREADS), bin_lds(a),
-                       (hipStream_t)s, (const double *)nullptr, (const float *)nullptr,
-                       (const float *)nullptr, a, (const double *)nullptr,
-                       (const double *)nullptr, 0.0, 0.0, 0.0, bins_dev,
-                       (unsigned long long *)counts_dev);
+    hipLaunchKernelGGL(k_bin<2>, dim3(bin_blocks(b->Ng, b->Ng / 2 + 1)), dim3(BIN_THREADS),
+                       bin_lds(a), (hipStream_t)s, (const double *)nullptr,
+                       (const float *)nullptr, (const float *)nullptr, a,
+                       (const double *)nullptr, (const double *)nullptr, 0.0, 0.0, 0.0, 0.0,
+                       bins_dev, (unsigned long long *)counts_dev);
     FGK_LAUNCHED();
     return 0;
 }
 
 extern "C" int fgk_fused_bin_f32(const float *A, const float *B, const fgk_binspec *b,
                                  const double *phase_dev, const double *winv_dev, double norm,
-                                 double Vinv, double Pshot, double *bins_dev, int64_t *counts_dev,
+                                 double Vinv, double Pshot, double npart, double *bins_dev,
                                  fgk_stream_t s) {
+    if (b->geometry) {
+        fgk_set_error("fgk_fused_bin_f32: geometry is computed by fgk_bin_geometry");
+        return -2;
+    }
     BinArgs a = make_args(b);
-    hipLaunchKernelGGL(k_bin<1>, dim3(bin_blocks(b->Ng)), dim3(BIN_THREADS), bin_lds(a),
+    const int Ng = b->Ng;
+    hipLaunchKernelGGL(k_bin<1>, dim3(bin_blocks(Ng, Ng / 2)), dim3(BIN_THREADS), bin_lds(a),
                        (hipStream_t)s, (const double *)nullptr, A, B, a, phase_dev, winv_dev,
-                       norm, Vinv, Pshot, bins_dev, (unsigned long long *)counts_dev);
+                       norm, Vinv, Pshot, npart, bins_dev, (unsigned long long *)nullptr);
+    FGK_LAUNCHED();
+    const unsigned nplane = (unsigned)(((Ng + PI - 1) / PI) * ((Ng + PJ - 1) / PJ));
+    hipLaunchKernelGGL(k_bin_plane, dim3(nplane), dim3(PI * PJ), bin_lds(a), (hipStream_t)s, A, B,
+                       a, phase_dev, winv_dev, norm, Vinv, Pshot, npart, bins_dev);
     FGK_LAUNCHED();
     return 0;
 }

@@ -27,7 +27,11 @@
 //    with global integer atomics.
 //    Integer addition is associative, so the grids are bit-identical run to
 //    run whatever the scheduling.  Each weight is rounded once to 2^-S, S
-//    chosen so that cap particles cannot overflow int32.
+//    chosen so that cap particles cannot overflow int32.  Round to nearest
+//    is unbiased for large terms, but most PCS products are small and those
+//    below half a unit always round down: a particle paints on average about
+//    1 - 2^-(S+1) of its unit mass, a deficit the k-space kernel removes by
+//    normalising each grid by its painted total (its DC mode).
 //    Grids with more tiles than the two levels can address fall back to a
 //    single-level sort whose ranks come from global atomics.
 #include <hip/hip_runtime.h>

@@ -2,7 +2,7 @@
  *
  * One fftgal_power_t holds everything a catalog needs on the device:
  *   int64 fixed-point grids  gA (plain) and gB (dual)     Ng^3 each
- *   FP32 spectra             fA, fB  (padded r2c layout)  Ng^2 (Ng+2) each
+ *   FP32 spectra             fA, fB  (packed half spectrum) Ng^3 floats each
  *   paint plan (sort buffers), FFT plan (FP32 twiddles from FP64),
  *   offset-phase and deconvolution tables (FP64, built as src/fft.c:204-245),
  *   bin sums (FP64) and the geometry of the bins (K sums, mode counts),
@@ -179,7 +179,7 @@ int fftgal_power_create(fftgal_power_t **out, const fftgal_power_desc *desc) {
     }
     STEP(fgk_paint_plan_create(&p->paint, Ng, p->tune.paint_tile, desc->Np_max, desc->interlace));
     const size_t n64 = (size_t)Ng * Ng * Ng * sizeof(int64_t);
-    const size_t nf = (size_t)Ng * Ng * (2 * (Ng / 2 + 1)) * sizeof(float);
+    const size_t nf = (size_t)Ng * Ng * Ng * sizeof(float);   /* packed half spectrum */
     STEP(fgk_malloc((void **)&p->gA, n64));
     STEP(fgk_malloc((void **)&p->fA, nf));
     if (desc->interlace) {
@@ -286,7 +286,7 @@ static int enqueue_work(fftgal_power_t *p, const float *x, const float *y, const
     }
     MARK();
     TRY(fgk_fused_bin_f32(p->fA, p->d.interlace ? p->fB : NULL, &p->spec, p->d_phase, p->d_winv,
-                          norm, Vinv, Pshot, p->d_bins, NULL, p->s));
+                          norm, Vinv, Pshot, (double)Np, p->d_bins, p->s));
     MARK();
     TRY(fgk_d2h(p->h_bins, p->d_bins, p->nbins * sizeof(double), p->s));
     MARK();
@@ -442,6 +442,16 @@ int fftgal_power_tuning(const fftgal_power_t *p, int *paint_tile, int *fft_cols,
 int fftgal_power_use_graph(fftgal_power_t *p, int on) {
     p->use_graph = on ? 1 : 0;
     return 0;
+}
+
+int fftgal_power_fft_generic(fftgal_power_t *p, int on) {
+    TRY(fgk_stream_sync(p->s));
+    if (p->graph) {   /* the captured graph holds the other kernels */
+        fgk_graph_destroy(p->graph);
+        p->graph = NULL;
+    }
+    TRY(fgk_fft_force_generic(p->plan, on));
+    return fgk_fft_f32_specialised(p->d.Ng) && !on ? 0 : 1;
 }
 
 void *fftgal_power_stream(fftgal_power_t *p) { return p->s; }

@@ -69,12 +69,22 @@ int fftgal_power_results(fftgal_power_t *p, double *K, long *N, double *Pl, doub
 int fftgal_power_paint_only(fftgal_power_t *p, const float *x, const float *y, const float *z,
                             long Np, void **gridA, void **gridB);
 int fftgal_power_fixed_bits(const fftgal_power_t *p);
-/* device address of the FP32 spectra (plain, dual), padded layout */
+/* device address of the FP32 spectra (plain, dual) after an enqueue, in
+ * the packed half-spectrum layout c[(i*Ng+j)*(Ng/2) + k] (complex float,
+ * k < Ng/2, unnormalised forward transform of the fixed-point grid in
+ * particle units): column k = 0 holds Z = F(X0) + i F(XN), from which
+ * delta(i,j,0) = (Z[i,j] + conj Z[-i,-j]) / 2 and
+ * delta(i,j,Ng/2) = (Z[i,j] - conj Z[-i,-j]) / 2i */
 int fftgal_power_spectra(fftgal_power_t *p, void **A, void **B);
 /* tuning in use */
 int fftgal_power_tuning(const fftgal_power_t *p, int *paint_tile, int *fft_cols, int *fft_rows);
 /* use the hipGraph (1, default) or plain stream launches (0) */
 int fftgal_power_use_graph(fftgal_power_t *p, int on);
+/* FFT kernels: on = 1 forces the generic LDS Stockham kernels, on = 0
+ * restores the default (compile-time specialised kernels where the grid
+ * side has them).  Returns 1 if the generic kernels are now in use, 0 if
+ * the specialised ones, < 0 on error. */
+int fftgal_power_fft_generic(fftgal_power_t *p, int on);
 /* the plan's stream (hipStream_t) */
 void *fftgal_power_stream(fftgal_power_t *p);
 /* run one catalog eagerly with events between the pipeline stages and
