@@ -20,11 +20,13 @@
 //    19-bit tile-local fixed-point coordinate per axis (2^-15 cell for
 //    T = 16).  The paint then runs one workgroup per work item (<= cap
 //    particles of one tile): the 4x4x4 stencils of both grids go into LDS
-//    footprints ((T+3)^3 plain, (T+4)^3 dual) as int32 fixed point, each
-//    lane walking the 16 stencil rows in a lane-dependent order so that
-//    lanes holding particles of the same dense cell hit different
-//    addresses; non-zero footprint cells are flushed into the int64 grids
-//    with global integer atomics.
+//    footprints ((T+3)^3 plain, (T+4)^3 dual cells, rows padded to a stride
+//    of 20 so that the 64 offsets of a stencil fall in 64 distinct banks) as
+//    int32 fixed point.  Each lane walks its stencil in one of 64 rotated
+//    orders (the lane's six low bits rotate a, b and c), so lanes holding
+//    particles of the same dense cell never hit the same address together.
+//    Non-zero footprint cells are flushed into the int64 grids with global
+//    integer atomics.
 //    Integer addition is associative, so the grids are bit-identical run to
 //    run whatever the scheduling.  Each weight is rounded once to 2^-S, S
 //    chosen so that cap particles cannot overflow int32.  Round to nearest
@@ -499,18 +501,31 @@ __device__ __forceinline__ float sel4(const float w[4], int a) {
     return (a & 2) ? ((a & 1) ? w[3] : w[2]) : ((a & 1) ? w[1] : w[0]);
 }
 
+// o[q] = w[(q + r) & 3] as a two-level barrel shift with static indices
+// (a select on a lane-dependent index would be lowered to a scratch array)
+__device__ __forceinline__ void rot4(const float w[4], int r, float o[4]) {
+    const bool r0 = r & 1, r1 = r & 2;
+    const float t0 = r0 ? w[1] : w[0], t1 = r0 ? w[2] : w[1];
+    const float t2 = r0 ? w[3] : w[2], t3 = r0 ? w[0] : w[3];
+    o[0] = r1 ? t2 : t0;
+    o[1] = r1 ? t3 : t1;
+    o[2] = r1 ? t0 : t2;
+    o[3] = r1 ? t1 : t3;
+}
+
 // round-to-nearest-even of a*b (|a*b| < 2^22) with one FMA: adding
 // 1.5*2^23 puts the rounded integer in the low mantissa bits
 __device__ __forceinline__ int fixed_rne(float a, float b) {
     return __float_as_int(__builtin_fmaf(a, b, 12582912.0f)) - 0x4B400000;
 }
 
-// Flush a (side^3) LDS footprint whose cell (0,0,0) sits at grid cell
-// (oi, oj, ok) into the int64 grid with one integer atomic per non-zero cell.
-// Each wave takes groups of 64/side whole rows along k; a lane's k offset is
-// fixed, so the periodic wrap needs no division (footprints extend at most
-// 2 cells below 0 and side cells past the grid's last index).
-__device__ void flush_footprint(const int *__restrict__ fp, int side, int oi, int oj, int ok,
+// Flush an LDS footprint of side^3 cells (cell (i,j,k) at (i*P + j)*P + k)
+// whose cell (0,0,0) sits at grid cell (oi, oj, ok) into the int64 grid
+// with one integer atomic per non-zero cell.  Each wave takes groups of
+// 64/side whole rows along k; a lane's k offset is fixed, so the periodic
+// wrap needs no division (footprints extend at most 2 cells below 0 and
+// side cells past the grid's last index).
+__device__ void flush_footprint(const int *__restrict__ fp, int side, int P, int oi, int oj, int ok,
                                 int Ng, unsigned long long *__restrict__ G) {
     const int lane = __lane_id();
     const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
@@ -523,9 +538,9 @@ __device__ void flush_footprint(const int *__restrict__ fp, int side, int oi, in
     if (gk >= Ng) gk %= Ng;
     const int nrows = side * side;
     for (int r = wv * rpg + ro; r < nrows; r += nwv * rpg) {
-        const int v = fp[r * side + fk];
-        if (!v) continue;
         const int fi = r / side, fj = r - fi * side;
+        const int v = fp[(fi * P + fj) * P + fk];
+        if (!v) continue;
         int gi = oi + fi, gj = oj + fj;
         gi = gi < 0 ? gi + Ng : (gi >= Ng ? gi - Ng : gi);
         gj = gj < 0 ? gj + Ng : (gj >= Ng ? gj - Ng : gj);
@@ -552,26 +567,105 @@ __device__ __forceinline__ bool item_range(long it, const uint32_t *__restrict__
     return true;
 }
 
-// One particle's 4x4x4 stencil into an LDS footprint of side SIDE (a
-// compile-time constant, so every atomic uses an immediate LDS offset).
-// Rows a are visited in the lane-dependent order (a + ph) & 3, so lanes
-// holding particles of one dense cell are spread over 4 different rows.
-template <int SIDE>
+#ifndef PAINT_ROT
+#define PAINT_ROT 2
+#endif
+
+// LDS row stride of the footprints: with P = 20 (12 for small tiles) the 64
+// stencil offsets a*P*P + b*P + c fall in 64 distinct LDS banks, as do the
+// cells of any 4x4x4 block
+constexpr int paint_stride(int T) { return T >= 12 ? 20 : 12; }
+
+// One particle's 4x4x4 stencil into an LDS footprint with strides P*P, P
+// (compile-time, so row offsets are immediates).  Lanes walk the stencil
+// in rotated orders so that lanes holding particles of one dense cell hit
+// different addresses: ROT 0 rotates a by two lane bits (groups of 16
+// lanes), ROT 1 rotates a and b by lane bits 0-1 and 2-3, ROT 2 also c by
+// bits 4-5 (64 orders; c offsets then come from registers).
+template <int P, int ROT>
 __device__ __forceinline__ void stencil_lds(int *__restrict__ fp, int bi, int bj, int bk,
                                             const float wx[4], const float wy[4],
-                                            const float wzs[4], int ph) {
-    int *base = fp + (bi * SIDE + bj) * SIDE + bk;
+                                            const float wzs[4], int lane) {
+    int *base = fp + (bi * P + bj) * P + bk;
+    if (ROT == 0) {
+        const int ph = (lane >> 4) & 3;
+#pragma unroll
+        for (int aa = 0; aa < 4; ++aa) {
+            const int a = (aa + ph) & 3;
+            const float wa = sel4(wx, a);
+            int *pa = base + a * P * P;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const float wab = wa * wy[b];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) atomicAdd(pa + b * P + c, fixed_rne(wab, wzs[c]));
+            }
+        }
+        return;
+    }
+    const int ra = lane & 3, rb = (lane >> 2) & 3, rc = (lane >> 4) & 3;
+    float xr[4], yr[4], zr[4];
+    int oa[4], ob[4], oc[4];
+    rot4(wx, ra, xr);
+    rot4(wy, rb, yr);
+    if (ROT == 2) rot4(wzs, rc, zr);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        oa[q] = ((q + ra) & 3) * P * P;
+        ob[q] = ((q + rb) & 3) * P;
+        oc[q] = (q + rc) & 3;
+    }
 #pragma unroll
     for (int aa = 0; aa < 4; ++aa) {
-        const int a = (aa + ph) & 3;
-        const float wa = sel4(wx, a);
-        int *pa = base + a * SIDE * SIDE;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const float wab = wa * wy[b];
+        for (int bb = 0; bb < 4; ++bb) {
+            const float wab = xr[aa] * yr[bb];
+            int *row = base + oa[aa] + ob[bb];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) atomicAdd(pa + b * SIDE + c, fixed_rne(wab, wzs[c]));
+            for (int cc = 0; cc < 4; ++cc) {
+                if (ROT == 2) atomicAdd(row + oc[cc], fixed_rne(wab, zr[cc]));
+                else atomicAdd(row + cc, fixed_rne(wab, wzs[cc]));
+            }
         }
+    }
+}
+
+// both grids' stencils of one packed particle word into the LDS footprints
+template <int P, int ROT>
+__device__ __forceinline__ void paint_particle(int *__restrict__ fa, int *__restrict__ fb,
+                                               uint64_t pk, int F, float fsc, float wscale,
+                                               int interlace, int lane) {
+    const uint32_t q[3] = {(uint32_t)(pk >> (2 * POS_BITS)) & POS_MASK,
+                           (uint32_t)(pk >> POS_BITS) & POS_MASK, (uint32_t)pk & POS_MASK};
+    int li[3];
+    float d[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        li[a] = (int)(q[a] >> F);
+        d[a] = (float)(q[a] & ((1u << F) - 1u)) * fsc;
+    }
+    float wx[4], wy[4], wz[4], wzs[4];
+    pcs_w_f32(d[0], wx);
+    pcs_w_f32(d[1], wy);
+    pcs_w_f32(d[2], wz);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * wscale;
+    // plain grid: nodes li-1..li+2, footprint origin (tile origin - 1)
+    stencil_lds<P, ROT>(fa, li[0], li[1], li[2], wx, wy, wzs, lane);
+    if (interlace) {
+        // dual grid (offset 1/2): base li or li-1, footprint origin (tile origin - 2)
+        int ob[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (d[a] >= 0.5f) { d[a] -= 0.5f; ob[a] = li[a] + 1; }
+            else { d[a] += 0.5f; ob[a] = li[a]; }
+        }
+        pcs_w_f32(d[0], wx);
+        pcs_w_f32(d[1], wy);
+        pcs_w_f32(d[2], wz);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * wscale;
+        stencil_lds<P, ROT>(fb, ob[0], ob[1], ob[2], wx, wy, wzs, lane);
     }
 }
 
@@ -584,8 +678,8 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
               const uint32_t *__restrict__ items, long nslots, KeyGeo g, int nb2, float wscale,
               int cap, int interlace, unsigned long long *__restrict__ GA,
               unsigned long long *__restrict__ GB) {
-    constexpr int SA = T + 3, SB = T + 4;
-    constexpr int NA = SA * SA * SA, NB = SB * SB * SB;
+    constexpr int SA = T + 3, SB = T + 4, P = paint_stride(T);
+    constexpr int NA = SA * P * P, NB = SB * P * P;
     extern __shared__ int lds[];
     long t, p0, p1;
     if (!item_range(blockIdx.x, tilebase, items, nslots, cap, t, p0, p1)) return;
@@ -596,44 +690,13 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
 
     const int F = g.F;
     const float fsc = 1.0f / (float)(1 << F);
-    const int ph = (threadIdx.x >> 4) & 3;
+    const int lane = __lane_id();
     long p = p0 + threadIdx.x;
     uint64_t nextpk = p < p1 ? sorted[p] : 0ull;
     for (; p < p1; p += PAINT_THREADS) {
         const uint64_t pk = nextpk;
         if (p + PAINT_THREADS < p1) nextpk = sorted[p + PAINT_THREADS];
-        const uint32_t q[3] = {(uint32_t)(pk >> (2 * POS_BITS)) & POS_MASK,
-                               (uint32_t)(pk >> POS_BITS) & POS_MASK, (uint32_t)pk & POS_MASK};
-        int li[3];
-        float d[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            li[a] = (int)(q[a] >> F);
-            d[a] = (float)(q[a] & ((1u << F) - 1u)) * fsc;
-        }
-        float wx[4], wy[4], wz[4], wzs[4];
-        pcs_w_f32(d[0], wx);
-        pcs_w_f32(d[1], wy);
-        pcs_w_f32(d[2], wz);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * wscale;
-        // plain grid: nodes li-1..li+2, footprint origin (tile origin - 1)
-        stencil_lds<SA>(fa, li[0], li[1], li[2], wx, wy, wzs, ph);
-        if (interlace) {
-            // dual grid (offset 1/2): base li or li-1, footprint origin (tile origin - 2)
-            int ob[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                if (d[a] >= 0.5f) { d[a] -= 0.5f; ob[a] = li[a] + 1; }
-                else { d[a] += 0.5f; ob[a] = li[a]; }
-            }
-            pcs_w_f32(d[0], wx);
-            pcs_w_f32(d[1], wy);
-            pcs_w_f32(d[2], wz);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * wscale;
-            stencil_lds<SB>(fb, ob[0], ob[1], ob[2], wx, wy, wzs, ph);
-        }
+        paint_particle<P, PAINT_ROT>(fa, fb, pk, F, fsc, wscale, interlace, lane);
     }
     __syncthreads();
     const long b1 = t / nb2, f = t - b1 * nb2;
@@ -642,15 +705,15 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
     const int fx = (int)(f / (g.cg1 * g.cg2)), fy = (int)((f / g.cg2) % g.cg1),
               fz = (int)(f % g.cg2);
     const int ti = cx * g.cg0 + fx, tj = cy * g.cg1 + fy, tk = cz * g.cg2 + fz;
-    flush_footprint(fa, SA, ti * T - 1, tj * T - 1, tk * T - 1, g.Ng, GA);
-    if (interlace) flush_footprint(fb, SB, ti * T - 2, tj * T - 2, tk * T - 2, g.Ng, GB);
+    flush_footprint(fa, SA, P, ti * T - 1, tj * T - 1, tk * T - 1, g.Ng, GA);
+    if (interlace) flush_footprint(fb, SB, P, ti * T - 2, tj * T - 2, tk * T - 2, g.Ng, GB);
 }
 
 // ------------------------------------------------------------------ host
 
 static int paint_lds_bytes(int T, int interlace) {
-    int a = (T + 3) * (T + 3) * (T + 3);
-    int b = interlace ? (T + 4) * (T + 4) * (T + 4) : 0;
+    const int P = paint_stride(T);
+    const int a = (T + 3) * P * P, b = interlace ? (T + 4) * P * P : 0;
     return (a + b) * 4;
 }
 
