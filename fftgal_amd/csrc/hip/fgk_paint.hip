@@ -204,14 +204,13 @@ __device__ __forceinline__ void axis_key(float xf, const KeyGeo &g, int &tc, uin
     q = (uint32_t)(qq < qmax ? qq : qmax);
 }
 
-__device__ __forceinline__ void particle_key(const float *x, const float *y, const float *z,
-                                             long p, const KeyGeo &g, uint32_t &coarse,
-                                             uint64_t &pk) {
+__device__ __forceinline__ void particle_key(float xv, float yv, float zv, const KeyGeo &g,
+                                             uint32_t &coarse, uint64_t &pk) {
     int tx, ty, tz;
     uint32_t qx, qy, qz;
-    axis_key(x[p], g, tx, qx);
-    axis_key(y[p], g, ty, qy);
-    axis_key(z[p], g, tz, qz);
+    axis_key(xv, g, tx, qx);
+    axis_key(yv, g, ty, qy);
+    axis_key(zv, g, tz, qz);
     const int cx = tx >> g.lg0, cy = ty >> g.lg1, cz = tz >> g.lg2;
     coarse = (uint32_t)((cx * g.nc1 + cy) * g.nc2 + cz);
     const uint32_t fine = (uint32_t)((((tx & (g.cg0 - 1)) << g.lg1 | (ty & (g.cg1 - 1))) << g.lg2) |
@@ -221,19 +220,25 @@ __device__ __forceinline__ void particle_key(const float *x, const float *y, con
 }
 
 // Rank of this lane among the valid lanes of its wave holding the same
-// digit, added to the workgroup histogram with one LDS atomic per distinct
-// digit per wave.  All lanes of the wave must call it together.
-__device__ __forceinline__ uint32_t wave_rank(uint32_t d, bool valid, int nbits,
-                                              uint32_t *hist) {
+// NB-bit digit, added to the workgroup histogram with one LDS atomic per
+// distinct digit per wave (RANK = false: the histogram only).  All lanes of
+// the wave must call it together.
+template <int NB, bool RANK>
+__device__ __forceinline__ uint32_t wave_rank(uint32_t d, bool valid, uint32_t *hist) {
     uint64_t peers = __ballot(valid);
-    for (int b = 0; b < nbits; ++b) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
         const bool bit = (d >> b) & 1u;
         const uint64_t bal = __ballot(valid && bit);
         peers &= bit ? bal : ~bal;
     }
     const int lane = __lane_id();
-    const uint32_t below = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
     const int leader = __ffsll((unsigned long long)peers) - 1;
+    if (!RANK) {
+        if (valid && lane == leader) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
+        return 0;
+    }
+    const uint32_t below = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
     uint32_t base = 0;
     if (valid && lane == leader) base = atomicAdd(&hist[d], (uint32_t)__popcll(peers));
     base = __shfl(base, leader < 0 ? 0 : leader);
@@ -288,10 +293,11 @@ __global__ void __launch_bounds__(SORT_THREADS)
 k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
        long Np, KeyGeo g, const uint64_t *__restrict__ in, const uint32_t *__restrict__ chunk0,
        const uint32_t *__restrict__ base1, const uint32_t *__restrict__ T1, int nb1, int nd,
-       int nbits, uint32_t *__restrict__ H, const uint32_t *__restrict__ O,
-       uint64_t *__restrict__ out) {
+       uint32_t *__restrict__ H, const uint32_t *__restrict__ O, uint64_t *__restrict__ out) {
+    constexpr int NB = LEVEL == 1 ? 8 : 7;       // digit bits: MAX_D1, MAX_D2 digits
     __shared__ uint32_t hist[MAX_D1];
     __shared__ uint32_t off[MAX_D1];
+    __shared__ uint32_t dbase[COUNT ? 1 : MAX_D1];
     __shared__ uint64_t stage[COUNT ? 1 : SORT_CHUNK];
     __shared__ uint8_t sdig[COUNT ? 1 : SORT_CHUNK];
     __shared__ long s_start;
@@ -316,6 +322,22 @@ k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__
     const int n = s_n;
     uint32_t dk[SORT_PER_THREAD], rk[SORT_PER_THREAD];
     uint64_t vk[SORT_PER_THREAD];
+    float xs[LEVEL == 1 ? SORT_PER_THREAD : 1], ys[LEVEL == 1 ? SORT_PER_THREAD : 1],
+        zs[LEVEL == 1 ? SORT_PER_THREAD : 1];
+    // all loads first, so that they are in flight together
+#pragma unroll
+    for (int k = 0; k < SORT_PER_THREAD; ++k) {
+        const int i = k * SORT_THREADS + threadIdx.x;
+        if (i < n) {
+            if (LEVEL == 1) {
+                xs[k] = x[st + i];
+                ys[k] = y[st + i];
+                zs[k] = z[st + i];
+            } else {
+                vk[k] = in[st + i];
+            }
+        }
+    }
 #pragma unroll
     for (int k = 0; k < SORT_PER_THREAD; ++k) {
         const int i = k * SORT_THREADS + threadIdx.x;
@@ -324,13 +346,13 @@ k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__
         uint64_t pk = 0;
         if (valid) {
             if (LEVEL == 1) {
-                particle_key(x, y, z, st + i, g, d, pk);
+                particle_key(xs[k], ys[k], zs[k], g, d, pk);
             } else {
-                pk = in[st + i];
+                pk = vk[k];
                 d = (uint32_t)(pk >> FINE_SHIFT);
             }
         }
-        rk[k] = wave_rank(d, valid, nbits, hist);
+        rk[k] = wave_rank<NB, !COUNT>(d, valid, hist);
         dk[k] = d;
         vk[k] = pk;
     }
@@ -341,6 +363,8 @@ k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__
     }
     small_exclusive_scan(hist, off, nd);
     __syncthreads();
+    // global destination of staged position pos with digit d: dbase[d] + pos
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) dbase[i] = O[j * nd + i] - off[i];
 #pragma unroll
     for (int k = 0; k < SORT_PER_THREAD; ++k) {
         const int i = k * SORT_THREADS + threadIdx.x;
@@ -351,11 +375,8 @@ k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__
         }
     }
     __syncthreads();
-    const uint32_t *orow = O + j * nd;
-    for (int pos = threadIdx.x; pos < n; pos += blockDim.x) {
-        const uint32_t d = sdig[pos];
-        out[orow[d] + (pos - off[d])] = stage[pos];
-    }
+    for (int pos = threadIdx.x; pos < n; pos += blockDim.x)
+        out[dbase[sdig[pos]] + pos] = stage[pos];
 }
 
 // S[b][d] = sum of H[j][d] over the RB chunks j of block b
@@ -383,6 +404,35 @@ __global__ void k_segscan(const uint32_t *__restrict__ S, const uint32_t *__rest
         run += S[b * nd + d];
     }
     T[t] = run;
+}
+
+// the same for one long segment (all blocks [0, nblk)): one workgroup per
+// digit d scans S[.][d] 256 blocks at a time, carrying the running total
+__global__ void __launch_bounds__(256)
+k_segscan_wide(const uint32_t *__restrict__ S, long nblk, int nd, uint32_t *__restrict__ B,
+               uint32_t *__restrict__ T) {
+    __shared__ uint32_t wsum[4];
+    const int d = blockIdx.x;
+    const int lane = __lane_id(), wv = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (long b0 = 0; b0 < nblk; b0 += 256) {
+        const long b = b0 + threadIdx.x;
+        const uint32_t v = b < nblk ? S[b * nd + d] : 0u;
+        uint32_t incl = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wsum[wv] = incl;
+        __syncthreads();
+        uint32_t before = carry;
+        for (int w = 0; w < wv; ++w) before += wsum[w];
+        if (b < nblk) B[b * nd + d] = before + incl - v;
+        carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) T[d] = carry;
 }
 
 // single-workgroup exclusive scan of v[0..n) into base[0..n] and of an
@@ -472,7 +522,7 @@ __global__ void k_count_global(const float *__restrict__ x, const float *__restr
     if (p >= Np) return;
     uint32_t t;
     uint64_t pk;
-    particle_key(x, y, z, p, g, t, pk);
+    particle_key(x[p], y[p], z[p], g, t, pk);
     rank[p] = atomicAdd(&tot[t], 1u);
 }
 
@@ -484,7 +534,7 @@ __global__ void k_scatter_global(const float *__restrict__ x, const float *__res
     if (p >= Np) return;
     uint32_t t;
     uint64_t pk;
-    particle_key(x, y, z, p, g, t, pk);
+    particle_key(x[p], y[p], z[p], g, t, pk);
     out[base[t] + rank[p]] = pk;
 }
 
@@ -630,19 +680,31 @@ __device__ __forceinline__ void stencil_lds(int *__restrict__ fp, int bi, int bj
     }
 }
 
-// both grids' stencils of one packed particle word into the LDS footprints
-template <int P, int ROT>
-__device__ __forceinline__ void paint_particle(int *__restrict__ fa, int *__restrict__ fb,
-                                               uint64_t pk, int F, float fsc, float wscale,
-                                               int interlace, int lane) {
-    const uint32_t q[3] = {(uint32_t)(pk >> (2 * POS_BITS)) & POS_MASK,
-                           (uint32_t)(pk >> POS_BITS) & POS_MASK, (uint32_t)pk & POS_MASK};
+// tile-local cell li and fraction d of one axis of a packed particle word
+__device__ __forceinline__ void unpack_axis(uint32_t q, int F, float fsc, int &li, float &d) {
+    li = (int)(q >> F);
+    d = (float)(q & ((1u << F) - 1u)) * fsc;
+}
+
+__device__ __forceinline__ uint32_t pk_axis(uint64_t pk, int a) {
+    return (uint32_t)(pk >> ((2 - a) * POS_BITS)) & POS_MASK;
+}
+
+// one grid's stencil of one packed particle word into an LDS footprint:
+// DUAL = 0: plain grid, nodes li-1..li+2, footprint origin tile origin - 1;
+// DUAL = 1: dual grid (offset 1/2), base li or li+1, origin tile origin - 2
+template <int P, int ROT, int DUAL>
+__device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, int F, float fsc,
+                                           float wscale, int lane) {
     int li[3];
     float d[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        li[a] = (int)(q[a] >> F);
-        d[a] = (float)(q[a] & ((1u << F) - 1u)) * fsc;
+        unpack_axis(pk_axis(pk, a), F, fsc, li[a], d[a]);
+        if (DUAL) {
+            if (d[a] >= 0.5f) { d[a] -= 0.5f; li[a] += 1; }
+            else d[a] += 0.5f;
+        }
     }
     float wx[4], wy[4], wz[4], wzs[4];
     pcs_w_f32(d[0], wx);
@@ -650,30 +712,51 @@ __device__ __forceinline__ void paint_particle(int *__restrict__ fa, int *__rest
     pcs_w_f32(d[2], wz);
 #pragma unroll
     for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * wscale;
-    // plain grid: nodes li-1..li+2, footprint origin (tile origin - 1)
-    stencil_lds<P, ROT>(fa, li[0], li[1], li[2], wx, wy, wzs, lane);
-    if (interlace) {
-        // dual grid (offset 1/2): base li or li-1, footprint origin (tile origin - 2)
-        int ob[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (d[a] >= 0.5f) { d[a] -= 0.5f; ob[a] = li[a] + 1; }
-            else { d[a] += 0.5f; ob[a] = li[a]; }
-        }
-        pcs_w_f32(d[0], wx);
-        pcs_w_f32(d[1], wy);
-        pcs_w_f32(d[2], wz);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * wscale;
-        stencil_lds<P, ROT>(fb, ob[0], ob[1], ob[2], wx, wy, wzs, lane);
-    }
+    stencil_lds<P, ROT>(fp, li[0], li[1], li[2], wx, wy, wzs, lane);
 }
 
-// Paint of one work item (<= cap particles of one tile), one lane per
-// particle with coalesced loads; both grids' stencils go into the LDS
-// footprints as int32 fixed point, then the footprints are flushed.
+// LDS word index mod 64 of a particle's stencil base in one footprint: the
+// bank of its first stencil offset
+template <int P, int DUAL>
+__device__ __forceinline__ uint32_t stencil_bank(uint64_t pk, int F) {
+    uint32_t li[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t q = pk_axis(pk, a);
+        li[a] = (q >> F) + (DUAL ? (q >> (F - 1)) & 1u : 0u);
+    }
+    return ((li[0] * P + li[1]) * P + li[2]) & 63u;
+}
+
+#ifndef PAINT_BANKSORT
+#define PAINT_BANKSORT 1
+#endif
+
+// stage[] = this chunk's particles (pk, valid per thread) ordered by the
+// bank of their stencil base in one footprint, so that a wave's lanes
+// mostly share that bank; with the lanes' 64 rotated stencil orders their
+// atomics then hit 64 distinct banks at every step
+template <int P, int DUAL>
+__device__ __forceinline__ void bank_sort(uint64_t pk, bool valid, int F, uint32_t *hist,
+                                          uint32_t *off, uint64_t *stage) {
+    const uint32_t key = stencil_bank<P, DUAL>(pk, F);
+    __syncthreads();                           // stage and hist free
+    const uint32_t rank = wave_rank<6, true>(key, valid, hist);
+    __syncthreads();
+    small_exclusive_scan(hist, off, 64);
+    __syncthreads();
+    if (valid) stage[off[key] + rank] = pk;
+    if (threadIdx.x < 64) hist[threadIdx.x] = 0;
+    __syncthreads();
+}
+
+// Paint of one work item (<= cap particles of one tile), in chunks of
+// PAINT_THREADS particles (coalesced loads).  Each chunk is painted into the
+// plain footprint, then the dual one, each time after a bank sort; the
+// stencils are accumulated as int32 fixed point, then the footprints are
+// flushed.
 template <int T>
-__global__ void __launch_bounds__(PAINT_THREADS)
+__global__ void __launch_bounds__(PAINT_THREADS, 8)
 k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ tilebase,
               const uint32_t *__restrict__ items, long nslots, KeyGeo g, int nb2, float wscale,
               int cap, int interlace, unsigned long long *__restrict__ GA,
@@ -681,11 +764,14 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
     constexpr int SA = T + 3, SB = T + 4, P = paint_stride(T);
     constexpr int NA = SA * P * P, NB = SB * P * P;
     extern __shared__ int lds[];
+    __shared__ uint32_t hist[64], off[64];
     long t, p0, p1;
     if (!item_range(blockIdx.x, tilebase, items, nslots, cap, t, p0, p1)) return;
     const int nb = interlace ? NB : 0;
     int *fa = lds, *fb = lds + NA;
+    uint64_t *stage = (uint64_t *)(lds + NA + nb);             // [PAINT_THREADS]
     for (int i = threadIdx.x; i < NA + nb; i += blockDim.x) lds[i] = 0;
+    if (threadIdx.x < 64) hist[threadIdx.x] = 0;
     __syncthreads();
 
     const int F = g.F;
@@ -693,10 +779,24 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
     const int lane = __lane_id();
     long p = p0 + threadIdx.x;
     uint64_t nextpk = p < p1 ? sorted[p] : 0ull;
-    for (; p < p1; p += PAINT_THREADS) {
+    for (long c0 = p0; c0 < p1; c0 += PAINT_THREADS, p += PAINT_THREADS) {
         const uint64_t pk = nextpk;
         if (p + PAINT_THREADS < p1) nextpk = sorted[p + PAINT_THREADS];
-        paint_particle<P, PAINT_ROT>(fa, fb, pk, F, fsc, wscale, interlace, lane);
+        const bool valid = p < p1;
+        if (!PAINT_BANKSORT) {
+            if (valid) {
+                paint_grid<P, PAINT_ROT, 0>(fa, pk, F, fsc, wscale, lane);
+                if (interlace) paint_grid<P, PAINT_ROT, 1>(fb, pk, F, fsc, wscale, lane);
+            }
+            continue;
+        }
+        const bool mine = threadIdx.x < p1 - c0;
+        bank_sort<P, 0>(pk, valid, F, hist, off, stage);
+        if (mine) paint_grid<P, PAINT_ROT, 0>(fa, stage[threadIdx.x], F, fsc, wscale, lane);
+        if (interlace) {
+            bank_sort<P, 1>(pk, valid, F, hist, off, stage);
+            if (mine) paint_grid<P, PAINT_ROT, 1>(fb, stage[threadIdx.x], F, fsc, wscale, lane);
+        }
     }
     __syncthreads();
     const long b1 = t / nb2, f = t - b1 * nb2;
@@ -711,10 +811,11 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
 
 // ------------------------------------------------------------------ host
 
+// footprints and the staging of one chunk for the bank sort
 static int paint_lds_bytes(int T, int interlace) {
     const int P = paint_stride(T);
     const int a = (T + 3) * P * P, b = interlace ? (T + 4) * P * P : 0;
-    return (a + b) * 4;
+    return (a + b) * 4 + PAINT_THREADS * 8;
 }
 
 static int ceil_log2(long v) {
@@ -847,8 +948,11 @@ static int level_scans(hipStream_t s, const uint32_t *H, long nw, int nd, uint32
     hipLaunchKernelGGL(k_blocksum, dim3(blocks_for(nblk * nd, 256)), dim3(256), 0, s, H, nblk, nd,
                        S);
     FGK_LAUNCHED();
-    hipLaunchKernelGGL(k_segscan, dim3(blocks_for((long)nseg * nd, 256)), dim3(256), 0, s, S,
-                       blk0, nseg, nd, B, T);
+    if (nseg == 1)
+        hipLaunchKernelGGL(k_segscan_wide, dim3(nd), dim3(256), 0, s, S, nblk, nd, B, T);
+    else
+        hipLaunchKernelGGL(k_segscan, dim3(blocks_for((long)nseg * nd, 256)), dim3(256), 0, s,
+                           S, blk0, nseg, nd, B, T);
     FGK_LAUNCHED();
     hipLaunchKernelGGL(k_digitscan, dim3(1), dim3(SCAN_THREADS), 0, s, T, nslots, mode, cap, base,
                        aux, aux_blk);
@@ -905,7 +1009,7 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
             hipLaunchKernelGGL((k_sort<1, 1>), dim3((unsigned)nw1), dim3(SORT_THREADS), 0, s, x, y,
                                z, Np, g, (const uint64_t *)nullptr, (const uint32_t *)nullptr,
                                (const uint32_t *)nullptr, (const uint32_t *)nullptr, pp->nb1,
-                               pp->nb1, pp->bits1, pp->H1, (const uint32_t *)nullptr,
+                               pp->nb1, pp->H1, (const uint32_t *)nullptr,
                                (uint64_t *)nullptr);
             FGK_LAUNCHED();
             return 0;
@@ -919,7 +1023,7 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
             hipLaunchKernelGGL((k_sort<1, 0>), dim3((unsigned)nw1), dim3(SORT_THREADS), 0, s, x, y,
                                z, Np, g, (const uint64_t *)nullptr, (const uint32_t *)nullptr,
                                (const uint32_t *)nullptr, (const uint32_t *)nullptr, pp->nb1,
-                               pp->nb1, pp->bits1, (uint32_t *)nullptr, pp->O1, pp->sorted1);
+                               pp->nb1, (uint32_t *)nullptr, pp->O1, pp->sorted1);
             FGK_LAUNCHED();
             return 0;
         case 3: { // level 2 histograms (every capacity chunk; empty ones write zeros)
@@ -927,7 +1031,7 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
             hipLaunchKernelGGL((k_sort<2, 1>), dim3((unsigned)nw2), dim3(SORT_THREADS), 0, s,
                                (const float *)nullptr, (const float *)nullptr,
                                (const float *)nullptr, Np, g, pp->sorted1, pp->chunk0, pp->base1,
-                               pp->T1, pp->nb1, pp->nb2, pp->bits2, pp->H2,
+                               pp->T1, pp->nb1, pp->nb2, pp->H2,
                                (const uint32_t *)nullptr, (uint64_t *)nullptr);
             FGK_LAUNCHED();
             return 0;
@@ -943,7 +1047,7 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
             hipLaunchKernelGGL((k_sort<2, 0>), dim3((unsigned)nw2), dim3(SORT_THREADS), 0, s,
                                (const float *)nullptr, (const float *)nullptr,
                                (const float *)nullptr, Np, g, pp->sorted1, pp->chunk0, pp->base1,
-                               pp->T1, pp->nb1, pp->nb2, pp->bits2, (uint32_t *)nullptr, pp->O2,
+                               pp->T1, pp->nb1, pp->nb2, (uint32_t *)nullptr, pp->O2,
                                pp->sorted2);
             FGK_LAUNCHED();
             return 0;
