@@ -46,4 +46,15 @@ __device__ __forceinline__ cpx<T> cmul_si(cpx<T> a, int sign) {
 
 static inline unsigned fgk_ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8), each
+// with its own L2.  xcd_block maps blockIdx to a logical block such that each
+// XCD owns a contiguous range of logical blocks: neighbouring blocks share
+// cache lines at their edges, and this keeps those lines in one XCD's L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+    unsigned q = nb >> 3;
+    if (b < (q << 3)) return (b & 7u) * q + (b >> 3);
+    return b;
+}
+
+
 #endif

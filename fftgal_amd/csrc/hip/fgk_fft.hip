@@ -208,15 +208,6 @@ __device__ cpx<T> *lds_fft(cpx<T> *a, cpx<T> *b, const Stages &S, int P, int lgP
     return a;
 }
 
-// blockIdx -> logical block such that each XCD (blockIdx % 8) owns a
-// contiguous range of logical blocks; neighbouring blocks share cache lines
-// at their edges, and this keeps those lines in one XCD's L2.
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
-    unsigned q = nb >> 3;
-    if (b < (q << 3)) return (b & 7u) * q + (b >> 3);
-    return b;
-}
-
 // ------------------------------------------------------------ row kernels
 
 // MODE 0: real input in place in f; MODE 1: int64 fixed-point input src;

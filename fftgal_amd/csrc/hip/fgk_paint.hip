@@ -575,27 +575,27 @@ __device__ __forceinline__ int fixed_rne(float a, float b) {
 // 64/side whole rows along k; a lane's k offset is fixed, so the periodic
 // wrap needs no division (footprints extend at most 2 cells below 0 and
 // side cells past the grid's last index).
-__device__ void flush_footprint(const int *__restrict__ fp, int side, int P, int oi, int oj, int ok,
-                                int Ng, unsigned long long *__restrict__ G) {
+template <int SIDE, int P>
+__device__ void flush_footprint(const int *__restrict__ fp, int oi, int oj, int ok, int Ng,
+                                unsigned long long *__restrict__ G) {
+    constexpr int RPG = 64 / SIDE;             // rows per wave step
     const int lane = __lane_id();
     const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    const int rpg = 64 / side;                 // rows per wave step
-    const int ro = lane / side, fk = lane - ro * side;
-    if (ro >= rpg) return;
-    int gk = ok + fk;
-    gk = gk < 0 ? gk + Ng : (gk >= Ng ? gk - Ng : gk);
-    if (gk < 0) gk += Ng;        // grids smaller than a footprint wrap more than once
-    if (gk >= Ng) gk %= Ng;
-    const int nrows = side * side;
-    for (int r = wv * rpg + ro; r < nrows; r += nwv * rpg) {
-        const int fi = r / side, fj = r - fi * side;
+    const int ro = lane / SIDE, fk = lane - ro * SIDE;
+    if (ro >= RPG) return;
+    // a footprint starts at most 2 cells below 0 and ends at most SIDE past
+    // the grid's last index; grids smaller than that wrap more than once
+    auto wrap = [Ng](int v) {
+        v = v < 0 ? v + Ng : (v >= Ng ? v - Ng : v);
+        if ((unsigned)v >= (unsigned)Ng) v = ((v % Ng) + Ng) % Ng;
+        return v;
+    };
+    const int gk = wrap(ok + fk);
+    for (int r = wv * RPG + ro; r < SIDE * SIDE; r += nwv * RPG) {
+        const int fi = r / SIDE, fj = r - fi * SIDE;
         const int v = fp[(fi * P + fj) * P + fk];
         if (!v) continue;
-        int gi = oi + fi, gj = oj + fj;
-        gi = gi < 0 ? gi + Ng : (gi >= Ng ? gi - Ng : gi);
-        gj = gj < 0 ? gj + Ng : (gj >= Ng ? gj - Ng : gj);
-        if ((unsigned)gi >= (unsigned)Ng) gi = ((gi % Ng) + Ng) % Ng;
-        if ((unsigned)gj >= (unsigned)Ng) gj = ((gj % Ng) + Ng) % Ng;
+        const int gi = wrap(oi + fi), gj = wrap(oj + fj);
         atomicAdd(G + ((long)gi * Ng + gj) * Ng + gk, (unsigned long long)(long long)v);
     }
 }
@@ -805,8 +805,8 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
     const int fx = (int)(f / (g.cg1 * g.cg2)), fy = (int)((f / g.cg2) % g.cg1),
               fz = (int)(f % g.cg2);
     const int ti = cx * g.cg0 + fx, tj = cy * g.cg1 + fy, tk = cz * g.cg2 + fz;
-    flush_footprint(fa, SA, P, ti * T - 1, tj * T - 1, tk * T - 1, g.Ng, GA);
-    if (interlace) flush_footprint(fb, SB, P, ti * T - 2, tj * T - 2, tk * T - 2, g.Ng, GB);
+    flush_footprint<SA, P>(fa, ti * T - 1, tj * T - 1, tk * T - 1, g.Ng, GA);
+    if (interlace) flush_footprint<SB, P>(fb, ti * T - 2, tj * T - 2, tk * T - 2, g.Ng, GB);
 }
 
 // ------------------------------------------------------------------ host
