@@ -497,8 +497,10 @@ static BinArgs make_args(const fgk_binspec *b) {
     a.nvals = (b->do_pl ? 4 : 0) + Wm + (b->geometry ? 1 + Wm : 0);
     a.ncnt = b->geometry ? 1 + Wm : 0;
     size_t per = (size_t)a.nvals * 8 + (size_t)a.ncnt * 4;
+    // room for the widest window of a BI x BJ x BK block near the grid's
+    // corner (about BK + 4 bins at dK = KF) even with many wedges
     a.wmax = per ? (int)(BIN_LDS_BYTES / per) : 1;
-    if (a.wmax < 1) a.wmax = 1;
+    if (a.wmax < BK + 32) a.wmax = BK + 32;
     a.dKinv = b->dKinv;
     a.l0 = b->los[0];
     a.l1 = b->los[1];

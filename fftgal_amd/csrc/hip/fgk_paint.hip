@@ -514,16 +514,22 @@ __global__ void k_set_seg1(uint32_t *seg, uint32_t nblk) {
     }
 }
 
-// single-level path (grids with more tiles than two levels address)
-__global__ void k_count_global(const float *__restrict__ x, const float *__restrict__ y,
-                               const float *__restrict__ z, long Np, KeyGeo g,
-                               uint32_t *__restrict__ tot, uint32_t *__restrict__ rank) {
+// single-level path (grids with more tiles than two levels address): the
+// rank of a particle within its tile comes from a global counter per tile,
+// one atomic per distinct tile per wave (the wave-level match keeps
+// clustered catalogs from serialising on a few counters); every lane of a
+// wave calls wave_rank, valid or not
+__global__ void __launch_bounds__(256)
+k_count_global(const float *__restrict__ x, const float *__restrict__ y,
+               const float *__restrict__ z, long Np, KeyGeo g, uint32_t *__restrict__ tot,
+               uint32_t *__restrict__ rank) {
     const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (p >= Np) return;
-    uint32_t t;
+    const bool valid = p < Np;
+    uint32_t t = 0;
     uint64_t pk;
-    particle_key(x[p], y[p], z[p], g, t, pk);
-    rank[p] = atomicAdd(&tot[t], 1u);
+    if (valid) particle_key(x[p], y[p], z[p], g, t, pk);
+    const uint32_t r = wave_rank<24, true>(t, valid, tot);
+    if (valid) rank[p] = r;
 }
 
 __global__ void k_scatter_global(const float *__restrict__ x, const float *__restrict__ y,
@@ -869,6 +875,12 @@ extern "C" int fgk_paint_plan_create(fgk_paintplan **out, int Ng, int tile, long
     if (!pp->two_level) {   // single level: every tile is its own bucket
         cg[0] = cg[1] = cg[2] = 1;
         nc[0] = nc[1] = nc[2] = pp->nt;
+        if ((long)pp->nt * pp->nt * pp->nt >= (1l << 24)) {
+            fgk_set_error("paint: %d^3 tiles of side %d exceed the 2^24 the sort addresses",
+                          pp->nt, T);
+            free(pp);
+            return -2;
+        }
     }
     KeyGeo g = {0.0,   Ng,    T,     pp->nt, pp->F,
                 cg[0], cg[1], cg[2], nc[0],  nc[1],
@@ -904,7 +916,7 @@ extern "C" int fgk_paint_plan_create(fgk_paintplan **out, int Ng, int tile, long
     rc |= fgk_malloc((void **)&pp->T2, (size_t)pp->nslots * u);
     rc |= fgk_malloc((void **)&pp->tilebase, (size_t)(pp->nslots + 1) * u);
     rc |= fgk_malloc((void **)&pp->items, (size_t)(pp->nslots + 1) * u);
-    rc |= fgk_malloc((void **)&pp->sorted1, (size_t)Np_max * sizeof(uint64_t));
+    if (pp->two_level) rc |= fgk_malloc((void **)&pp->sorted1, (size_t)Np_max * sizeof(uint64_t));
     rc |= fgk_malloc((void **)&pp->sorted2, (size_t)Np_max * sizeof(uint64_t));
     if (rc) {
         fgk_paint_plan_destroy(pp);
