@@ -34,55 +34,68 @@ __device__ __forceinline__ void mode_ijk(long m, int Ng, int &i, int &j, int &k)
     i = (int)(r / Ng);
 }
 
+// The per-mode kernels walk the Ng^2 (Ng/2+1) modes with grid-stride loops:
+// at Ng = 2048 there are more modes than one dispatch may have work-items
+// (2^32).
 __global__ void k_offset_phase_f64(double *f, int Ng, const double *__restrict__ ph) {
 #pragma clang fp contract(off)
     const long nmodes = (long)Ng * Ng * (Ng / 2 + 1);
-    long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (m >= nmodes) return;
-    int i, j, k;
-    mode_ijk(m, Ng, i, j, k);
-    const double *p0 = ph + 2 * i, *p1 = ph + 2 * (Ng + j), *p2 = ph + 2 * (2 * Ng + k);
-    double pr = p0[0] * p1[0] * p2[0] - p0[0] * p1[1] * p2[1] - p0[1] * p1[0] * p2[1]
-              - p0[1] * p1[1] * p2[0];
-    double pi = p0[1] * p1[0] * p2[0] + p0[0] * p1[1] * p2[0] + p0[0] * p1[0] * p2[1]
-              - p0[1] * p1[1] * p2[1];
-    double *c = f + 2 * m;
-    double fr = c[0], fi = c[1];
-    c[0] = fr * pr - fi * pi;
-    c[1] = fr * pi + fi * pr;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < nmodes; m += stride) {
+        int i, j, k;
+        mode_ijk(m, Ng, i, j, k);
+        const double *p0 = ph + 2 * i, *p1 = ph + 2 * (Ng + j), *p2 = ph + 2 * (2 * Ng + k);
+        double pr = p0[0] * p1[0] * p2[0] - p0[0] * p1[1] * p2[1] - p0[1] * p1[0] * p2[1]
+                  - p0[1] * p1[1] * p2[0];
+        double pi = p0[1] * p1[0] * p2[0] + p0[0] * p1[1] * p2[0] + p0[0] * p1[0] * p2[1]
+                  - p0[1] * p1[1] * p2[1];
+        double *c = f + 2 * m;
+        double fr = c[0], fi = c[1];
+        c[0] = fr * pr - fi * pi;
+        c[1] = fr * pi + fi * pr;
+    }
 }
 
 __global__ void k_deconv_f64(double *f, int Ng, const double *__restrict__ winv) {
 #pragma clang fp contract(off)
     const long nmodes = (long)Ng * Ng * (Ng / 2 + 1);
-    long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (m >= nmodes) return;
-    int i, j, k;
-    mode_ijk(m, Ng, i, j, k);
-    double w3 = winv[i] * winv[j] * winv[k];
-    f[2 * m] *= w3;
-    f[2 * m + 1] *= w3;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < nmodes; m += stride) {
+        int i, j, k;
+        mode_ijk(m, Ng, i, j, k);
+        double w3 = winv[i] * winv[j] * winv[k];
+        f[2 * m] *= w3;
+        f[2 * m + 1] *= w3;
+    }
 }
 
 __global__ void k_P_f64(double *f, int Ng, double Vinv, double Pshot) {
 #pragma clang fp contract(off)
     const long nmodes = (long)Ng * Ng * (Ng / 2 + 1);
-    long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (m >= nmodes) return;
-    if (m == 0) {
-        f[0] = 0;
-        f[1] = 0;
-        return;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < nmodes; m += stride) {
+        if (m == 0) {
+            f[0] = 0;
+            f[1] = 0;
+            continue;
+        }
+        double re = f[2 * m], im = f[2 * m + 1];
+        double d2 = re * re + im * im;
+        f[2 * m] = d2 * Vinv - Pshot;
+        f[2 * m + 1] = 0;
     }
-    double re = f[2 * m], im = f[2 * m + 1];
-    double d2 = re * re + im * im;
-    f[2 * m] = d2 * Vinv - Pshot;
-    f[2 * m + 1] = 0;
+}
+
+// workgroups for a grid-stride walk over n elements with 256 threads
+static unsigned stride_blocks(long n) {
+    long b = (n + 255) / 256;
+    const long cap = 1l << 20;          // 2^28 work-items per dispatch
+    return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
 extern "C" int fgk_offset_phase_f64(double *f, int Ng, const double *phase_dev, fgk_stream_t s) {
     long n = (long)Ng * Ng * (Ng / 2 + 1);
-    hipLaunchKernelGGL(k_offset_phase_f64, dim3(fgk_ceil_div(n, 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_offset_phase_f64, dim3(stride_blocks(n)), dim3(256), 0,
                        (hipStream_t)s, f, Ng, phase_dev);
     FGK_LAUNCHED();
     return 0;
@@ -90,7 +103,7 @@ extern "C" int fgk_offset_phase_f64(double *f, int Ng, const double *phase_dev, 
 
 extern "C" int fgk_deconv_f64(double *f, int Ng, const double *winv_dev, fgk_stream_t s) {
     long n = (long)Ng * Ng * (Ng / 2 + 1);
-    hipLaunchKernelGGL(k_deconv_f64, dim3(fgk_ceil_div(n, 256)), dim3(256), 0, (hipStream_t)s, f,
+    hipLaunchKernelGGL(k_deconv_f64, dim3(stride_blocks(n)), dim3(256), 0, (hipStream_t)s, f,
                        Ng, winv_dev);
     FGK_LAUNCHED();
     return 0;
@@ -98,7 +111,7 @@ extern "C" int fgk_deconv_f64(double *f, int Ng, const double *winv_dev, fgk_str
 
 extern "C" int fgk_P_f64(double *f, int Ng, double Vinv, double Pshot, fgk_stream_t s) {
     long n = (long)Ng * Ng * (Ng / 2 + 1);
-    hipLaunchKernelGGL(k_P_f64, dim3(fgk_ceil_div(n, 256)), dim3(256), 0, (hipStream_t)s, f, Ng,
+    hipLaunchKernelGGL(k_P_f64, dim3(stride_blocks(n)), dim3(256), 0, (hipStream_t)s, f, Ng,
                        Vinv, Pshot);
     FGK_LAUNCHED();
     return 0;

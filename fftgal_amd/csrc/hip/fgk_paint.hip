@@ -65,22 +65,23 @@ k_paint_f64(double *f, int Ng, int Ng_pad, const double *__restrict__ x,
             const double *__restrict__ y, const double *__restrict__ z, long Np, double Hinv,
             double o0, double o1, double o2, double alpha, int negate) {
 #pragma clang fp contract(off)
-    long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (p >= Np) return;
-    int gi[4], gj[4], gk[4];
-    double wx[4], wy[4], wz[4];
-    pcs_axis_f64(x[p] * Hinv - o0, Ng, gi, wx);
-    pcs_axis_f64(y[p] * Hinv - o1, Ng, gj, wy);
-    pcs_axis_f64(z[p] * Hinv - o2, Ng, gk, wz);
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) {
-            double *row = f + ((long)gi[i] * Ng + gj[j]) * Ng_pad;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < Np; p += stride) {
+        int gi[4], gj[4], gk[4];
+        double wx[4], wy[4], wz[4];
+        pcs_axis_f64(x[p] * Hinv - o0, Ng, gi, wx);
+        pcs_axis_f64(y[p] * Hinv - o1, Ng, gj, wy);
+        pcs_axis_f64(z[p] * Hinv - o2, Ng, gk, wz);
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                double *row = f + ((long)gi[i] * Ng + gj[j]) * Ng_pad;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                double v = negate ? alpha * wx[i] * wy[j] * wz[k] : wx[i] * wy[j] * wz[k];
-                atomicAdd(row + gk[k], negate ? -v : v);
+                for (int k = 0; k < 4; ++k) {
+                    double v = negate ? alpha * wx[i] * wy[j] * wz[k] : wx[i] * wy[j] * wz[k];
+                    atomicAdd(row + gk[k], negate ? -v : v);
+                }
             }
-        }
+    }
 }
 
 __global__ void k_scale_f64(double *f, long n, double s) {
@@ -113,7 +114,9 @@ extern "C" int fgk_paint_f64(double *f, int Ng, const double *x, const double *y
                              int weight_sign, fgk_stream_t s) {
     if (Np <= 0) return 0;
     const int Ng_pad = 2 * (Ng / 2 + 1);
-    hipLaunchKernelGGL(k_paint_f64, dim3(fgk_ceil_div(Np, 256)), dim3(256), 0, (hipStream_t)s, f,
+    const long nb = (Np + 255) / 256;    // grid-stride beyond 2^28 work-items
+    hipLaunchKernelGGL(k_paint_f64, dim3((unsigned)(nb < (1l << 20) ? nb : (1l << 20))),
+                       dim3(256), 0, (hipStream_t)s, f,
                        Ng, Ng_pad, x, y, z, Np, Hinv, off[0], off[1], off[2], alpha,
                        weight_sign < 0 ? 1 : 0);
     FGK_LAUNCHED();
@@ -892,6 +895,13 @@ extern "C" int fgk_paint_plan_create(fgk_paintplan **out, int Ng, int tile, long
     pp->bits1 = ceil_log2(pp->nb1);
     pp->bits2 = ceil_log2(pp->nb2);
     pp->nslots = (long)pp->nb1 * pp->nb2;
+    // one paint workgroup per work item; a dispatch has fewer than 2^32 work-items
+    if ((pp->nslots + (Np_max + pp->cap - 1) / pp->cap + 8) * (long)PAINT_THREADS >= (1l << 32)) {
+        fgk_set_error("paint: %ld tiles of side %d and %ld particles exceed one dispatch",
+                      pp->nslots, T, Np_max);
+        free(pp);
+        return -2;
+    }
     pp->nw1max = round_up((Np_max + SORT_CHUNK - 1) / SORT_CHUNK, RB);
     pp->nw2max = pp->nw1max + (long)pp->nb1 * RB;
     int rc = 0;
