@@ -37,17 +37,37 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--np", type=int, default=1 << 27)
-    ap.add_argument("--ng", type=int, default=512)
-    ap.add_argument("--L", type=float, default=2000.0)
-    ap.add_argument("--nmu", type=int, default=10)
+    ap.add_argument("--config", type=int, default=3, choices=(3, 4, 5),
+                    help="3: 2^27 particles, Ng=512, L=2000, 10 wedges (default); "
+                         "4: batch of 64 lognormal mocks (L 250-2000, Ng 128-384, "
+                         "5e5-2e7 particles) each at folds 1, 2, 4 (192 catalogs); "
+                         "5: 2^30 particles, Ng=2048, L=4000, 20 wedges (large single "
+                         "grid; the CPU baseline is skipped there)")
+    ap.add_argument("--mocks", type=int, default=64, help="config 4: number of mocks")
+    ap.add_argument("--workers", type=int, default=16,
+                    help="config 4: processes generating the mocks")
+    ap.add_argument("--fold", type=int, default=1,
+                    help="grid box L/fold with the catalog in box L (src/app/P.c:35-36)")
+    ap.add_argument("--np", type=int, default=None)
+    ap.add_argument("--ng", type=int, default=None)
+    ap.add_argument("--L", type=float, default=None)
+    ap.add_argument("--nmu", type=int, default=None)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 21)
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed graph loop (for rocprofv3 runs)")
-    return ap.parse_args()
+    a = ap.parse_args()
+    preset = {3: (1 << 27, 512, 2000.0, 10), 4: (0, 0, 0.0, 10),
+              5: (1 << 30, 2048, 4000.0, 20)}[a.config]
+    a.np = a.np if a.np is not None else preset[0]
+    a.ng = a.ng if a.ng is not None else preset[1]
+    a.L = a.L if a.L is not None else preset[2]
+    a.nmu = a.nmu if a.nmu is not None else preset[3]
+    if a.config == 5:
+        a.no_cpu = True
+    return a
 
 
 def stage_bytes(name, Np, Ng):
@@ -60,11 +80,10 @@ def stage_bytes(name, Np, Ng):
         "sort2_count": 8 * Np,                  # read packed words
         "sort2_scatter": 8 * Np + 8 * Np,       # read, write packed words
         "paint_tiles": 8 * Np + 2 * 16 * n3,    # read packed, RMW both int64 grids once
-        # read the int64 grid, clear it for the next paint, write the spectrum
-        "fft_rows_A": 8 * n3 + 8 * n3 + spec,
-        "fft_rows_B": 8 * n3 + 8 * n3 + spec,
-        "fft_cols_j_A": 2 * spec, "fft_cols_i_A": 2 * spec,   # read + write FP32
-        "fft_cols_j_B": 2 * spec, "fft_cols_i_B": 2 * spec,
+        # both grids: read the int64 grid, clear it for the next paint,
+        # write the spectrum
+        "fft_rows": 2 * (8 * n3 + 8 * n3 + spec),
+        "fft_cols_j": 2 * 2 * spec, "fft_cols_i": 2 * 2 * spec,   # read + write FP32
         "fused_bin": 2 * spec,                  # read both spectra
     }
     return table.get(name, 0)
@@ -154,12 +173,157 @@ def _cpu_model():
     return "unknown"
 
 
+def batch_specs(nmocks):
+    """Mock parameters of the batched workload: per seed s (numpy PCG64
+    stream [s, 3]) L ~ U[250, 2000], Ng from {128, 192, 256, 384}, Np =
+    round(exp U[ln 5e5, ln 2e7])."""
+    import numpy as np
+    out = []
+    for s in range(nmocks):
+        r = np.random.default_rng([s, 3])
+        L = float(r.uniform(250.0, 2000.0))
+        Ng = int((128, 192, 256, 384)[r.integers(4)])
+        Np = int(round(np.exp(r.uniform(np.log(5e5), np.log(2e7)))))
+        out.append((s, L, Ng, Np))
+    return out
+
+
+def _batch_mock(spec):
+    from fftgal_amd import mockgen
+    s, L, Ng, Np = spec
+    (x, y, z), _ = mockgen.lognormal(Np, L, seed=s, Ngs=256)
+    return x, y, z
+
+
+def main_batch(args):
+    """Batched uneven workload: every mock at folds 1, 2 and 4, all
+    catalogs through one fftgal_batch_t (one launch per stage, one graph),
+    against the plain FP64 form run catalog by catalog."""
+    import multiprocessing as mp
+    import numpy as np
+    specs = batch_specs(args.mocks)
+    t0 = time.perf_counter()
+    # the mocks are drawn before anything touches the GPU (forked workers)
+    with mp.get_context("fork").Pool(max(1, args.workers)) as pool:
+        mocks = pool.map(_batch_mock, specs, chunksize=1)
+    t_mock = time.perf_counter() - t0
+    print(f"# {len(mocks)} mocks, {sum(s[3] for s in specs)} particles in {t_mock:.1f}s",
+          file=sys.stderr, flush=True)
+    import torch
+    import fftgal_amd as fa
+    fa.load()
+    nmu = args.nmu if args.nmu is not None else 10
+    folds = (1, 2, 4)
+    dev = []
+    for (x, y, z) in mocks:
+        dev.append([torch.from_numpy(a).cuda() for a in (x, y, z)])
+    torch.cuda.synchronize()
+    jobs, cats = [], []
+    for (s, L, Ng, Np), d in zip(specs, dev):
+        for fold in folds:
+            Lg = L / fold
+            jobs.append({"Ng": Ng, "L": Lg, "Np_max": Np, "dK": 2 * np.pi / Lg})
+            cats.append((d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), Np, L ** 3))
+    Ntot = sum(c[3] for c in cats)
+    t0 = time.perf_counter()
+    b = fa.BatchPlan(jobs, los=(0, 0, 1), Nmu=nmu)
+    t_plan = time.perf_counter() - t0
+    print(f"# batch plan: {len(jobs)} catalogs in {t_plan:.1f}s", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        b.enqueue(cats)
+    b.sync()
+    stream = torch.cuda.ExternalStream(b.stream())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        b.enqueue(cats)
+    e1.record(stream)
+    b.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = e0.elapsed_time(e1) / args.steps
+    value = Ntot / (ms_per_step * 1e-3) / 1e6
+    print(f"# batched: {ms_per_step:.2f} ms per batch ({value:.0f} Mparticles/s)",
+          file=sys.stderr, flush=True)
+    fast = [b.results(j) for j in range(len(jobs))]
+    prof = b.profile(cats)
+    b.close()
+    torch.cuda.empty_cache()
+
+    # plain FP64 form, catalog by catalog (device resident)
+    t_plain = 0.0
+    worst = {"P0": 0.0, "P2": 0.0, "P4": 0.0, "wedges": 0.0}
+    counts_equal = True
+    for q, ((s, L, Ng, Np), fold) in enumerate(
+            [(sp, f) for sp in specs for f in folds]):
+        d = dev[q // len(folds)]
+        dd = [t.double() for t in d]
+        torch.cuda.synchronize()
+        Lg = L / fold
+        g = fa.Grid(Ng, Lg, resident=True)
+        lib = g.lib
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        lib.fft_p2k_device(g.ptr, dd[0].data_ptr(), dd[1].data_ptr(), dd[2].data_ptr(), Np,
+                           L ** 3)
+        lib.P_params(g.ptr, Np, L ** 3, 0)
+        K, N, Pl = g.Pl_bins((0, 0, 1), 2 * np.pi / Lg)
+        Km, Nm, Pm = g.Pmu_bins((0, 0, 1), 2 * np.pi / Lg, nmu)
+        t_plain += time.perf_counter() - t1
+        g.free()
+        del dd
+        o = fast[q]
+        counts_equal &= bool(np.array_equal(o["N"], N) and np.array_equal(o["Nmu"], Nm))
+        sel = (N > 0) & (np.arange(len(N)) < Ng // 2)
+        Pshot = L ** 3 / Np
+        scale = np.abs(Pl[sel, 0]) + Pshot
+        for l, key in enumerate(("P0", "P2", "P4")):
+            e = float((np.abs(o["Pl"][sel, l] - Pl[sel, l]) / scale).max())
+            worst[key] = max(worst[key], e)
+        wm = (Nm[:Ng // 2] > 0)
+        ew = np.abs(o["Pmu"][:Ng // 2][wm] - Pm[:Ng // 2][wm]) / (np.abs(Pm[:Ng // 2][wm]) + Pshot)
+        worst["wedges"] = max(worst["wedges"], float(ew.max()))
+    torch.cuda.empty_cache()
+    plain_value = Ntot / t_plain / 1e6
+    print(f"# plain FP64: {t_plain * 1e3:.0f} ms for all catalogs", file=sys.stderr, flush=True)
+    cells = sum(j["Ng"] ** 3 for j in jobs)
+    line = {
+        "metric": "Mparticles/s to P_l(k), batched uneven workload (192 catalogs), "
+                  "mixed vs plain FP64",
+        "value": round(value, 1), "unit": "Mparticles/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64/f32/f64",
+        "data": ("64 synthetic lognormal mocks (numpy, seeds 0..63, 256^3 seed grids, "
+                 "Zel'dovich RSD along z), each at folds 1, 2, 4"),
+        "config": {"workload": f"{len(jobs)} catalogs (64 mocks x folds 1,2,4), "
+                               f"sum Np = {Ntot}, sum Ng^3 = {cells}, interlaced, "
+                               f"l=0,2,4(,6) and {nmu} |mu|-wedges, dK=kF of each grid box",
+                   "grids": {str(n): sum(1 for j in jobs if j["Ng"] == n)
+                             for n in (128, 192, 256, 384)},
+                   "parallelism": "1 GPU"},
+        "stages_ms": {k: round(v, 3) for k, v in prof.items()},
+        "plain_fp64": {"value": round(plain_value, 2), "unit": "Mparticles/s",
+                       "ms_total": round(t_plain * 1e3, 1),
+                       "form": "drop-in API per catalog, device resident, FP64"},
+        "speedup_vs_plain_fp64": round(value / plain_value, 2),
+        "error_vs_plain_fp64": {"max_rel_dP_over_P0_plus_Pshot_below_kNyq": worst,
+                                "counts_equal": counts_equal},
+        "timing": {"mock_s": round(t_mock, 1), "plan_s": round(t_plan, 2),
+                   "wall_s": round(wall, 3)},
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.config == 4:
+        return main_batch(args)
     import numpy as np
     import torch
     import fftgal_amd as fa
@@ -167,10 +331,14 @@ def main():
 
     fa.load()
     Np, Ng, L, nmu = args.np, args.ng, args.L, args.nmu
-    V = L ** 3
-    dK = 2 * np.pi / L
+    V = L ** 3                 # catalog volume
+    Lg = L / args.fold         # grid box (folding)
+    dK = 2 * np.pi / Lg
     t0 = time.perf_counter()
-    (x, y, z), sigma2 = mockgen.lognormal(Np, L, seed=0, Ngs=256)
+    (x, y, z), sigma2 = mockgen.lognormal(
+        Np, L, seed=0, Ngs=256,
+        log=lambda m: print(f"# {m} ({time.perf_counter() - t0:.0f}s)", file=sys.stderr,
+                            flush=True))
     t_mock = time.perf_counter() - t0
     print(f"# mock: {Np} particles in {t_mock:.1f}s (sigma_G^2 = {sigma2:.2f})",
           file=sys.stderr, flush=True)
@@ -179,8 +347,9 @@ def main():
     ptr = [d.data_ptr() for d in dev]
 
     t0 = time.perf_counter()
-    plan = fa.PowerPlan(Ng, L, interlace=True, dK=dK, los=(0, 0, 1), Nmu=nmu, Np_max=Np)
+    plan = fa.PowerPlan(Ng, Lg, interlace=True, dK=dK, los=(0, 0, 1), Nmu=nmu, Np_max=Np)
     t_plan = time.perf_counter() - t0
+    print(f"# plan: {t_plan:.2f}s", file=sys.stderr, flush=True)
     stream = torch.cuda.ExternalStream(plan.stream())
 
     for _ in range(args.warmup):
@@ -222,6 +391,7 @@ def main():
         st = plan.profile(*ptr, Np, V)
         for k, v in st.items():
             acc[k] = acc.get(k, 0.0) + v / nprof
+    print("# stage profile done", file=sys.stderr, flush=True)
     stages = {}
     for k, ms in acc.items():
         b = stage_bytes(k, Np, Ng)
@@ -249,7 +419,7 @@ def main():
     if not args.no_plain:
         dd = [d.double() for d in dev]
         torch.cuda.synchronize()
-        g = fa.Grid(Ng, L, resident=True)
+        g = fa.Grid(Ng, Lg, resident=True)
         lib = g.lib
         times = []
         for _ in range(args.plain_steps):
@@ -263,6 +433,7 @@ def main():
         g.free()
         del dd
         tp = min(times)
+        print(f"# plain FP64: {tp * 1e3:.1f} ms/catalog", file=sys.stderr, flush=True)
         plain = {"value": round(Np / tp / 1e6, 2), "unit": "Mparticles/s",
                  "ms_per_catalog": round(tp * 1e3, 2),
                  "form": "drop-in API, device resident: FP64 atomic paint x2, FP64 FFT x2, "
@@ -282,7 +453,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(x, y, z, V, Ng, L, nmu, args.cpu_sample)
+        cpu = cpu_baseline(x, y, z, V, Ng, Lg, nmu, args.cpu_sample)
 
     name = torch.cuda.get_device_name(0)
     line = {
@@ -300,9 +471,11 @@ def main():
         "data": ("synthetic lognormal mock (numpy, seed 0): P(k)=2e4 (k/0.1)^-1.5 on a 256^3 "
                  "seed grid, Poisson-sampled, Zel'dovich RSD along z f=0.76, random order"),
         "config": {"workload": (f"lognormal mock, {Np} float32 particles, L={L:g} Mpc/h, "
-                                f"Ng={Ng}, PCS paint + interlacing, l=0,2,4(,6) and "
+                                f"Ng={Ng}" + (f", fold {args.fold}" if args.fold > 1 else "") +
+                                f", PCS paint + interlacing, l=0,2,4(,6) and "
                                 f"{nmu} |mu|-wedges, dK=kF"),
-                   "Np": Np, "Ng": Ng, "L": L, "interlace": True, "Nmu": nmu,
+                   "Np": Np, "Ng": Ng, "L": L, "fold": args.fold, "interlace": True,
+                   "Nmu": nmu,
                    "precision": "int64 fixed-point paint, FP32 FFT (FP64 twiddles), "
                                 "FP64 k-space math and bin sums",
                    "parallelism": "1 GPU"},

@@ -106,6 +106,22 @@ API = {
     "fftgal_power_stage_name": (C.c_char_p, [C.c_int]),
     "fftgal_power_profile": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_long, C.c_double, C.POINTER(C.c_float)]),
+    "fftgal_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_void_p,
+                                      C.POINTER(C.c_double), C.c_int, C.c_char_p]),
+    "fftgal_batch_destroy": (C.c_int, [C.c_void_p]),
+    "fftgal_batch_NK": (C.c_int, [C.c_void_p, C.c_int]),
+    "fftgal_batch_enqueue": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p]),
+    "fftgal_batch_sync": (C.c_int, [C.c_void_p]),
+    "fftgal_batch_results": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_long), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_long),
+                                       C.POINTER(C.c_double)]),
+    "fftgal_batch_profile": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
+    "fftgal_batch_use_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "fftgal_batch_tile": (C.c_int, [C.c_void_p, C.c_int]),
+    "fftgal_batch_stream": (C.c_void_p, [C.c_void_p]),
 }
 
 
@@ -399,6 +415,116 @@ class PowerPlan:
     def close(self):
         if getattr(self, "h", None):
             self.lib.fftgal_power_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class JobDesc(C.Structure):
+    """fftgal_job_desc (include/fftgal_batch.h)."""
+    _fields_ = [("Ng", C.c_int), ("L", C.c_double), ("interlace", C.c_int),
+                ("dK", C.c_double), ("Np_max", C.c_long), ("paint_tile", C.c_int)]
+
+
+class BatchPlan:
+    """fftgal_batch_t: many catalogs, each on its own grid, through one
+    launch per pipeline stage.  ``jobs`` is a list of dicts with keys Ng, L
+    (grid box = catalog box / fold), Np_max and optionally interlace (True),
+    dK (2 pi / L) and paint_tile (0)."""
+
+    def __init__(self, jobs, los=(0.0, 0.0, 1.0), Nmu=0, wisdom=None):
+        self.lib = load()
+        n = len(jobs)
+        arr = (JobDesc * n)()
+        for i, j in enumerate(jobs):
+            arr[i].Ng = int(j["Ng"])
+            arr[i].L = float(j["L"])
+            arr[i].interlace = 1 if j.get("interlace", True) else 0
+            arr[i].dK = float(j.get("dK", 2 * np.pi / j["L"]))
+            arr[i].Np_max = int(j["Np_max"])
+            arr[i].paint_tile = int(j.get("paint_tile", 0))
+        self.n = n
+        self.Nmu = int(Nmu)
+        lo = (C.c_double * 3)(*los)
+        h = C.c_void_p()
+        rc = self.lib.fftgal_batch_create(C.byref(h), n, C.cast(arr, C.c_void_p), lo, int(Nmu),
+                                          wisdom.encode() if wisdom else None)
+        if rc:
+            raise RuntimeError(f"fftgal_batch_create failed ({rc}): {last_error()}")
+        self.h = h
+        self.NK = [self.lib.fftgal_batch_NK(h, i) for i in range(n)]
+
+    def _check(self, rc, what):
+        if rc:
+            raise RuntimeError(f"{what} failed ({rc}): {last_error()}")
+
+    def _args(self, cats):
+        """cats: list of (x_ptr, y_ptr, z_ptr, Np, V) per catalog."""
+        n = self.n
+        if len(cats) != n:
+            raise ValueError(f"expected {n} catalogs, got {len(cats)}")
+        xs = (C.c_void_p * n)(*[c[0] for c in cats])
+        ys = (C.c_void_p * n)(*[c[1] for c in cats])
+        zs = (C.c_void_p * n)(*[c[2] for c in cats])
+        nps = (C.c_long * n)(*[int(c[3]) for c in cats])
+        vs = (C.c_double * n)(*[float(c[4]) for c in cats])
+        self._keep = (xs, ys, zs, nps, vs)
+        return xs, ys, zs, nps, vs
+
+    def enqueue(self, cats):
+        self._check(self.lib.fftgal_batch_enqueue(self.h, *self._args(cats)),
+                    "fftgal_batch_enqueue")
+
+    def sync(self):
+        self._check(self.lib.fftgal_batch_sync(self.h), "fftgal_batch_sync")
+
+    def results(self, job):
+        nk, nm = self.NK[job], max(self.Nmu, 1)
+        K = np.zeros(nk)
+        N = np.zeros(nk, dtype=np.int64)
+        P = np.zeros((nk, 4))
+        Km = np.zeros((nk, nm))
+        Nm = np.zeros((nk, nm), dtype=np.int64)
+        Pm = np.zeros((nk, nm))
+        self._check(self.lib.fftgal_batch_results(self.h, int(job), _dp(K),
+                                                  N.ctypes.data_as(lptr), _dp(P), _dp(Km),
+                                                  Nm.ctypes.data_as(lptr), _dp(Pm)),
+                    "fftgal_batch_results")
+        out = {"K": K, "N": N, "Pl": P}
+        if self.Nmu:
+            out.update(Kmu=Km, Nmu=Nm, Pmu=Pm)
+        return out
+
+    def run(self, cats):
+        self.enqueue(cats)
+        self.sync()
+        return [self.results(j) for j in range(self.n)]
+
+    def profile(self, cats):
+        """Eager run with events between stages: {stage name: ms} (each
+        stage covers every catalog)."""
+        n = self.lib.fftgal_power_stages()
+        ms = (C.c_float * n)()
+        self._check(self.lib.fftgal_batch_profile(self.h, *self._args(cats), ms),
+                    "fftgal_batch_profile")
+        return {self.lib.fftgal_power_stage_name(i).decode(): float(ms[i]) for i in range(n)}
+
+    def use_graph(self, on):
+        self.lib.fftgal_batch_use_graph(self.h, 1 if on else 0)
+
+    def tile(self, job):
+        return self.lib.fftgal_batch_tile(self.h, int(job))
+
+    def stream(self):
+        return self.lib.fftgal_batch_stream(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fftgal_batch_destroy(self.h)
         self.h = None
 
     def __del__(self):

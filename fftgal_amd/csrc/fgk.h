@@ -68,10 +68,12 @@ int  fgk_fft_c2r_f64(fgk_fftplan *p, double *f, double scale, fgk_stream_t s);
  *   F(X0)[m] = (Z[m] + conj Z[-m]) / 2,  F(XN)[m] = (Z[m] - conj Z[-m]) / 2i */
 int  fgk_fft_r2c_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *src,
                                 double in_scale, int clear_src, fgk_stream_t s);
-/* the three passes separately (for the profiler and the tuner) */
+/* the three passes separately, each over nbatch grids stored one after the
+ * other (src: Ng^3 int64 apart; f: Ng^3/2 complex apart), in one launch */
 int  fgk_fft_rows_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *src,
-                                 double in_scale, int clear_src, fgk_stream_t s);
-int  fgk_fft_cols_f32(fgk_fftplan *p, float *f, int axis, int sign, fgk_stream_t s);
+                                 double in_scale, int clear_src, int nbatch, fgk_stream_t s);
+int  fgk_fft_cols_f32(fgk_fftplan *p, float *f, int axis, int sign, int nbatch,
+                      fgk_stream_t s);
 /* 1 if the FP32 passes of this Ng run compile-time specialised kernels
  * (otherwise the generic LDS Stockham kernels) */
 int  fgk_fft_f32_specialised(int Ng);
@@ -109,6 +111,29 @@ const char *fgk_paint_stage_name(int i);
 int  fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x,
                      const float *y, const float *z, long Np, double Hinv,
                      int64_t *gridA, int64_t *gridB, fgk_stream_t s);
+
+/* the same paint for many catalogs, each with its own grid, in one launch
+ * per sort stage and one paint launch per tile side (the same
+ * fgk_paint_nstages() stages).  A catalog's grid must need at most
+ * 256 x 128 tiles (Ng <= 512 with tiles of 16). */
+typedef struct fgk_paintjob_desc {
+    int Ng;
+    int tile;            /* tile side (0: default) */
+    double Hinv;         /* Ng / grid box */
+    long Np_max;
+    int interlace;
+    int64_t *gridA, *gridB;   /* zeroed int64 grids (gridB: interlace only) */
+} fgk_paintjob_desc;
+typedef struct fgk_paintbatch fgk_paintbatch;
+int  fgk_paint_batch_create(fgk_paintbatch **pb, int njobs, const fgk_paintjob_desc *jobs);
+void fgk_paint_batch_destroy(fgk_paintbatch *pb);
+/* positions of every catalog (device float32) and Np[j] <= Np_max; when
+ * they differ from the last call, waits for s and uploads the job table */
+int  fgk_paint_batch_set(fgk_paintbatch *pb, const float *const *x, const float *const *y,
+                         const float *const *z, const long *Np, fgk_stream_t s);
+int  fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t s);
+double fgk_paint_batch_fixed_scale(const fgk_paintbatch *pb);
+int  fgk_paint_batch_tile(const fgk_paintbatch *pb, int job);
 
 /* ---- k space (fgk_kspace.hip) ---- */
 /* phase[d][j] complex table (3*Ng*2 doubles) built on the host */
@@ -155,6 +180,23 @@ int  fgk_fused_bin_f32(const float *A, const float *B, const fgk_binspec *b,
                        const double *phase_dev, const double *winv_dev,
                        double norm, double Vinv, double Pshot, double npart,
                        double *bins_dev, fgk_stream_t s);
+
+/* one catalog of a batched fused bin over grids of one side: packed
+ * spectra A, B (B NULL: no interlacing), its bin sums (layout as above for
+ * NK bins, geometry 0), dKinv = KF/dK of its box, and the arguments of
+ * fgk_fused_bin_f32 */
+typedef struct fgk_binjob {
+    const float *A, *B;
+    double *bins;
+    int NK;
+    double dKinv;
+    double norm, Vinv, Pshot, npart;
+} fgk_binjob;
+/* one launch pair for njobs catalogs (table in device memory) whose grids
+ * share b->Ng, Nmu, los and do_pl; NK_max = largest NK of the catalogs */
+int  fgk_fused_bin_f32_batch(const fgk_binspec *b, const fgk_binjob *jobs_dev, int njobs,
+                             int NK_max, const double *phase_dev, const double *winv_dev,
+                             fgk_stream_t s);
 
 #ifdef __cplusplus
 }

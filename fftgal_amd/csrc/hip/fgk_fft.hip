@@ -581,14 +581,16 @@ extern "C" int fgk_fft_f32_specialised(int Ng) {
     }
 }
 
-static int launch_cols32(int N, cpx<float> *g, long stride_t, long stride_o,
-                         const cpx<float> *stw, const cpx<float> *tw, int sign, hipStream_t s) {
-    const int ncols = N / 2;
+// nouter groups of ncols contiguous columns, group o at g + o * stride_o,
+// pencil element t at stride stride_t
+static int launch_cols32(int N, cpx<float> *g, int ncols, long stride_t, long stride_o,
+                         long nouter, const cpx<float> *stw, const cpx<float> *tw, int sign,
+                         hipStream_t s) {
     switch (N) {
 #define CASE_NG(n)                                                                            \
     case n: {                                                                                 \
         constexpr int C = cols_C(n), TPP = cols_TPP(n);                                       \
-        const long nb = (long)N * ((ncols + C - 1) / C);                                      \
+        const long nb = nouter * ((ncols + C - 1) / C);                                       \
         hipLaunchKernelGGL((k_cols32<n, C, TPP>), dim3((unsigned)nb), dim3(C * TPP), 0, s, g, \
                            ncols, stride_t, stride_o, stw, tw, 1, sign);                      \
         break;                                                                                \
@@ -603,9 +605,9 @@ static int launch_cols32(int N, cpx<float> *g, long stride_t, long stride_o,
     return 0;
 }
 
-static int launch_rows32(int N, cpx<float> *out, int64_t *src, double in_scale, int clear_src,
-                         const cpx<float> *stw, const cpx<float> *tw, hipStream_t s) {
-    const long nrows = (long)N * N;
+static int launch_rows32(int N, long nrows, cpx<float> *out, int64_t *src, double in_scale,
+                         int clear_src, const cpx<float> *stw, const cpx<float> *tw,
+                         hipStream_t s) {
     switch (N) {
 #define CASE_NG(n)                                                                            \
     case n: {                                                                                 \
@@ -758,32 +760,46 @@ static int ilog2(int c) {
 
 // packed: the FP32 packed half-spectrum layout (Ng/2 columns), else the
 // padded layout (Ng/2+1 columns in rows of Ng/2+1 complex)
-template <typename T>
-static int launch_cols(fgk_fftplan *p, T *f, int axis, int sign, T scale, hipStream_t s,
-                       int packed = 0) {
-    const int N = p->Ng, Nk = packed ? N / 2 : N / 2 + 1;
-    const int C = cols_per_wg(p, sizeof(cpx<T>));
+// Column passes of nbatch contiguous grids.  axis 1 (along j): N * nbatch
+// planes of Nk columns at stride rowc; axis 0 (along i): nbatch grids of
+// N * Nk columns (the whole (j, k) plane is contiguous) at stride planec.
+struct ColShape {
+    int ncols;
+    long stride_t, stride_o, nouter;
+};
+
+static ColShape col_shape(int N, int Nk, int axis, int nbatch) {
     const long rowc = Nk;                  // complex elements between j rows
     const long planec = (long)N * Nk;      // complex elements between i planes
-    long stride_t = axis == 1 ? rowc : planec;
-    long stride_o = axis == 1 ? planec : rowc;
-    const long nb = (long)N * ((Nk + C - 1) / C);
+    if (axis == 1) return ColShape{Nk, rowc, planec, (long)N * nbatch};
+    return ColShape{N * Nk, planec, (long)N * planec, (long)nbatch};
+}
+
+template <typename T>
+static int launch_cols(fgk_fftplan *p, T *f, int axis, int sign, T scale, hipStream_t s,
+                       int packed = 0, int nbatch = 1) {
+    const int N = p->Ng, Nk = packed ? N / 2 : N / 2 + 1;
+    const int C = cols_per_wg(p, sizeof(cpx<T>));
+    const ColShape cs = col_shape(N, Nk, axis, nbatch);
+    const long nb = cs.nouter * ((cs.ncols + C - 1) / C);
     size_t lds = (size_t)2 * N * C * sizeof(cpx<T>);
-    hipLaunchKernelGGL(k_cols<T>, dim3((unsigned)nb), dim3(256), lds, s, (cpx<T> *)f, N, Nk,
-                       stride_t, stride_o, C, ilog2(C), p->sN, (const cpx<T> *)p->tw, sign,
-                       scale);
+    hipLaunchKernelGGL(k_cols<T>, dim3((unsigned)nb), dim3(256), lds, s, (cpx<T> *)f, N,
+                       cs.ncols, cs.stride_t, cs.stride_o, C, ilog2(C), p->sN,
+                       (const cpx<T> *)p->tw, sign, scale);
     FGK_LAUNCHED();
     return 0;
 }
 
-extern "C" int fgk_fft_cols_f32(fgk_fftplan *p, float *f, int axis, int sign, fgk_stream_t s) {
+extern "C" int fgk_fft_cols_f32(fgk_fftplan *p, float *f, int axis, int sign, int nbatch,
+                                fgk_stream_t s) {
     if (p->prec != 4) { fgk_set_error("plan precision is not FP32"); return -2; }
+    if (nbatch < 1) { fgk_set_error("FFT: nbatch must be >= 1"); return -2; }
     const int N = p->Ng, M = N / 2;
-    if (!p->spec) return launch_cols<float>(p, f, axis, sign, 1.0f, (hipStream_t)s, 1);
-    const long rowc = M, planec = (long)N * M;
-    return launch_cols32(N, (cpx<float> *)f, axis == 1 ? rowc : planec,
-                         axis == 1 ? planec : rowc, (const cpx<float> *)p->stw_cols,
-                         (const cpx<float> *)p->tw, sign, (hipStream_t)s);
+    if (!p->spec) return launch_cols<float>(p, f, axis, sign, 1.0f, (hipStream_t)s, 1, nbatch);
+    const ColShape cs = col_shape(N, M, axis, nbatch);
+    return launch_cols32(N, (cpx<float> *)f, cs.ncols, cs.stride_t, cs.stride_o, cs.nouter,
+                         (const cpx<float> *)p->stw_cols, (const cpx<float> *)p->tw, sign,
+                         (hipStream_t)s);
 }
 
 extern "C" int fgk_fft_r2c_f64(fgk_fftplan *p, double *f, double scale, fgk_stream_t st) {
@@ -817,13 +833,15 @@ extern "C" int fgk_fft_c2r_f64(fgk_fftplan *p, double *f, double scale, fgk_stre
 }
 
 extern "C" int fgk_fft_rows_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *src,
-                                           double in_scale, int clear_src, fgk_stream_t st) {
+                                           double in_scale, int clear_src, int nbatch,
+                                           fgk_stream_t st) {
     if (p->prec != 4) { fgk_set_error("plan precision is not FP32"); return -2; }
+    if (nbatch < 1) { fgk_set_error("FFT: nbatch must be >= 1"); return -2; }
     hipStream_t s = (hipStream_t)st;
     const int N = p->Ng, M = N / 2;
-    const long nrows = (long)N * N;
+    const long nrows = (long)N * N * nbatch;   // rows of contiguous grids
     if (p->spec)
-        return launch_rows32(N, (cpx<float> *)f, src, in_scale, clear_src,
+        return launch_rows32(N, nrows, (cpx<float> *)f, src, in_scale, clear_src,
                              (const cpx<float> *)p->stw_rows, (const cpx<float> *)p->tw, s);
     const int P = rows_per_wg(p, sizeof(cpx<float>), 0);
     size_t lds = (size_t)2 * P * M * sizeof(cpx<float>);
@@ -836,7 +854,7 @@ extern "C" int fgk_fft_rows_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *sr
 
 extern "C" int fgk_fft_r2c_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *src,
                                           double in_scale, int clear_src, fgk_stream_t s) {
-    if (fgk_fft_rows_f32_from_fixed(p, f, src, in_scale, clear_src, s)) return -1;
-    if (fgk_fft_cols_f32(p, f, 1, -1, s)) return -1;
-    return fgk_fft_cols_f32(p, f, 0, -1, s);
+    if (fgk_fft_rows_f32_from_fixed(p, f, src, in_scale, clear_src, 1, s)) return -1;
+    if (fgk_fft_cols_f32(p, f, 1, -1, 1, s)) return -1;
+    return fgk_fft_cols_f32(p, f, 0, -1, 1, s);
 }

@@ -377,14 +377,32 @@ __device__ __forceinline__ BinOut make_out(const BinArgs &a, double *sh, int blo
 //         0 < k < Ng/2 (the k = 0 and k = Ng/2 planes: k_bin_plane);
 // MODE 2: geometry only (K sums and mode counts), padded k range.
 // Each workgroup owns a BI x BJ x BK block of modes, one thread per k.
+// A batch of catalogs on grids of one side (jobs != nullptr): blockIdx.y
+// selects the catalog, whose spectra, bins, k binning and normalisations
+// replace A, B, bins, a.NK, a.dKinv, norm, Vinv, Pshot and npart.
+#define BIN_JOB(jobs)                      \
+    do {                                   \
+        const fgk_binjob &J = jobs[blockIdx.y]; \
+        A = J.A;                           \
+        B = J.B;                           \
+        bins = J.bins;                     \
+        a.NK = J.NK;                       \
+        a.dKinv = J.dKinv;                 \
+        norm = J.norm;                     \
+        Vinv = J.Vinv;                     \
+        Pshot = J.Pshot;                   \
+        npart = J.npart;                   \
+    } while (0)
+
 template <int MODE>
 __global__ void __launch_bounds__(BIN_THREADS)
 k_bin(const double *__restrict__ fP, const float *__restrict__ A, const float *__restrict__ B,
       BinArgs a, const double *__restrict__ ph, const double *__restrict__ winv, double norm,
       double Vinv, double Pshot, double npart, double *__restrict__ bins,
-      unsigned long long *__restrict__ cnts) {
+      unsigned long long *__restrict__ cnts, const fgk_binjob *__restrict__ jobs) {
 #pragma clang fp contract(off)
     extern __shared__ double sh[];
+    if (MODE == 1 && jobs) BIN_JOB(jobs);
     const int Ng = a.Ng;
     const int Mh = Ng / 2;
     const int Nk = MODE == 1 ? Mh : Mh + 1;      // k columns walked
@@ -452,9 +470,11 @@ k_bin(const double *__restrict__ fP, const float *__restrict__ A, const float *_
 __global__ void __launch_bounds__(PI * PJ)
 k_bin_plane(const float *__restrict__ A, const float *__restrict__ B, BinArgs a,
             const double *__restrict__ ph, const double *__restrict__ winv, double norm,
-            double Vinv, double Pshot, double npart, double *__restrict__ bins) {
+            double Vinv, double Pshot, double npart, double *__restrict__ bins,
+            const fgk_binjob *__restrict__ jobs) {
 #pragma clang fp contract(off)
     extern __shared__ double sh[];
+    if (jobs) BIN_JOB(jobs);
     const int Ng = a.Ng;
     const int Mh = Ng / 2;
     const int nbj = (Ng + PJ - 1) / PJ;
@@ -538,7 +558,7 @@ extern "C" int fgk_bin_pgrid_f64(const double *f, const fgk_binspec *b, double *
                        bin_lds(a), (hipStream_t)s, f, (const float *)nullptr,
                        (const float *)nullptr, a, (const double *)nullptr,
                        (const double *)nullptr, 0.0, 0.0, 0.0, 0.0, bins_dev,
-                       (unsigned long long *)counts_dev);
+                       (unsigned long long *)counts_dev, (const fgk_binjob *)nullptr);
     FGK_LAUNCHED();
     return 0;
 }
@@ -554,7 +574,7 @@ extern "C" int fgk_bin_geometry(const fgk_binspec *b, double *bins_dev, int64_t 
                        bin_lds(a), (hipStream_t)s, (const double *)nullptr,
                        (const float *)nullptr, (const float *)nullptr, a,
                        (const double *)nullptr, (const double *)nullptr, 0.0, 0.0, 0.0, 0.0,
-                       bins_dev, (unsigned long long *)counts_dev);
+                       bins_dev, (unsigned long long *)counts_dev, (const fgk_binjob *)nullptr);
     FGK_LAUNCHED();
     return 0;
 }
@@ -571,11 +591,41 @@ extern "C" int fgk_fused_bin_f32(const float *A, const float *B, const fgk_binsp
     const int Ng = b->Ng;
     hipLaunchKernelGGL(k_bin<1>, dim3(bin_blocks(Ng, Ng / 2)), dim3(BIN_THREADS), bin_lds(a),
                        (hipStream_t)s, (const double *)nullptr, A, B, a, phase_dev, winv_dev,
-                       norm, Vinv, Pshot, npart, bins_dev, (unsigned long long *)nullptr);
+                       norm, Vinv, Pshot, npart, bins_dev, (unsigned long long *)nullptr,
+                       (const fgk_binjob *)nullptr);
     FGK_LAUNCHED();
     const unsigned nplane = (unsigned)(((Ng + PI - 1) / PI) * ((Ng + PJ - 1) / PJ));
     hipLaunchKernelGGL(k_bin_plane, dim3(nplane), dim3(PI * PJ), bin_lds(a), (hipStream_t)s, A, B,
-                       a, phase_dev, winv_dev, norm, Vinv, Pshot, npart, bins_dev);
+                       a, phase_dev, winv_dev, norm, Vinv, Pshot, npart, bins_dev,
+                       (const fgk_binjob *)nullptr);
+    FGK_LAUNCHED();
+    return 0;
+}
+
+extern "C" int fgk_fused_bin_f32_batch(const fgk_binspec *b, const fgk_binjob *jobs_dev,
+                                       int njobs, int NK_max, const double *phase_dev,
+                                       const double *winv_dev, fgk_stream_t s) {
+    if (b->geometry) {
+        fgk_set_error("fgk_fused_bin_f32_batch: geometry is computed by fgk_bin_geometry");
+        return -2;
+    }
+    if (njobs < 1 || njobs > 65535) {
+        fgk_set_error("fgk_fused_bin_f32_batch: need 1..65535 catalogs");
+        return -2;
+    }
+    fgk_binspec bb = *b;
+    bb.NK = NK_max;          // the LDS window size does not depend on NK
+    BinArgs a = make_args(&bb);
+    const int Ng = b->Ng;
+    hipLaunchKernelGGL(k_bin<1>, dim3(bin_blocks(Ng, Ng / 2), (unsigned)njobs), dim3(BIN_THREADS),
+                       bin_lds(a), (hipStream_t)s, (const double *)nullptr, (const float *)nullptr,
+                       (const float *)nullptr, a, phase_dev, winv_dev, 0.0, 0.0, 0.0, 0.0,
+                       (double *)nullptr, (unsigned long long *)nullptr, jobs_dev);
+    FGK_LAUNCHED();
+    const unsigned nplane = (unsigned)(((Ng + PI - 1) / PI) * ((Ng + PJ - 1) / PJ));
+    hipLaunchKernelGGL(k_bin_plane, dim3(nplane, (unsigned)njobs), dim3(PI * PJ), bin_lds(a),
+                       (hipStream_t)s, (const float *)nullptr, (const float *)nullptr, a,
+                       phase_dev, winv_dev, 0.0, 0.0, 0.0, 0.0, (double *)nullptr, jobs_dev);
     FGK_LAUNCHED();
     return 0;
 }

@@ -188,6 +188,28 @@ struct fgk_paintplan {
     uint64_t *sorted1, *sorted2;
 };
 
+// One catalog of a batched paint (fgk_paintbatch): its positions, grid
+// geometry and grids, and where its work sits in the batch's launches.
+struct PaintJob {
+    const float *x, *y, *z;
+    long Np;
+    KeyGeo g;
+    unsigned long long *GA, *GB;
+    int nb1;             // coarse buckets (slots: nb1 * MAX_D2, MAX_D2 per bucket)
+    int interlace;
+};
+
+// job owning entry e of a batch's per-job ranges off[0..njobs] (uniform
+// operands: scalar loads)
+__device__ __forceinline__ int batch_job(long e, const long *__restrict__ off, int njobs) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= e) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 // wrapped plain-grid PCS base cell of one coordinate, its tile and the
 // tile-local position li + d as POS_BITS-bit fixed point (F fraction bits).
 // The cell index is taken in FP64 as the reference does (src/fft.c:68-76).
@@ -291,12 +313,16 @@ __device__ __forceinline__ bool chunk_range2(long j, const uint32_t *chunk0, con
 
 // LEVEL: 1 reads float positions (digit = coarse bucket); 2 reads packed
 // words of sorted1 (digit = tile within bucket).  COUNT: histogram only.
+// Batched level 1 (jobs != nullptr): chunk j belongs to the job whose chunk
+// range jc1[job] <= j < jc1[job+1] holds it, whose positions and geometry
+// replace x, y, z, Np and g.
 template <int LEVEL, int COUNT>
 __global__ void __launch_bounds__(SORT_THREADS)
 k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
        long Np, KeyGeo g, const uint64_t *__restrict__ in, const uint32_t *__restrict__ chunk0,
        const uint32_t *__restrict__ base1, const uint32_t *__restrict__ T1, int nb1, int nd,
-       uint32_t *__restrict__ H, const uint32_t *__restrict__ O, uint64_t *__restrict__ out) {
+       uint32_t *__restrict__ H, const uint32_t *__restrict__ O, uint64_t *__restrict__ out,
+       const PaintJob *__restrict__ jobs, const long *__restrict__ jc1, int njobs) {
     constexpr int NB = LEVEL == 1 ? 8 : 7;       // digit bits: MAX_D1, MAX_D2 digits
     __shared__ uint32_t hist[MAX_D1];
     __shared__ uint32_t off[MAX_D1];
@@ -306,11 +332,22 @@ k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__
     __shared__ long s_start;
     __shared__ int s_n;
     const long j = blockIdx.x;
+    long cj = j;                                 // chunk within its catalog
+    if (LEVEL == 1 && jobs) {
+        const int jb = batch_job(j, jc1, njobs);
+        const PaintJob &J = jobs[jb];
+        x = J.x;
+        y = J.y;
+        z = J.z;
+        Np = J.Np;
+        g = J.g;
+        cj = j - jc1[jb];
+    }
     if (threadIdx.x == 0) {
         long st = 0;
         int n = 0;
         if (LEVEL == 1) {
-            st = j * SORT_CHUNK;
+            st = cj * SORT_CHUNK;
             const long m = Np - st;
             n = m <= 0 ? 0 : (m < SORT_CHUNK ? (int)m : SORT_CHUNK);
         } else if (!chunk_range2(j, chunk0, base1, T1, nb1, st, n)) {
@@ -486,11 +523,14 @@ k_digitscan(const uint32_t *__restrict__ v, long n, int mode, int cap,
     }
 }
 
-// O[j][d] = base[s*nd + d] + B[b][d] + sum of H[j'][d] for the earlier
-// chunks j' of block b (s = segment of block b)
+// O[j][d] = base[bidx(s, d)] + B[b][d] + sum of H[j'][d] for the earlier
+// chunks j' of block b (s = segment of block b), with
+// bidx(s, d) = (s / spj) * jst + (s % spj) * nd + d: spj segments per job,
+// jst base entries per job (one job: spj = nseg)
 __global__ void k_colscan(const uint32_t *__restrict__ H, const uint32_t *__restrict__ blk0,
                           int nseg, long nblk, int nd, const uint32_t *__restrict__ base,
-                          const uint32_t *__restrict__ B, uint32_t *__restrict__ O) {
+                          const uint32_t *__restrict__ B, uint32_t *__restrict__ O, int spj,
+                          long jst) {
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= nblk * nd) return;
     const long b = t / nd;
@@ -501,7 +541,7 @@ __global__ void k_colscan(const uint32_t *__restrict__ H, const uint32_t *__rest
         const int mid = (lo + hi + 1) >> 1;
         if ((long)blk0[mid] <= b) lo = mid; else hi = mid - 1;
     }
-    uint32_t run = base[(long)lo * nd + d] + B[t];
+    uint32_t run = base[(long)(lo / spj) * jst + (long)(lo % spj) * nd + d] + B[t];
     for (int r = 0; r < RB; ++r) {
         const long j = b * RB + r;
         O[j * nd + d] = run;
@@ -764,18 +804,38 @@ __device__ __forceinline__ void bank_sort(uint64_t pk, bool valid, int F, uint32
 // plain footprint, then the dual one, each time after a bank sort; the
 // stencils are accumulated as int32 fixed point, then the footprints are
 // flushed.
+// Batched (jobs != nullptr): workgroup b paints item ji0[ja] + b of the
+// batch; the job owning it (ji0[job] <= item < ji0[job+1], ja <= job < jb)
+// supplies the geometry, grids and tile tables (tilebase, items + job * sl,
+// MAX_D2 slots per coarse bucket).
 template <int T>
 __global__ void __launch_bounds__(PAINT_THREADS, 8)
 k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ tilebase,
               const uint32_t *__restrict__ items, long nslots, KeyGeo g, int nb2, float wscale,
               int cap, int interlace, unsigned long long *__restrict__ GA,
-              unsigned long long *__restrict__ GB) {
+              unsigned long long *__restrict__ GB, const PaintJob *__restrict__ jobs,
+              const long *__restrict__ ji0, int ja, int jb, long sl) {
     constexpr int SA = T + 3, SB = T + 4, P = paint_stride(T);
     constexpr int NA = SA * P * P, NB = SB * P * P;
     extern __shared__ int lds[];
     __shared__ uint32_t hist[64], off[64];
+    long it = blockIdx.x;
+    if (jobs) {
+        const long e = ji0[ja] + blockIdx.x;
+        const int job = ja + batch_job(e, ji0 + ja, jb - ja);
+        const PaintJob &J = jobs[job];
+        it = e - ji0[job];
+        tilebase += job * sl;
+        items += job * sl;
+        nslots = (long)J.nb1 * MAX_D2;
+        g = J.g;
+        nb2 = MAX_D2;
+        interlace = J.interlace;
+        GA = J.GA;
+        GB = J.GB;
+    }
     long t, p0, p1;
-    if (!item_range(blockIdx.x, tilebase, items, nslots, cap, t, p0, p1)) return;
+    if (!item_range(it, tilebase, items, nslots, cap, t, p0, p1)) return;
     const int nb = interlace ? NB : 0;
     int *fa = lds, *fb = lds + NA;
     uint64_t *stage = (uint64_t *)(lds + NA + nb);             // [PAINT_THREADS]
@@ -835,61 +895,80 @@ static int ceil_log2(long v) {
 
 static long round_up(long v, long m) { return (v + m - 1) / m * m; }
 
+// tile sides with a compiled paint kernel
+static const int kSides[] = {16, 12, 8, 4, 2, 1};
+
+static int paint_side(int Ng, int tile) {
+    int T = tile > 0 ? tile : 16;
+    if (T > Ng) T = Ng;
+    for (int i = 0; i < 6; ++i)
+        if (kSides[i] <= T) return kSides[i];
+    return 1;
+}
+
+// fixed-point fraction bits S of the weights for cap particles per item:
+// largest S with cap * (2/3)^3 * 2^S < 2^31 (int32 LDS sums cannot
+// overflow) and every rounded weight below 2^22 (single-FMA rounding)
+static int fixed_bits(int cap) {
+    int S = 0;
+    while ((double)cap * (8.0 / 27.0) * ldexp(1.0, S + 1) < 2147483647.0 &&
+           (8.0 / 27.0) * ldexp(1.0, S + 1) < 4194304.0)
+        ++S;
+    return S;
+}
+
+// Tile geometry of a grid of side Ng painted in tiles of side T: coarse
+// buckets group up to MAX_D2 tiles, doubling the group along the axis with
+// the most buckets, so that small grids get few, large buckets (few level-2
+// segments, long level-1 runs).  Two levels when at most MAX_D1 buckets
+// remain, else every tile is its own bucket (single level).
+static KeyGeo tile_geometry(int Ng, int T, int &two_level) {
+    const int nt = (Ng + T - 1) / T;
+    int cg[3] = {1, 1, 1}, nc[3] = {nt, nt, nt};
+    while (cg[0] * cg[1] * cg[2] < MAX_D2 && (nc[0] > 1 || nc[1] > 1 || nc[2] > 1)) {
+        int ax = 2;
+        if (nc[1] > nc[ax]) ax = 1;
+        if (nc[0] > nc[ax]) ax = 0;
+        cg[ax] *= 2;
+        nc[ax] = (nt + cg[ax] - 1) / cg[ax];
+    }
+    two_level = (long)nc[0] * nc[1] * nc[2] <= MAX_D1;
+    if (!two_level) {
+        cg[0] = cg[1] = cg[2] = 1;
+        nc[0] = nc[1] = nc[2] = nt;
+    }
+    KeyGeo g = {0.0,   Ng,    T,     nt,    POS_BITS - ceil_log2(T),
+                cg[0], cg[1], cg[2], nc[0], nc[1],
+                nc[2], ceil_log2(cg[0]), ceil_log2(cg[1]), ceil_log2(cg[2]),
+                ((1ull << 32) + (uint64_t)T - 1) / (uint64_t)T};
+    return g;
+}
+
 extern "C" int fgk_paint_plan_create(fgk_paintplan **out, int Ng, int tile, long Np_max,
                                      int interlace) {
     *out = NULL;
     if (Ng < 2 || Ng > (1 << 20)) { fgk_set_error("paint: Ng must be in [2, 2^20]"); return -2; }
     if (Np_max < 1) Np_max = 1;
     if (Np_max >= (1l << 32) - 1) { fgk_set_error("paint: at most 2^32-2 particles"); return -2; }
-    // tile sides with a compiled paint kernel: 16, 12, 8, 4, 2, 1
-    int T = tile > 0 ? tile : 16;
-    if (T > Ng) T = Ng;
-    static const int kSides[] = {16, 12, 8, 4, 2, 1};
-    for (int i = 0; i < 6; ++i)
-        if (kSides[i] <= T) {
-            T = kSides[i];
-            break;
-        }
+    const int T = paint_side(Ng, tile);
     fgk_paintplan *pp = (fgk_paintplan *)calloc(1, sizeof(fgk_paintplan));
     if (!pp) { fgk_set_error("out of host memory"); return -1; }
     pp->Ng = Ng;
     pp->T = T;
-    pp->nt = (Ng + T - 1) / T;
     pp->interlace = interlace ? 1 : 0;
-    pp->F = POS_BITS - ceil_log2(T);
     pp->cap = 8192;
-    // largest S with cap * (2/3)^3 * 2^S < 2^31 (int32 LDS sums cannot
-    // overflow) and every rounded weight below 2^22 (single-FMA rounding)
-    pp->S = 0;
-    while ((double)pp->cap * (8.0 / 27.0) * ldexp(1.0, pp->S + 1) < 2147483647.0 &&
-           (8.0 / 27.0) * ldexp(1.0, pp->S + 1) < 4194304.0)
-        ++pp->S;
+    pp->S = fixed_bits(pp->cap);
     pp->Np_max = Np_max;
-    // coarse buckets: group tiles until at most MAX_D1 buckets remain
-    int cg[3] = {1, 1, 1}, nc[3] = {pp->nt, pp->nt, pp->nt};
-    while ((long)nc[0] * nc[1] * nc[2] > MAX_D1) {
-        int ax = 2;
-        if (nc[1] > nc[ax]) ax = 1;
-        if (nc[0] > nc[ax]) ax = 0;
-        cg[ax] *= 2;
-        nc[ax] = (pp->nt + cg[ax] - 1) / cg[ax];
+    pp->g = tile_geometry(Ng, T, pp->two_level);
+    pp->nt = pp->g.nt;
+    pp->F = pp->g.F;
+    if (!pp->two_level && (long)pp->nt * pp->nt * pp->nt >= (1l << 24)) {
+        fgk_set_error("paint: %d^3 tiles of side %d exceed the 2^24 the sort addresses", pp->nt,
+                      T);
+        free(pp);
+        return -2;
     }
-    pp->two_level = (long)cg[0] * cg[1] * cg[2] <= MAX_D2;
-    if (!pp->two_level) {   // single level: every tile is its own bucket
-        cg[0] = cg[1] = cg[2] = 1;
-        nc[0] = nc[1] = nc[2] = pp->nt;
-        if ((long)pp->nt * pp->nt * pp->nt >= (1l << 24)) {
-            fgk_set_error("paint: %d^3 tiles of side %d exceed the 2^24 the sort addresses",
-                          pp->nt, T);
-            free(pp);
-            return -2;
-        }
-    }
-    KeyGeo g = {0.0,   Ng,    T,     pp->nt, pp->F,
-                cg[0], cg[1], cg[2], nc[0],  nc[1],
-                nc[2], ceil_log2(cg[0]), ceil_log2(cg[1]), ceil_log2(cg[2]),
-                ((1ull << 32) + (uint64_t)T - 1) / (uint64_t)T};
-    pp->g = g;
+    const int nc[3] = {pp->g.nc0, pp->g.nc1, pp->g.nc2}, cg[3] = {pp->g.cg0, pp->g.cg1, pp->g.cg2};
     pp->nb1 = nc[0] * nc[1] * nc[2];
     pp->nb2 = cg[0] * cg[1] * cg[2];
     pp->bits1 = ceil_log2(pp->nb1);
@@ -965,7 +1044,8 @@ static unsigned blocks_for(long n, int threads) { return (unsigned)((n + threads
 // (segment s = blocks [blk0[s], blk0[s+1])) -> O[j][d] output offsets
 static int level_scans(hipStream_t s, const uint32_t *H, long nw, int nd, uint32_t *S, uint32_t *B,
                        const uint32_t *blk0, int nseg, uint32_t *T, uint32_t *base, int mode,
-                       int cap, uint32_t *aux, uint32_t *aux_blk, uint32_t *O, long nslots) {
+                       int cap, uint32_t *aux, uint32_t *aux_blk, uint32_t *O, long nslots,
+                       int spj, long jst) {
     const long nblk = nw / RB;
     hipLaunchKernelGGL(k_blocksum, dim3(blocks_for(nblk * nd, 256)), dim3(256), 0, s, H, nblk, nd,
                        S);
@@ -980,7 +1060,7 @@ static int level_scans(hipStream_t s, const uint32_t *H, long nw, int nd, uint32
                        aux, aux_blk);
     FGK_LAUNCHED();
     hipLaunchKernelGGL(k_colscan, dim3(blocks_for(nblk * nd, 256)), dim3(256), 0, s, H, blk0, nseg,
-                       nblk, nd, base, B, O);
+                       nblk, nd, base, B, O, spj, jst);
     FGK_LAUNCHED();
     return 0;
 }
@@ -1032,7 +1112,7 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
                                z, Np, g, (const uint64_t *)nullptr, (const uint32_t *)nullptr,
                                (const uint32_t *)nullptr, (const uint32_t *)nullptr, pp->nb1,
                                pp->nb1, pp->H1, (const uint32_t *)nullptr,
-                               (uint64_t *)nullptr);
+                               (uint64_t *)nullptr, (const PaintJob *)nullptr, (const long *)nullptr, 0);
             FGK_LAUNCHED();
             return 0;
         case 1:   // level 1 offsets and the level 2 chunk layout
@@ -1040,12 +1120,13 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
                                (uint32_t)(nw1 / RB));
             FGK_LAUNCHED();
             return level_scans(s, pp->H1, nw1, pp->nb1, pp->S1, pp->B1, pp->seg1, 1, pp->T1,
-                               pp->base1, 1, pp->cap, pp->chunk0, pp->blk0, pp->O1, pp->nb1);
+                               pp->base1, 1, pp->cap, pp->chunk0, pp->blk0, pp->O1, pp->nb1, 1,
+                               (long)pp->nb1);
         case 2:
             hipLaunchKernelGGL((k_sort<1, 0>), dim3((unsigned)nw1), dim3(SORT_THREADS), 0, s, x, y,
                                z, Np, g, (const uint64_t *)nullptr, (const uint32_t *)nullptr,
                                (const uint32_t *)nullptr, (const uint32_t *)nullptr, pp->nb1,
-                               pp->nb1, (uint32_t *)nullptr, pp->O1, pp->sorted1);
+                               pp->nb1, (uint32_t *)nullptr, pp->O1, pp->sorted1, (const PaintJob *)nullptr, (const long *)nullptr, 0);
             FGK_LAUNCHED();
             return 0;
         case 3: { // level 2 histograms (every capacity chunk; empty ones write zeros)
@@ -1054,7 +1135,7 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
                                (const float *)nullptr, (const float *)nullptr,
                                (const float *)nullptr, Np, g, pp->sorted1, pp->chunk0, pp->base1,
                                pp->T1, pp->nb1, pp->nb2, pp->H2,
-                               (const uint32_t *)nullptr, (uint64_t *)nullptr);
+                               (const uint32_t *)nullptr, (uint64_t *)nullptr, (const PaintJob *)nullptr, (const long *)nullptr, 0);
             FGK_LAUNCHED();
             return 0;
         }
@@ -1062,7 +1143,7 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
             const long nw2 = nw1 + (long)pp->nb1 * RB;
             return level_scans(s, pp->H2, nw2, pp->nb2, pp->S2, pp->B2, pp->blk0, pp->nb1, pp->T2,
                                pp->tilebase, 2, pp->cap, pp->items, (uint32_t *)nullptr, pp->O2,
-                               pp->nslots);
+                               pp->nslots, pp->nb1, 0l);
         }
         case 5: {
             const long nw2 = nw1 + (long)pp->nb1 * RB;
@@ -1070,7 +1151,7 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
                                (const float *)nullptr, (const float *)nullptr,
                                (const float *)nullptr, Np, g, pp->sorted1, pp->chunk0, pp->base1,
                                pp->T1, pp->nb1, pp->nb2, (uint32_t *)nullptr, pp->O2,
-                               pp->sorted2);
+                               pp->sorted2, (const PaintJob *)nullptr, (const long *)nullptr, 0);
             FGK_LAUNCHED();
             return 0;
         }
@@ -1086,7 +1167,8 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
         unsigned long long *A = (unsigned long long *)gridA, *B = (unsigned long long *)gridB;
 #define PAINT_T(TT)                                                                           \
     hipLaunchKernelGGL(k_paint_tiles<TT>, grid, block, lds, s, pp->sorted2, pp->tilebase,     \
-                       pp->items, pp->nslots, g, pp->nb2, ws, pp->cap, pp->interlace, A, B)
+                       pp->items, pp->nslots, g, pp->nb2, ws, pp->cap, pp->interlace, A, B,    \
+                       (const PaintJob *)nullptr, (const long *)nullptr, 0, 0, 0l)
         switch (pp->T) {
         case 16: PAINT_T(16); break;
         case 12: PAINT_T(12); break;
@@ -1112,4 +1194,391 @@ extern "C" int fgk_paint_fixed(fgk_paintplan *pp, const float *x, const float *y
     for (int stage = 0; stage < fgk_paint_nstages(); ++stage)
         if (fgk_paint_stage(pp, stage, x, y, z, Np, Hinv, gridA, gridB, s)) return -1;
     return 0;
+}
+
+// ------------------------------------------------------- batched catalogs
+//
+// Many catalogs, each with its own grid (side, box, tile side), painted by
+// one launch per sort stage and one paint launch per tile side.  Level 1
+// runs over the concatenated 4096-particle chunks of all catalogs (each
+// catalog's chunk range a multiple of RB, its digits its own <= 256 coarse
+// buckets); level 2 over the (catalog, bucket) segments; the tile tables of
+// catalog j sit at j * BATCH_SL with MAX_D2 slots per bucket.  The sorted
+// words of catalog j follow those of catalogs 0..j-1, because the level-1
+// offsets are one exclusive scan over (catalog, bucket).
+
+#define BATCH_SL ((long)MAX_D1 * MAX_D2 + 1)   // tile-table entries per catalog
+
+// per catalog j (one workgroup each): exclusive scans of its slot counts
+// T2[j][s], s < nb1_j * MAX_D2, into tilebase (starting at the catalog's
+// first sorted position base1[j * MAX_D1]) and paint items (ceil(count/cap))
+__global__ void __launch_bounds__(SCAN_THREADS)
+k_digitscan_jobs(const uint32_t *__restrict__ T2, const uint32_t *__restrict__ base1,
+                 const PaintJob *__restrict__ jobs, int cap, uint32_t *__restrict__ tilebase,
+                 uint32_t *__restrict__ items) {
+    __shared__ uint32_t sa[SCAN_THREADS], sb[SCAN_THREADS];
+    const int job = blockIdx.x;
+    const long n = (long)jobs[job].nb1 * MAX_D2;
+    const uint32_t *v = T2 + (long)job * MAX_D1 * MAX_D2;
+    uint32_t *base = tilebase + job * BATCH_SL, *aux = items + job * BATCH_SL;
+    const uint32_t b0 = base1[(long)job * MAX_D1];
+    const long per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
+    const long i0 = threadIdx.x * per;
+    const long i1 = i0 + per < n ? i0 + per : n;
+    uint32_t ca = 0, cb = 0;
+    for (long i = i0; i < i1; ++i) {
+        const uint32_t c = v[i];
+        ca += c;
+        cb += (c + cap - 1) / cap;
+    }
+    sa[threadIdx.x] = ca;
+    sb[threadIdx.x] = cb;
+    __syncthreads();
+    for (int o = 1; o < SCAN_THREADS; o <<= 1) {
+        const uint32_t va = threadIdx.x >= o ? sa[threadIdx.x - o] : 0;
+        const uint32_t vb = threadIdx.x >= o ? sb[threadIdx.x - o] : 0;
+        __syncthreads();
+        sa[threadIdx.x] += va;
+        sb[threadIdx.x] += vb;
+        __syncthreads();
+    }
+    uint32_t ra = b0 + sa[threadIdx.x] - ca, rb = sb[threadIdx.x] - cb;
+    for (long i = i0; i < i1; ++i) {
+        base[i] = ra;
+        aux[i] = rb;
+        const uint32_t c = v[i];
+        ra += c;
+        rb += (c + cap - 1) / cap;
+    }
+    if (threadIdx.x == SCAN_THREADS - 1) {
+        base[n] = b0 + sa[SCAN_THREADS - 1];
+        aux[n] = sb[SCAN_THREADS - 1];
+    }
+}
+
+struct fgk_paintbatch {
+    int njobs, S, cap;
+    long NW1, NW2, Np_tot;
+    int nsides, side[6], sja[7];   // jobs [sja[i], sja[i+1]) paint with tile side side[i]
+    int *perm;                     // internal job -> caller's job
+    PaintJob *hjobs, *djobs;
+    long *hNpmax;                  // per internal job
+    long *hc1, *hi0, *dc1, *di0;   // [njobs+1] chunk and paint-item offsets
+    int interlace_any;
+    uint32_t *blk1;                // [njobs+1] level-1 segments in RB-chunk blocks
+    uint32_t *H1, *O1, *S1, *B1, *T1;
+    uint32_t *base1, *chunk0, *blk2;
+    uint32_t *H2, *O2, *S2, *B2, *T2;
+    uint32_t *tilebase, *items;
+    uint64_t *sorted1, *sorted2;
+    int table_ok;                  // djobs holds hjobs
+};
+
+extern "C" void fgk_paint_batch_destroy(fgk_paintbatch *pb) {
+    if (!pb) return;
+    uint32_t *bufs[] = {pb->blk1, pb->H1,    pb->O1,     pb->S1,       pb->B1,   pb->T1,
+                        pb->base1, pb->chunk0, pb->blk2, pb->H2,       pb->O2,   pb->S2,
+                        pb->B2,   pb->T2,    pb->tilebase, pb->items};
+    for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; ++i) fgk_free(bufs[i]);
+    fgk_free(pb->sorted1);
+    fgk_free(pb->sorted2);
+    fgk_free(pb->djobs);
+    fgk_free(pb->dc1);
+    fgk_free(pb->di0);
+    free(pb->perm);
+    free(pb->hjobs);
+    free(pb->hNpmax);
+    free(pb->hc1);
+    free(pb->hi0);
+    free(pb);
+}
+
+extern "C" int fgk_paint_batch_create(fgk_paintbatch **out, int njobs,
+                                      const fgk_paintjob_desc *jobs) {
+    *out = NULL;
+    if (njobs < 1 || njobs > 65535) {
+        fgk_set_error("paint batch: need 1..65535 catalogs");
+        return -2;
+    }
+    fgk_paintbatch *pb = (fgk_paintbatch *)calloc(1, sizeof(fgk_paintbatch));
+    if (!pb) { fgk_set_error("out of host memory"); return -1; }
+    pb->njobs = njobs;
+    pb->cap = 8192;
+    pb->S = fixed_bits(pb->cap);
+    pb->perm = (int *)calloc((size_t)njobs, sizeof(int));
+    pb->hjobs = (PaintJob *)calloc((size_t)njobs, sizeof(PaintJob));
+    pb->hNpmax = (long *)calloc((size_t)njobs, sizeof(long));
+    pb->hc1 = (long *)calloc((size_t)njobs + 1, sizeof(long));
+    pb->hi0 = (long *)calloc((size_t)njobs + 1, sizeof(long));
+    if (!pb->perm || !pb->hjobs || !pb->hNpmax || !pb->hc1 || !pb->hi0) {
+        fgk_set_error("out of host memory");
+        fgk_paint_batch_destroy(pb);
+        return -1;
+    }
+    // internal order: by tile side (largest first), so that each side's
+    // paint launch covers one contiguous range of work items
+    int n = 0;
+    for (int si = 0; si < 6; ++si) {
+        const int T = kSides[si];
+        int first = n;
+        for (int j = 0; j < njobs; ++j) {
+            const fgk_paintjob_desc &d = jobs[j];
+            if (d.Ng < 2 || d.Ng > (1 << 20) || !(d.Hinv > 0) || d.Np_max < 1 || !d.gridA ||
+                (d.interlace && !d.gridB)) {
+                fgk_set_error("paint batch: invalid catalog %d", j);
+                fgk_paint_batch_destroy(pb);
+                return -2;
+            }
+            if (paint_side(d.Ng, d.tile) == T) pb->perm[n++] = j;
+        }
+        if (n > first) {
+            pb->side[pb->nsides] = T;
+            pb->sja[pb->nsides] = first;
+            ++pb->nsides;
+        }
+    }
+    pb->sja[pb->nsides] = n;
+    long Np_tot = 0, c1 = 0, i0 = 0;
+    long nb1_tot = 0;
+    for (int q = 0; q < njobs; ++q) {
+        const fgk_paintjob_desc &d = jobs[pb->perm[q]];
+        const int T = paint_side(d.Ng, d.tile);
+        int two_level = 0;
+        KeyGeo g = tile_geometry(d.Ng, T, two_level);
+        if (!two_level) {
+            fgk_set_error("paint batch: catalog %d, Ng=%d with tiles of side %d has more tiles "
+                          "than a batch addresses (%d x %d); paint it with its own plan",
+                          pb->perm[q], d.Ng, T, MAX_D1, MAX_D2);
+            fgk_paint_batch_destroy(pb);
+            return -2;
+        }
+        g.Hinv = d.Hinv;
+        PaintJob &J = pb->hjobs[q];
+        J.g = g;
+        J.GA = (unsigned long long *)d.gridA;
+        J.GB = (unsigned long long *)(d.interlace ? d.gridB : nullptr);
+        J.nb1 = g.nc0 * g.nc1 * g.nc2;
+        J.interlace = d.interlace ? 1 : 0;
+        pb->interlace_any |= J.interlace;
+        pb->hNpmax[q] = d.Np_max;
+        Np_tot += d.Np_max;
+        nb1_tot += J.nb1;
+        pb->hc1[q] = c1;
+        c1 += round_up((d.Np_max + SORT_CHUNK - 1) / SORT_CHUNK, RB);
+        pb->hi0[q] = i0;
+        i0 += (long)J.nb1 * (g.cg0 * g.cg1 * g.cg2) + (d.Np_max + pb->cap - 1) / pb->cap;
+    }
+    pb->hc1[njobs] = c1;
+    pb->hi0[njobs] = i0;
+    pb->Np_tot = Np_tot;
+    pb->NW1 = c1;
+    pb->NW2 = c1 + nb1_tot * RB;
+    if (Np_tot >= (1l << 32) - 1) {
+        fgk_set_error("paint batch: at most 2^32-2 particles over all catalogs (%ld)", Np_tot);
+        fgk_paint_batch_destroy(pb);
+        return -2;
+    }
+    if (pb->NW2 >= (1l << 32) / SORT_THREADS || i0 * (long)PAINT_THREADS >= (1l << 32)) {
+        fgk_set_error("paint batch: too much work for one dispatch per stage");
+        fgk_paint_batch_destroy(pb);
+        return -2;
+    }
+    const long nseg2 = (long)njobs * MAX_D1;
+    const size_t u = sizeof(uint32_t);
+    int rc = 0;
+    rc |= fgk_malloc((void **)&pb->djobs, (size_t)njobs * sizeof(PaintJob));
+    rc |= fgk_malloc((void **)&pb->dc1, (size_t)(njobs + 1) * sizeof(long));
+    rc |= fgk_malloc((void **)&pb->di0, (size_t)(njobs + 1) * sizeof(long));
+    rc |= fgk_malloc((void **)&pb->blk1, (size_t)(njobs + 1) * u);
+    rc |= fgk_malloc((void **)&pb->H1, (size_t)pb->NW1 * MAX_D1 * u);
+    rc |= fgk_malloc((void **)&pb->O1, (size_t)pb->NW1 * MAX_D1 * u);
+    rc |= fgk_malloc((void **)&pb->S1, (size_t)(pb->NW1 / RB) * MAX_D1 * u);
+    rc |= fgk_malloc((void **)&pb->B1, (size_t)(pb->NW1 / RB) * MAX_D1 * u);
+    rc |= fgk_malloc((void **)&pb->T1, (size_t)nseg2 * u);
+    rc |= fgk_malloc((void **)&pb->base1, (size_t)(nseg2 + 1) * u);
+    rc |= fgk_malloc((void **)&pb->chunk0, (size_t)(nseg2 + 1) * u);
+    rc |= fgk_malloc((void **)&pb->blk2, (size_t)(nseg2 + 1) * u);
+    rc |= fgk_malloc((void **)&pb->H2, (size_t)pb->NW2 * MAX_D2 * u);
+    rc |= fgk_malloc((void **)&pb->O2, (size_t)pb->NW2 * MAX_D2 * u);
+    rc |= fgk_malloc((void **)&pb->S2, (size_t)(pb->NW2 / RB) * MAX_D2 * u);
+    rc |= fgk_malloc((void **)&pb->B2, (size_t)(pb->NW2 / RB) * MAX_D2 * u);
+    rc |= fgk_malloc((void **)&pb->T2, (size_t)nseg2 * MAX_D2 * u);
+    rc |= fgk_malloc((void **)&pb->tilebase, (size_t)njobs * BATCH_SL * u);
+    rc |= fgk_malloc((void **)&pb->items, (size_t)njobs * BATCH_SL * u);
+    rc |= fgk_malloc((void **)&pb->sorted1, (size_t)Np_tot * sizeof(uint64_t));
+    rc |= fgk_malloc((void **)&pb->sorted2, (size_t)Np_tot * sizeof(uint64_t));
+    uint32_t *hb = (uint32_t *)malloc((size_t)(njobs + 1) * u);
+    if (!rc && !hb) { fgk_set_error("out of host memory"); rc = -1; }
+    if (!rc) {
+        for (int q = 0; q <= njobs; ++q) hb[q] = (uint32_t)(pb->hc1[q] / RB);
+        rc = fgk_h2d(pb->blk1, hb, (size_t)(njobs + 1) * u, NULL);
+    }
+    if (!rc) rc = fgk_h2d(pb->dc1, pb->hc1, (size_t)(njobs + 1) * sizeof(long), NULL);
+    if (!rc) rc = fgk_h2d(pb->di0, pb->hi0, (size_t)(njobs + 1) * sizeof(long), NULL);
+    if (!rc) rc = fgk_stream_sync(NULL);
+    free(hb);
+    if (rc) {
+        fgk_paint_batch_destroy(pb);
+        return -1;
+    }
+    *out = pb;
+    return 0;
+}
+
+extern "C" double fgk_paint_batch_fixed_scale(const fgk_paintbatch *pb) {
+    return ldexp(1.0, -pb->S);
+}
+
+extern "C" int fgk_paint_batch_tile(const fgk_paintbatch *pb, int job) {
+    for (int q = 0; q < pb->njobs; ++q)
+        if (pb->perm[q] == job) return pb->hjobs[q].g.T;
+    return -1;
+}
+
+extern "C" int fgk_paint_batch_set(fgk_paintbatch *pb, const float *const *x,
+                                   const float *const *y, const float *const *z, const long *Np,
+                                   fgk_stream_t s) {
+    int same = pb->table_ok;
+    for (int q = 0; q < pb->njobs; ++q) {
+        const int j = pb->perm[q];
+        if (Np[j] < 0 || Np[j] > pb->hNpmax[q]) {
+            fgk_set_error("paint batch: catalog %d has Np=%ld outside [0, Np_max=%ld]", j, Np[j],
+                          pb->hNpmax[q]);
+            return -2;
+        }
+        if (Np[j] > 0 && (!x[j] || !y[j] || !z[j])) {
+            fgk_set_error("paint batch: catalog %d has NULL positions", j);
+            return -2;
+        }
+        PaintJob &J = pb->hjobs[q];
+        if (J.x != x[j] || J.y != y[j] || J.z != z[j] || J.Np != Np[j]) same = 0;
+        J.x = x[j];
+        J.y = y[j];
+        J.z = z[j];
+        J.Np = Np[j];
+    }
+    if (same) return 0;
+    // the table is read by kernels already queued: wait for them, copy, wait
+    if (fgk_stream_sync(s)) return -1;
+    if (fgk_h2d(pb->djobs, pb->hjobs, (size_t)pb->njobs * sizeof(PaintJob), s)) return -1;
+    if (fgk_stream_sync(s)) return -1;
+    pb->table_ok = 1;
+    return 0;
+}
+
+extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t st) {
+    hipStream_t s = (hipStream_t)st;
+    if (!pb->table_ok) {
+        fgk_set_error("paint batch: catalogs not set");
+        return -2;
+    }
+    const int nj = pb->njobs;
+    const long nseg2 = (long)nj * MAX_D1;
+    const KeyGeo g0 = pb->hjobs[0].g;   // unused by batched launches
+    switch (stage) {
+    case 0:
+        hipLaunchKernelGGL((k_sort<1, 1>), dim3((unsigned)pb->NW1), dim3(SORT_THREADS), 0, s,
+                           (const float *)nullptr, (const float *)nullptr,
+                           (const float *)nullptr, 0l, g0, (const uint64_t *)nullptr,
+                           (const uint32_t *)nullptr, (const uint32_t *)nullptr,
+                           (const uint32_t *)nullptr, 0, MAX_D1, pb->H1,
+                           (const uint32_t *)nullptr, (uint64_t *)nullptr,
+                           (const PaintJob *)pb->djobs, (const long *)pb->dc1, nj);
+        FGK_LAUNCHED();
+        return 0;
+    case 1: {
+        const long nblk = pb->NW1 / RB;
+        hipLaunchKernelGGL(k_blocksum, dim3(blocks_for(nblk * MAX_D1, 256)), dim3(256), 0, s,
+                           pb->H1, nblk, MAX_D1, pb->S1);
+        FGK_LAUNCHED();
+        hipLaunchKernelGGL(k_segscan, dim3(blocks_for((long)nj * MAX_D1, 256)), dim3(256), 0, s,
+                           pb->S1, pb->blk1, nj, MAX_D1, pb->B1, pb->T1);
+        FGK_LAUNCHED();
+        hipLaunchKernelGGL(k_digitscan, dim3(1), dim3(SCAN_THREADS), 0, s, pb->T1, nseg2, 1,
+                           pb->cap, pb->base1, pb->chunk0, pb->blk2);
+        FGK_LAUNCHED();
+        hipLaunchKernelGGL(k_colscan, dim3(blocks_for(nblk * MAX_D1, 256)), dim3(256), 0, s,
+                           pb->H1, pb->blk1, nj, nblk, MAX_D1, pb->base1, pb->B1, pb->O1, 1,
+                           (long)MAX_D1);
+        FGK_LAUNCHED();
+        return 0;
+    }
+    case 2:
+        hipLaunchKernelGGL((k_sort<1, 0>), dim3((unsigned)pb->NW1), dim3(SORT_THREADS), 0, s,
+                           (const float *)nullptr, (const float *)nullptr,
+                           (const float *)nullptr, 0l, g0, (const uint64_t *)nullptr,
+                           (const uint32_t *)nullptr, (const uint32_t *)nullptr,
+                           (const uint32_t *)nullptr, 0, MAX_D1, (uint32_t *)nullptr, pb->O1,
+                           pb->sorted1, (const PaintJob *)pb->djobs, (const long *)pb->dc1, nj);
+        FGK_LAUNCHED();
+        return 0;
+    case 3:
+        hipLaunchKernelGGL((k_sort<2, 1>), dim3((unsigned)pb->NW2), dim3(SORT_THREADS), 0, s,
+                           (const float *)nullptr, (const float *)nullptr,
+                           (const float *)nullptr, 0l, g0, pb->sorted1, pb->chunk0, pb->base1,
+                           pb->T1, (int)nseg2, MAX_D2, pb->H2, (const uint32_t *)nullptr,
+                           (uint64_t *)nullptr, (const PaintJob *)nullptr, (const long *)nullptr,
+                           0);
+        FGK_LAUNCHED();
+        return 0;
+    case 4: {
+        const long nblk = pb->NW2 / RB;
+        hipLaunchKernelGGL(k_blocksum, dim3(blocks_for(nblk * MAX_D2, 256)), dim3(256), 0, s,
+                           pb->H2, nblk, MAX_D2, pb->S2);
+        FGK_LAUNCHED();
+        hipLaunchKernelGGL(k_segscan, dim3(blocks_for(nseg2 * MAX_D2, 256)), dim3(256), 0, s,
+                           pb->S2, pb->blk2, (int)nseg2, MAX_D2, pb->B2, pb->T2);
+        FGK_LAUNCHED();
+        hipLaunchKernelGGL(k_digitscan_jobs, dim3((unsigned)nj), dim3(SCAN_THREADS), 0, s, pb->T2,
+                           pb->base1, (const PaintJob *)pb->djobs, pb->cap, pb->tilebase,
+                           pb->items);
+        FGK_LAUNCHED();
+        hipLaunchKernelGGL(k_colscan, dim3(blocks_for(nblk * MAX_D2, 256)), dim3(256), 0, s,
+                           pb->H2, pb->blk2, (int)nseg2, nblk, MAX_D2, pb->tilebase, pb->B2,
+                           pb->O2, MAX_D1, BATCH_SL);
+        FGK_LAUNCHED();
+        return 0;
+    }
+    case 5:
+        hipLaunchKernelGGL((k_sort<2, 0>), dim3((unsigned)pb->NW2), dim3(SORT_THREADS), 0, s,
+                           (const float *)nullptr, (const float *)nullptr,
+                           (const float *)nullptr, 0l, g0, pb->sorted1, pb->chunk0, pb->base1,
+                           pb->T1, (int)nseg2, MAX_D2, (uint32_t *)nullptr, pb->O2, pb->sorted2,
+                           (const PaintJob *)nullptr, (const long *)nullptr, 0);
+        FGK_LAUNCHED();
+        return 0;
+    case 6: {
+        const float ws = (float)ldexp(1.0, pb->S);
+        for (int i = 0; i < pb->nsides; ++i) {
+            const int ja = pb->sja[i], jb = pb->sja[i + 1];
+            const long nitems = pb->hi0[jb] - pb->hi0[ja];
+            if (nitems <= 0) continue;
+            const size_t lds = (size_t)paint_lds_bytes(pb->side[i], pb->interlace_any);
+            const dim3 grid((unsigned)nitems), block(PAINT_THREADS);
+#define PAINT_TB(TT)                                                                          \
+    hipLaunchKernelGGL(k_paint_tiles<TT>, grid, block, lds, s, pb->sorted2, pb->tilebase,     \
+                       pb->items, 0l, g0, MAX_D2, ws, pb->cap, 0,                             \
+                       (unsigned long long *)nullptr, (unsigned long long *)nullptr,          \
+                       (const PaintJob *)pb->djobs, (const long *)pb->di0, ja, jb, BATCH_SL)
+            switch (pb->side[i]) {
+            case 16: PAINT_TB(16); break;
+            case 12: PAINT_TB(12); break;
+            case 8: PAINT_TB(8); break;
+            case 4: PAINT_TB(4); break;
+            case 2: PAINT_TB(2); break;
+            case 1: PAINT_TB(1); break;
+            default:
+                fgk_set_error("paint: no kernel instance for tile side %d", pb->side[i]);
+                return -2;
+            }
+#undef PAINT_TB
+            FGK_LAUNCHED();
+        }
+        return 0;
+    }
+    default:
+        break;
+    }
+    fgk_set_error("paint batch: unknown stage %d", stage);
+    return -2;
 }

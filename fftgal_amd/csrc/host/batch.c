@@ -1,16 +1,17 @@
 /* Mixed-precision power-spectrum pipeline (include/fftgal_batch.h).
  *
  * One fftgal_power_t holds everything a catalog needs on the device:
- *   int64 fixed-point grids  gA (plain) and gB (dual)     Ng^3 each
- *   FP32 spectra             fA, fB  (packed half spectrum) Ng^3 floats each
+ *   int64 fixed-point grids  gA (plain) and gB (dual)     Ng^3 each, gB = gA + Ng^3
+ *   FP32 spectra             fA, fB  (packed half spectrum) Ng^3 floats each, fB after fA
+ *   (one allocation each, so that the FFT passes run over both grids in one launch)
  *   paint plan (sort buffers), FFT plan (FP32 twiddles from FP64),
  *   offset-phase and deconvolution tables (FP64, built as src/fft.c:204-245),
  *   bin sums (FP64) and the geometry of the bins (K sums, mode counts),
  *   computed once at creation because they depend only on Ng, L, dK, los
  *   and Nmu.
- * An enqueue runs: zero bins -> fixed-point paint -> per grid (rows r2c
- * from int64, cols j, cols i) -> fused bin kernel -> copy sums to pinned
- * host memory.  The sequence is captured in a hipGraph keyed by the
+ * An enqueue runs: zero bins -> fixed-point paint -> rows r2c from int64,
+ * cols j, cols i (each over both grids) -> fused bin kernel -> copy sums to
+ * pinned host memory.  The sequence is captured in a hipGraph keyed by the
  * catalog pointers, Np and V, and replayed while they stay the same.
  */
 #define _GNU_SOURCE
@@ -69,8 +70,12 @@ const char *fftgal_last_error(void) { return fgk_last_error(); }
         if (rc_) return rc_;      \
     } while (0)
 
-static int compute_geometry(fftgal_power_t *p) {
-    fgk_binspec g = p->spec;
+/* K sums and mode counts of a binning (they depend only on Ng, L, dK, los
+ * and Nmu), normalised by nothing yet: gK[NK], gN[NK], gKmu[NK*W],
+ * gNmu[NK*W] */
+static int bin_geometry(const fgk_binspec *spec, int NK, int W, fgk_stream_t s, double *gK,
+                        long *gN, double *gKmu, long *gNmu) {
+    fgk_binspec g = *spec;
     g.geometry = 1;
     g.do_pl = 1;
     size_t nb = fgk_bins_doubles(&g), nc = fgk_bins_counts(&g);
@@ -81,27 +86,26 @@ static int compute_geometry(fftgal_power_t *p) {
     double *hb = (double *)malloc(nb * sizeof(double));
     long *hc = (long *)malloc(nc * sizeof(long));
     int rc = (!hb || !hc) ? -1 : 0;
-    if (!rc) rc = fgk_fill_zero(db, nb * sizeof(double), p->s);
-    if (!rc) rc = fgk_fill_zero(dc, nc * sizeof(int64_t), p->s);
-    if (!rc) rc = fgk_bin_geometry(&g, db, dc, p->s);
-    if (!rc) rc = fgk_d2h(hb, db, nb * sizeof(double), p->s);
-    if (!rc) rc = fgk_d2h(hc, dc, nc * sizeof(int64_t), p->s);
-    if (!rc) rc = fgk_stream_sync(p->s);
+    if (!rc) rc = fgk_fill_zero(db, nb * sizeof(double), s);
+    if (!rc) rc = fgk_fill_zero(dc, nc * sizeof(int64_t), s);
+    if (!rc) rc = fgk_bin_geometry(&g, db, dc, s);
+    if (!rc) rc = fgk_d2h(hb, db, nb * sizeof(double), s);
+    if (!rc) rc = fgk_d2h(hc, dc, nc * sizeof(int64_t), s);
+    if (!rc) rc = fgk_stream_sync(s);
     if (!rc) {
-        const int NK = p->NK, W = p->W;
         const double *Ks = hb + (size_t)NK * (4 + W);
         const double *Kmu = Ks + NK;
         for (int b = 0; b < NK; ++b) {
-            p->gK[b] = Ks[b];
-            p->gN[b] = hc[b];
+            gK[b] = Ks[b];
+            gN[b] = hc[b];
             for (int m = 0; m < W; ++m) {
-                p->gKmu[(size_t)b * W + m] = Kmu[(size_t)b * W + m];
-                p->gNmu[(size_t)b * W + m] = hc[NK + (size_t)b * W + m];
+                gKmu[(size_t)b * W + m] = Kmu[(size_t)b * W + m];
+                gNmu[(size_t)b * W + m] = hc[NK + (size_t)b * W + m];
             }
         }
         long tot = 0;
-        for (int b = 0; b < NK; ++b) tot += p->gN[b];
-        if (tot != (long)p->d.Ng * p->d.Ng * p->d.Ng - 1) {
+        for (int b = 0; b < NK; ++b) tot += gN[b];
+        if (tot != (long)spec->Ng * spec->Ng * spec->Ng - 1) {
             fgk_set_error("geometry: sum of mode counts %ld != Ng^3-1", tot);
             rc = -3;
         }
@@ -115,16 +119,51 @@ static int compute_geometry(fftgal_power_t *p) {
     return rc;
 }
 
+static int compute_geometry(fftgal_power_t *p) {
+    return bin_geometry(&p->spec, p->NK, p->W, p->s, p->gK, p->gN, p->gKmu, p->gNmu);
+}
+
+/* normalised Pl and Pmu of one catalog from its bin sums hb ([NK][4] then
+ * [NK][W]) and the geometry: src/power.c:70-74 (no guard: empty bins give
+ * NaN like the reference) and src/power.c:139-151 */
+static void normalise_bins(int NK, int W, int Nmu, double KF, const double *hb, const double *gK,
+                           const long *gN, const double *gKmu, const long *gNmu, double *K,
+                           long *N, double *Pl, double *Kmu, long *Nmu_count, double *Pmu) {
+    const double *Psum = hb;                   /* [NK][4] */
+    const double *Pw = hb + (size_t)NK * 4;    /* [NK][W] */
+    for (int b = 0; b < NK; ++b) {
+        long n = gN[b];
+        if (K) K[b] = gK[b] * (KF / n);
+        if (N) N[b] = n;
+        if (Pl)
+            for (int l = 0; l < 4; ++l) Pl[(size_t)b * 4 + l] = Psum[(size_t)b * 4 + l] * ((4 * l + 1.0) / n);
+        for (int m = 0; m < Nmu; ++m) {
+            double ks = gKmu[(size_t)b * W + m], ps = Pw[(size_t)b * W + m];
+            long c = gNmu[(size_t)b * W + m];
+            if (m == Nmu - 1) {
+                ks += gKmu[(size_t)b * W + Nmu];
+                ps += Pw[(size_t)b * W + Nmu];
+                c += gNmu[(size_t)b * W + Nmu];
+            }
+            if (c) {
+                ks *= KF / c;
+                ps /= c;
+            }
+            if (Kmu) Kmu[(size_t)b * Nmu + m] = ks;
+            if (Nmu_count) Nmu_count[(size_t)b * Nmu + m] = c;
+            if (Pmu) Pmu[(size_t)b * Nmu + m] = ps;
+        }
+    }
+}
+
 int fftgal_power_destroy(fftgal_power_t *p) {
     if (!p) return 0;
     if (p->s) fgk_stream_sync(p->s);
     fgk_graph_destroy(p->graph);
     fgk_fft_plan_destroy(p->plan);
     fgk_paint_plan_destroy(p->paint);
-    fgk_free(p->gA);
-    fgk_free(p->gB);
+    fgk_free(p->gA);   /* gB, fB share these allocations */
     fgk_free(p->fA);
-    fgk_free(p->fB);
     fgk_free(p->d_phase);
     fgk_free(p->d_winv);
     fgk_free(p->d_bins);
@@ -180,11 +219,12 @@ int fftgal_power_create(fftgal_power_t **out, const fftgal_power_desc *desc) {
     STEP(fgk_paint_plan_create(&p->paint, Ng, p->tune.paint_tile, desc->Np_max, desc->interlace));
     const size_t n64 = (size_t)Ng * Ng * Ng * sizeof(int64_t);
     const size_t nf = (size_t)Ng * Ng * Ng * sizeof(float);   /* packed half spectrum */
-    STEP(fgk_malloc((void **)&p->gA, n64));
-    STEP(fgk_malloc((void **)&p->fA, nf));
-    if (desc->interlace) {
-        STEP(fgk_malloc((void **)&p->gB, n64));
-        STEP(fgk_malloc((void **)&p->fB, nf));
+    const int ngrids = desc->interlace ? 2 : 1;
+    STEP(fgk_malloc((void **)&p->gA, ngrids * n64));
+    STEP(fgk_malloc((void **)&p->fA, ngrids * nf));
+    if (!rc && desc->interlace) {
+        p->gB = p->gA + (size_t)Ng * Ng * Ng;
+        p->fB = p->fA + (size_t)Ng * Ng * Ng;
     }
     STEP(fgk_malloc((void **)&p->d_phase, (size_t)3 * Ng * 2 * sizeof(double)));
     STEP(fgk_malloc((void **)&p->d_winv, (size_t)Ng * sizeof(double)));
@@ -248,10 +288,22 @@ int fftgal_power_NK(const fftgal_power_t *p) { return p->NK; }
 /* pipeline stages, in order (fftgal_power_profile reports one time each):
  * zero_bins, the paint stages of fgk_paint_stage_name(), the FFT passes of
  * both grids, the fused bin kernel and the copy of the sums to the host */
-static const char *const kTailStages[] = {"fft_rows_A", "fft_cols_j_A", "fft_cols_i_A",
-                                          "fft_rows_B", "fft_cols_j_B", "fft_cols_i_B",
-                                          "fused_bin",  "copy_bins"};
+static const char *const kTailStages[] = {"fft_rows", "fft_cols_j", "fft_cols_i", "fused_bin",
+                                          "copy_bins"};
 #define NSTAGES (1 + NPAINT + (int)(sizeof kTailStages / sizeof kTailStages[0]))
+
+/* normalisations of one catalog's fused bin: norm = 1/(nbar H^3) of the
+ * paint (src/fft.c:125-127) times H^3 of x2k; Vinv and Pshot of P
+ * (src/power.c:15-17, no randoms) */
+static void catalog_norms(int Ng, double L, long Np, double V, double *norm, double *Vinv,
+                          double *Pshot) {
+    const double H3 = (L / Ng) * (L / Ng) * (L / Ng);
+    const double nbar = Np / V;
+    const double Npcinv = 1 / nbar / H3;
+    *norm = Npcinv * H3;
+    *Vinv = 1 / V;
+    *Pshot = V / Np;
+}
 
 /* enqueue the whole pipeline; ev (NSTAGES+1 events or NULL) brackets the
  * stages for profiling */
@@ -259,11 +311,8 @@ static int enqueue_work(fftgal_power_t *p, const float *x, const float *y, const
                         long Np, double V, fgk_event_t *ev) {
     const int Ng = p->d.Ng;
     const double Hinv = Ng / p->d.L;
-    const double H3 = (p->d.L / Ng) * (p->d.L / Ng) * (p->d.L / Ng);
-    const double nbar = Np / V;
-    const double Npcinv = 1 / nbar / H3;       /* src/fft.c:125-127 */
-    const double norm = Npcinv * H3;           /* paint normalisation x H^3 of x2k */
-    const double Vinv = 1 / V, Pshot = V / Np; /* src/power.c:15-17, no randoms */
+    double norm, Vinv, Pshot;
+    catalog_norms(Ng, p->d.L, Np, V, &norm, &Vinv, &Pshot);
     const double in_scale = fgk_paint_fixed_scale(p->paint);
     int st = 0;
 #define MARK() do { if (ev) TRY(fgk_event_record(ev[st], p->s)); ++st; } while (0)
@@ -273,17 +322,13 @@ static int enqueue_work(fftgal_power_t *p, const float *x, const float *y, const
         MARK();
         TRY(fgk_paint_stage(p->paint, stage, x, y, z, Np, Hinv, p->gA, p->gB, p->s));
     }
-    for (int g = 0; g < 2; ++g) {
-        int64_t *src = g ? p->gB : p->gA;
-        float *dst = g ? p->fB : p->fA;
-        const int on = g == 0 || p->d.interlace;
-        MARK();
-        if (on) TRY(fgk_fft_rows_f32_from_fixed(p->plan, dst, src, in_scale, 1, p->s));
-        MARK();
-        if (on) TRY(fgk_fft_cols_f32(p->plan, dst, 1, -1, p->s));
-        MARK();
-        if (on) TRY(fgk_fft_cols_f32(p->plan, dst, 0, -1, p->s));
-    }
+    const int ngrids = p->d.interlace ? 2 : 1;
+    MARK();
+    TRY(fgk_fft_rows_f32_from_fixed(p->plan, p->fA, p->gA, in_scale, 1, ngrids, p->s));
+    MARK();
+    TRY(fgk_fft_cols_f32(p->plan, p->fA, 1, -1, ngrids, p->s));
+    MARK();
+    TRY(fgk_fft_cols_f32(p->plan, p->fA, 0, -1, ngrids, p->s));
     MARK();
     TRY(fgk_fused_bin_f32(p->fA, p->d.interlace ? p->fB : NULL, &p->spec, p->d_phase, p->d_winv,
                           norm, Vinv, Pshot, (double)Np, p->d_bins, p->s));
@@ -377,35 +422,8 @@ int fftgal_power_sync(fftgal_power_t *p) { return fgk_stream_sync(p->s); }
 
 int fftgal_power_results(fftgal_power_t *p, double *K, long *N, double *Pl, double *Kmu,
                          long *Nmu_count, double *Pmu) {
-    const int NK = p->NK, W = p->W, Nmu = p->d.Nmu;
-    const double KF = p->KF;
-    const double *Psum = p->h_bins;                  /* [NK][4] */
-    const double *Pw = p->h_bins + (size_t)NK * 4;   /* [NK][W] */
-    for (int b = 0; b < NK; ++b) {
-        /* src/power.c:70-74 (no guard: empty bins give NaN like the reference) */
-        long n = p->gN[b];
-        if (K) K[b] = p->gK[b] * (KF / n);
-        if (N) N[b] = n;
-        if (Pl)
-            for (int l = 0; l < 4; ++l) Pl[(size_t)b * 4 + l] = Psum[(size_t)b * 4 + l] * ((4 * l + 1.0) / n);
-        /* src/power.c:139-151 */
-        for (int m = 0; m < Nmu; ++m) {
-            double ks = p->gKmu[(size_t)b * W + m], ps = Pw[(size_t)b * W + m];
-            long c = p->gNmu[(size_t)b * W + m];
-            if (m == Nmu - 1) {
-                ks += p->gKmu[(size_t)b * W + Nmu];
-                ps += Pw[(size_t)b * W + Nmu];
-                c += p->gNmu[(size_t)b * W + Nmu];
-            }
-            if (c) {
-                ks *= KF / c;
-                ps /= c;
-            }
-            if (Kmu) Kmu[(size_t)b * Nmu + m] = ks;
-            if (Nmu_count) Nmu_count[(size_t)b * Nmu + m] = c;
-            if (Pmu) Pmu[(size_t)b * Nmu + m] = ps;
-        }
-    }
+    normalise_bins(p->NK, p->W, p->d.Nmu, p->KF, p->h_bins, p->gK, p->gN, p->gKmu, p->gNmu, K, N,
+                   Pl, Kmu, Nmu_count, Pmu);
     return 0;
 }
 
@@ -461,3 +479,398 @@ int fftgal_memcpy_d2h(void *dst, const void *src, unsigned long bytes) {
     TRY(fgk_d2h(dst, src, bytes, NULL));
     return fgk_stream_sync(NULL);
 }
+
+/* ------------------------------------------------------------------------
+ * Batches of catalogs (fftgal_batch_*): many catalogs, each on its own grid
+ * (side, box = catalog box / fold, interlacing, k binning), through one
+ * launch per stage:
+ *   sort and paint  -- all catalogs together (fgk_paintbatch; one paint
+ *                      launch per tile side),
+ *   FFT passes      -- per grid side: the grids of every catalog of that
+ *                      side and interlacing lie one after the other, so the
+ *                      row pass and both column passes cover them all,
+ *   fused bin       -- per grid side, one launch pair with a catalog table,
+ *   bins            -- one zero fill and one copy to pinned host memory.
+ * The whole sequence is one hipGraph; catalog pointers, Np and V live in
+ * device tables, so a new set of catalogs updates the tables and replays
+ * the same graph.
+ * ---------------------------------------------------------------------- */
+
+typedef struct batch_group {
+    int Ng, interlace, n;        /* grids of side Ng; n catalogs */
+    int *jobs;                   /* caller's catalog indices, in grid order */
+    fgk_fftplan *plan;
+    int64_t *g;                  /* n * G int64 grids */
+    float *f;                    /* n * G packed spectra */
+    double *d_phase, *d_winv;
+    fgk_binspec spec;            /* Ng, Nmu, los, do_pl (NK per catalog) */
+    int NK_max;
+    fgk_binjob *hjob, *djob;
+} batch_group;
+
+typedef struct batch_job {
+    fftgal_job_desc d;
+    int NK, group, slot;
+    double KF;
+    size_t boff;                 /* offset of its bin sums in d_bins / h_bins */
+    double *gK, *gKmu;
+    long *gN, *gNmu;
+    long Np;
+    double V;
+} batch_job;
+
+struct fftgal_batch {
+    int njobs, ngroups, Nmu, W;
+    double los[3];
+    batch_job *job;
+    batch_group *grp;
+    fgk_stream_t s;
+    fgk_paintbatch *paint;
+    double *d_bins, *h_bins;
+    size_t nbins;
+    int tables_ok;
+    int use_graph;
+    fgk_graph_t graph;
+    int ran;                     /* an enqueue completed setup */
+};
+
+int fftgal_batch_destroy(fftgal_batch_t *b) {
+    if (!b) return 0;
+    if (b->s) fgk_stream_sync(b->s);
+    fgk_graph_destroy(b->graph);
+    fgk_paint_batch_destroy(b->paint);
+    for (int q = 0; q < b->ngroups; ++q) {
+        batch_group *G = &b->grp[q];
+        fgk_fft_plan_destroy(G->plan);
+        fgk_free(G->g);
+        fgk_free(G->f);
+        fgk_free(G->d_phase);
+        fgk_free(G->d_winv);
+        fgk_free(G->djob);
+        free(G->hjob);
+        free(G->jobs);
+    }
+    for (int j = 0; j < b->njobs && b->job; ++j) {
+        free(b->job[j].gK);
+        free(b->job[j].gKmu);
+        free(b->job[j].gN);
+        free(b->job[j].gNmu);
+    }
+    free(b->grp);
+    free(b->job);
+    fgk_free(b->d_bins);
+    fgk_host_free(b->h_bins);
+    if (b->s) fgk_stream_destroy(b->s);
+    free(b);
+    return 0;
+}
+
+static int group_of(fftgal_batch_t *b, int Ng, int interlace) {
+    for (int q = 0; q < b->ngroups; ++q)
+        if (b->grp[q].Ng == Ng && b->grp[q].interlace == interlace) return q;
+    batch_group *G = &b->grp[b->ngroups];
+    memset(G, 0, sizeof *G);
+    G->Ng = Ng;
+    G->interlace = interlace;
+    return b->ngroups++;
+}
+
+static int group_setup(fftgal_batch_t *b, batch_group *G, const char *wisdom) {
+    const int Ng = G->Ng, NG = G->interlace ? 2 : 1;
+    const size_t cells = (size_t)Ng * Ng * Ng;
+    fftgal_tuning t;
+    fftgal_tune_lookup(wisdom, Ng, G->interlace, &t);
+    TRY(fgk_fft_plan_create(&G->plan, Ng, 4));
+    TRY(fgk_fft_set_tuning(G->plan, t.fft_cols, t.fft_rows));
+    TRY(fgk_malloc((void **)&G->g, (size_t)G->n * NG * cells * sizeof(int64_t)));
+    TRY(fgk_malloc((void **)&G->f, (size_t)G->n * NG * cells * sizeof(float)));
+    TRY(fgk_fill_zero(G->g, (size_t)G->n * NG * cells * sizeof(int64_t), b->s));
+    TRY(fgk_malloc((void **)&G->d_phase, (size_t)3 * Ng * 2 * sizeof(double)));
+    TRY(fgk_malloc((void **)&G->d_winv, (size_t)Ng * sizeof(double)));
+    TRY(fgk_malloc((void **)&G->djob, (size_t)G->n * sizeof(fgk_binjob)));
+    G->hjob = (fgk_binjob *)calloc((size_t)G->n, sizeof(fgk_binjob));
+    double *tab = (double *)malloc((size_t)3 * Ng * 2 * sizeof(double));
+    double *winv = (double *)malloc((size_t)Ng * sizeof(double));
+    int rc = (!G->hjob || !tab || !winv) ? -1 : 0;
+    if (rc) fgk_set_error("out of host memory");
+    if (!rc) {
+        /* dual grid offset 1/2 per axis (src/fft.c:260), phase on */
+        double off[3] = {remainder(0.5, Ng), remainder(0.5, Ng), remainder(0.5, Ng)};
+        fftgal_phase_table(Ng, off, 1, tab);
+        fftgal_winv_table(Ng, winv);
+        rc = fgk_h2d(G->d_phase, tab, (size_t)3 * Ng * 2 * sizeof(double), b->s);
+        if (!rc) rc = fgk_h2d(G->d_winv, winv, (size_t)Ng * sizeof(double), b->s);
+        if (!rc) rc = fgk_stream_sync(b->s);
+    }
+    free(tab);
+    free(winv);
+    if (rc) return rc;
+    G->spec.Ng = Ng;
+    G->spec.Nmu = b->Nmu;
+    G->spec.los[0] = b->los[0];
+    G->spec.los[1] = b->los[1];
+    G->spec.los[2] = b->los[2];
+    G->spec.do_pl = 1;
+    G->spec.geometry = 0;
+    for (int k = 0; k < G->n; ++k) {
+        batch_job *J = &b->job[G->jobs[k]];
+        fgk_binjob *h = &G->hjob[k];
+        h->A = G->f + (size_t)k * NG * cells;
+        h->B = G->interlace ? h->A + cells : NULL;
+        h->bins = b->d_bins + J->boff;
+        h->NK = J->NK;
+        h->dKinv = J->KF / J->d.dK;
+        if (J->NK > G->NK_max) G->NK_max = J->NK;
+    }
+    return 0;
+}
+
+int fftgal_batch_create(fftgal_batch_t **out, int njobs, const fftgal_job_desc *jobs,
+                        const double los[3], int Nmu, const char *wisdom) {
+    *out = NULL;
+    if (njobs < 1 || njobs > 65535 || !jobs || Nmu < 0) {
+        fgk_set_error("batch: need 1..65535 catalogs and Nmu >= 0");
+        return -2;
+    }
+    for (int j = 0; j < njobs; ++j) {
+        const fftgal_job_desc *d = &jobs[j];
+        if (!fgk_fft_supported(d->Ng)) {
+            fgk_set_error("batch: catalog %d, Ng=%d unsupported (even, prime factors <= 13)", j,
+                          d->Ng);
+            return -2;
+        }
+        if (!(d->L > 0) || !(d->dK > 0) || d->Np_max < 1) {
+            fgk_set_error("batch: catalog %d needs L > 0, dK > 0, Np_max >= 1", j);
+            return -2;
+        }
+    }
+    TRY(fgk_init());
+    fftgal_batch_t *b = (fftgal_batch_t *)calloc(1, sizeof(fftgal_batch_t));
+    if (!b) { fgk_set_error("out of host memory"); return -1; }
+    b->njobs = njobs;
+    b->Nmu = Nmu;
+    b->W = Nmu > 0 ? Nmu + 1 : 0;
+    b->use_graph = 1;
+    b->los[0] = los ? los[0] : 0.0;
+    b->los[1] = los ? los[1] : 0.0;
+    b->los[2] = los ? los[2] : 1.0;
+    {   /* hat() of src/gal.h:19-26 */
+        double a = sqrt(b->los[0] * b->los[0] + b->los[1] * b->los[1] + b->los[2] * b->los[2]);
+        if (a > 1e-7) { b->los[0] /= a; b->los[1] /= a; b->los[2] /= a; }
+    }
+    b->job = (batch_job *)calloc((size_t)njobs, sizeof(batch_job));
+    b->grp = (batch_group *)calloc((size_t)njobs, sizeof(batch_group));
+    int rc = (!b->job || !b->grp) ? -1 : 0;
+    if (rc) fgk_set_error("out of host memory");
+#define STEP(call) do { if (!rc) rc = (call); } while (0)
+    STEP(fgk_stream_create(&b->s));
+    /* catalogs, bin layout and grid groups */
+    for (int j = 0; j < njobs && !rc; ++j) {
+        batch_job *J = &b->job[j];
+        J->d = jobs[j];
+        J->d.interlace = jobs[j].interlace ? 1 : 0;
+        J->KF = 2 * M_PI / J->d.L;
+        J->NK = fftgal_nk(J->d.Ng, J->d.L, J->d.dK);
+        fgk_binspec bs = {0};
+        bs.NK = J->NK;
+        bs.Nmu = Nmu;
+        bs.do_pl = 1;
+        J->boff = b->nbins;
+        b->nbins += fgk_bins_doubles(&bs);
+        J->group = group_of(b, J->d.Ng, J->d.interlace);
+        J->slot = b->grp[J->group].n++;
+        const size_t W1 = b->W ? (size_t)b->W : 1;
+        J->gK = (double *)calloc((size_t)J->NK, sizeof(double));
+        J->gN = (long *)calloc((size_t)J->NK, sizeof(long));
+        J->gKmu = (double *)calloc((size_t)J->NK * W1, sizeof(double));
+        J->gNmu = (long *)calloc((size_t)J->NK * W1, sizeof(long));
+        if (!J->gK || !J->gN || !J->gKmu || !J->gNmu) {
+            fgk_set_error("out of host memory");
+            rc = -1;
+        }
+    }
+    for (int q = 0; q < b->ngroups && !rc; ++q) {
+        batch_group *G = &b->grp[q];
+        G->jobs = (int *)calloc((size_t)G->n, sizeof(int));
+        if (!G->jobs) { fgk_set_error("out of host memory"); rc = -1; }
+    }
+    for (int j = 0; j < njobs && !rc; ++j) b->grp[b->job[j].group].jobs[b->job[j].slot] = j;
+    STEP(fgk_malloc((void **)&b->d_bins, b->nbins * sizeof(double)));
+    STEP(fgk_host_alloc((void **)&b->h_bins, b->nbins * sizeof(double)));
+    for (int q = 0; q < b->ngroups && !rc; ++q) rc = group_setup(b, &b->grp[q], wisdom);
+    /* the paint batch writes each catalog's grids in its group */
+    if (!rc) {
+        fgk_paintjob_desc *pd = (fgk_paintjob_desc *)calloc((size_t)njobs, sizeof *pd);
+        if (!pd) { fgk_set_error("out of host memory"); rc = -1; }
+        for (int j = 0; j < njobs && !rc; ++j) {
+            const batch_job *J = &b->job[j];
+            const batch_group *G = &b->grp[J->group];
+            const size_t cells = (size_t)G->Ng * G->Ng * G->Ng;
+            fftgal_tuning t;
+            fftgal_tune_lookup(wisdom, G->Ng, G->interlace, &t);
+            pd[j].Ng = J->d.Ng;
+            pd[j].tile = J->d.paint_tile > 0 ? J->d.paint_tile : t.paint_tile;
+            pd[j].Hinv = J->d.Ng / J->d.L;
+            pd[j].Np_max = J->d.Np_max;
+            pd[j].interlace = J->d.interlace;
+            pd[j].gridA = G->g + (size_t)J->slot * (G->interlace ? 2 : 1) * cells;
+            pd[j].gridB = G->interlace ? pd[j].gridA + cells : NULL;
+        }
+        if (!rc) rc = fgk_paint_batch_create(&b->paint, njobs, pd);
+        free(pd);
+    }
+    /* K sums and mode counts per catalog */
+    for (int j = 0; j < njobs && !rc; ++j) {
+        batch_job *J = &b->job[j];
+        fgk_binspec g = b->grp[J->group].spec;
+        g.NK = J->NK;
+        g.dKinv = J->KF / J->d.dK;
+        rc = bin_geometry(&g, J->NK, b->W, b->s, J->gK, J->gN, J->gKmu, J->gNmu);
+    }
+    STEP(fgk_stream_sync(b->s));
+#undef STEP
+    if (rc) {
+        fftgal_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return 0;
+}
+
+int fftgal_batch_NK(const fftgal_batch_t *b, int job) {
+    if (job < 0 || job >= b->njobs) return -2;
+    return b->job[job].NK;
+}
+
+/* the device tables for this set of catalogs (waits for queued work when
+ * they change) */
+static int batch_tables(fftgal_batch_t *b, const float *const *x, const float *const *y,
+                        const float *const *z, const long *Np, const double *V) {
+    for (int j = 0; j < b->njobs; ++j) {
+        if (Np[j] < 1 || Np[j] > b->job[j].d.Np_max || !(V[j] > 0)) {
+            fgk_set_error("batch: catalog %d has Np=%ld (Np_max=%ld) or V=%g invalid", j, Np[j],
+                          b->job[j].d.Np_max, V[j]);
+            return -2;
+        }
+    }
+    TRY(fgk_paint_batch_set(b->paint, x, y, z, Np, b->s));
+    int same = b->tables_ok;
+    for (int j = 0; j < b->njobs; ++j)
+        if (b->job[j].Np != Np[j] || b->job[j].V != V[j]) same = 0;
+    if (same) return 0;
+    TRY(fgk_stream_sync(b->s));
+    for (int q = 0; q < b->ngroups; ++q) {
+        batch_group *G = &b->grp[q];
+        for (int k = 0; k < G->n; ++k) {
+            batch_job *J = &b->job[G->jobs[k]];
+            fgk_binjob *h = &G->hjob[k];
+            const long n = Np[G->jobs[k]];
+            const double v = V[G->jobs[k]];
+            catalog_norms(G->Ng, J->d.L, n, v, &h->norm, &h->Vinv, &h->Pshot);
+            h->npart = (double)n;
+            J->Np = n;
+            J->V = v;
+        }
+        TRY(fgk_h2d(G->djob, G->hjob, (size_t)G->n * sizeof(fgk_binjob), b->s));
+    }
+    TRY(fgk_stream_sync(b->s));
+    b->tables_ok = 1;
+    return 0;
+}
+
+/* the batch's stages in the order and under the names of
+ * fftgal_power_stage_name; ev (NSTAGES+1 events or NULL) brackets them */
+static int batch_work(fftgal_batch_t *b, fgk_event_t *ev) {
+    int st = 0;
+#define MARK() do { if (ev) TRY(fgk_event_record(ev[st], b->s)); ++st; } while (0)
+    MARK();
+    TRY(fgk_fill_zero(b->d_bins, b->nbins * sizeof(double), b->s));
+    for (int stage = 0; stage < NPAINT; ++stage) {
+        MARK();
+        TRY(fgk_paint_batch_stage(b->paint, stage, b->s));
+    }
+    const double in_scale = fgk_paint_batch_fixed_scale(b->paint);
+    MARK();
+    for (int q = 0; q < b->ngroups; ++q) {
+        batch_group *G = &b->grp[q];
+        TRY(fgk_fft_rows_f32_from_fixed(G->plan, G->f, G->g, in_scale, 1,
+                                        G->n * (G->interlace ? 2 : 1), b->s));
+    }
+    for (int axis = 1; axis >= 0; --axis) {
+        MARK();
+        for (int q = 0; q < b->ngroups; ++q) {
+            batch_group *G = &b->grp[q];
+            TRY(fgk_fft_cols_f32(G->plan, G->f, axis, -1, G->n * (G->interlace ? 2 : 1), b->s));
+        }
+    }
+    MARK();
+    for (int q = 0; q < b->ngroups; ++q) {
+        batch_group *G = &b->grp[q];
+        TRY(fgk_fused_bin_f32_batch(&G->spec, G->djob, G->n, G->NK_max, G->d_phase, G->d_winv,
+                                    b->s));
+    }
+    MARK();
+    TRY(fgk_d2h(b->h_bins, b->d_bins, b->nbins * sizeof(double), b->s));
+    MARK();
+#undef MARK
+    return 0;
+}
+
+int fftgal_batch_enqueue(fftgal_batch_t *b, const float *const *x, const float *const *y,
+                         const float *const *z, const long *Np, const double *V) {
+    TRY(batch_tables(b, x, y, z, Np, V));
+    if (!b->use_graph) return batch_work(b, NULL);
+    if (!b->graph) {
+        TRY(fgk_graph_begin(b->s));
+        int rc = batch_work(b, NULL);
+        fgk_graph_t g = NULL;
+        int rc2 = fgk_graph_end(b->s, &g);
+        if (rc) {
+            if (g) fgk_graph_destroy(g);
+            return rc;
+        }
+        TRY(rc2);
+        b->graph = g;
+    }
+    return fgk_graph_launch(b->graph, b->s);
+}
+
+int fftgal_batch_sync(fftgal_batch_t *b) { return fgk_stream_sync(b->s); }
+
+int fftgal_batch_profile(fftgal_batch_t *b, const float *const *x, const float *const *y,
+                         const float *const *z, const long *Np, const double *V,
+                         float *stage_ms) {
+    TRY(batch_tables(b, x, y, z, Np, V));
+    fgk_event_t ev[NSTAGES + 1] = {0};
+    int rc = 0;
+    for (int i = 0; i <= NSTAGES && !rc; ++i) rc = fgk_event_create(&ev[i]);
+    if (!rc) rc = batch_work(b, ev);
+    if (!rc) rc = fgk_stream_sync(b->s);
+    for (int i = 0; i < NSTAGES && !rc; ++i) rc = fgk_event_elapsed_ms(&stage_ms[i], ev[i], ev[i + 1]);
+    for (int i = 0; i <= NSTAGES; ++i) fgk_event_destroy(ev[i]);
+    return rc;
+}
+
+int fftgal_batch_results(fftgal_batch_t *b, int job, double *K, long *N, double *Pl, double *Kmu,
+                         long *Nmu_count, double *Pmu) {
+    if (job < 0 || job >= b->njobs) {
+        fgk_set_error("batch: no catalog %d", job);
+        return -2;
+    }
+    const batch_job *J = &b->job[job];
+    normalise_bins(J->NK, b->W, b->Nmu, J->KF, b->h_bins + J->boff, J->gK, J->gN, J->gKmu,
+                   J->gNmu, K, N, Pl, Kmu, Nmu_count, Pmu);
+    return 0;
+}
+
+int fftgal_batch_use_graph(fftgal_batch_t *b, int on) {
+    b->use_graph = on ? 1 : 0;
+    return 0;
+}
+
+int fftgal_batch_tile(const fftgal_batch_t *b, int job) {
+    if (job < 0 || job >= b->njobs) return -2;
+    return fgk_paint_batch_tile(b->paint, job);
+}
+
+void *fftgal_batch_stream(fftgal_batch_t *b) { return b->s; }

@@ -96,11 +96,10 @@ fft_t *fft_init(int Ng, double L, char *wisdom) {
     for (int i = 0; i < 3; ++i) self->offset[i] = 0;
     self->Ng_pad = 2 * (Ng / 2 + 1);
     self->Ng3_pad = (long)Ng * Ng * self->Ng_pad;
-    void *f = NULL;
-    DI_ASSERT(posix_memalign(&f, 64, (size_t)self->Ng3_pad * sizeof(double)) == 0,
-              "fft_init: out of memory");
-    memset(f, 0, (size_t)self->Ng3_pad * sizeof(double));
-    self->f = (double *)f;
+    /* calloc: a large grid comes from fresh zero pages, touched only when a
+     * host transfer or the caller writes them (68.8 GB at Ng = 2048) */
+    self->f = (double *)calloc((size_t)self->Ng3_pad, sizeof(double));
+    DI_ASSERT(self->f != NULL, "fft_init: out of memory");
 
     fft_dev *d = (fft_dev *)calloc(1, sizeof(fft_dev));
     DI_ASSERT(d != NULL, "fft_init: out of memory");

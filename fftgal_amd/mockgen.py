@@ -46,7 +46,10 @@ def gaussian_field(Ngs, L, seed, A=2e4, k0=0.1, n=-1.5):
 
 
 def lognormal(Np, L, seed, Ngs=256, A=2e4, k0=0.1, n=-1.5, f=0.76, rsd=True,
-              dtype=np.float32):
+              dtype=np.float32, log=None):
+    """log: optional callable, called with a short message after each step
+    (large catalogs take minutes)"""
+    say = log if log is not None else (lambda msg: None)
     rng = np.random.default_rng([seed, 1])
     W, ki, kk, k2 = gaussian_field(Ngs, L, seed, A, k0, n)
     delta = np.fft.irfftn(W, s=(Ngs, Ngs, Ngs), axes=(0, 1, 2))
@@ -55,6 +58,7 @@ def lognormal(Np, L, seed, Ngs=256, A=2e4, k0=0.1, n=-1.5, f=0.76, rsd=True,
     del delta
     rho /= rho.sum()
     counts = rng.multinomial(int(Np), rho.ravel())
+    say("mock: counts drawn")
     del rho
     cells = np.repeat(np.arange(Ngs ** 3, dtype=np.int64), counts)
     del counts
@@ -65,15 +69,18 @@ def lognormal(Np, L, seed, Ngs=256, A=2e4, k0=0.1, n=-1.5, f=0.76, rsd=True,
     x = (ci + rng.random(Np)) * H
     y = (cj + rng.random(Np)) * H
     z = (ck + rng.random(Np)) * H
+    say("mock: positions placed")
     if rsd and f != 0:
         psi = np.fft.irfftn(1j * kk[None, None, :] / k2 * W, s=(Ngs, Ngs, Ngs), axes=(0, 1, 2))
         z += f * psi.ravel()[cells]
         del psi
+        say("mock: RSD applied")
     del cells, ci, cj, ck, W
     x %= L
     y %= L
     z %= L
     perm = rng.permutation(Np)
+    say("mock: permutation drawn")
     out = tuple(np.ascontiguousarray(a[perm]).astype(dtype) for a in (x, y, z))
     # float32 rounding can land exactly on L: wrap again
     for a in out:

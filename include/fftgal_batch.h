@@ -97,6 +97,54 @@ int fftgal_power_profile(fftgal_power_t *p, const float *x, const float *y, cons
 /* synchronous copy from device memory (e.g. the grids above) to host */
 int fftgal_memcpy_d2h(void *dst, const void *src, unsigned long bytes);
 
+/* ---- batches of catalogs ----
+ * Many catalogs, each on its own grid -- mocks and subvolumes of different
+ * boxes and sizes, and the same catalog folded (grid box L/fold with the
+ * positions in box L, src/app/P.c:35-36) -- through the pipeline above
+ * with one launch per stage for all of them: one sort launch per stage,
+ * one paint launch per tile side, one launch per FFT pass and grid side,
+ * one fused bin launch pair per grid side, one copy of all bin sums.  The
+ * sequence is captured once in a hipGraph; the catalogs' pointers, Np and
+ * V are read from device tables, so later enqueues replay the same graph.
+ * Every catalog's grids stay allocated: about 24 Ng^3 bytes per
+ * interlaced catalog (int64 grids and FP32 spectra).  A catalog's grid may
+ * need at most 256 x 128 paint tiles (Ng <= 512 with tiles of 16); larger
+ * grids take a fftgal_power_t of their own. */
+typedef struct fftgal_batch fftgal_batch_t;
+
+typedef struct fftgal_job_desc {
+    int Ng;              /* grid cells per side: even, prime factors <= 13 */
+    double L;            /* grid box side = catalog box / fold */
+    int interlace;       /* 1: plain + dual grid */
+    double dK;           /* k-bin width */
+    long Np_max;         /* largest catalog this entry accepts */
+    int paint_tile;      /* paint tile side (0: tuned / default) */
+} fftgal_job_desc;
+
+/* los and Nmu are shared by all catalogs; wisdom as in fftgal_power_desc */
+int fftgal_batch_create(fftgal_batch_t **out, int njobs, const fftgal_job_desc *jobs,
+                        const double los[3], int Nmu, const char *wisdom);
+int fftgal_batch_destroy(fftgal_batch_t *b);
+int fftgal_batch_NK(const fftgal_batch_t *b, int job);
+/* catalog j: Np[j] float32 positions x[j], y[j], z[j] in device memory,
+ * volume V[j] (several entries may share one catalog's arrays).
+ * Asynchronous on the batch's stream; when the arrays, Np or V differ
+ * from the previous enqueue it first waits for the stream. */
+int fftgal_batch_enqueue(fftgal_batch_t *b, const float *const *x, const float *const *y,
+                         const float *const *z, const long *Np, const double *V);
+int fftgal_batch_sync(fftgal_batch_t *b);
+/* results of catalog `job` after sync, as fftgal_power_results */
+int fftgal_batch_results(fftgal_batch_t *b, int job, double *K, long *N, double *Pl,
+                         double *Kmu, long *Nmu_count, double *Pmu);
+/* eager run with events between the stages (names: fftgal_power_stage_name) */
+int fftgal_batch_profile(fftgal_batch_t *b, const float *const *x, const float *const *y,
+                         const float *const *z, const long *Np, const double *V,
+                         float *stage_ms);
+int fftgal_batch_use_graph(fftgal_batch_t *b, int on);
+/* paint tile side of catalog `job` */
+int fftgal_batch_tile(const fftgal_batch_t *b, int job);
+void *fftgal_batch_stream(fftgal_batch_t *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -271,3 +271,66 @@ def test_ng1024_fast_vs_plain_fp64(torch_cuda, lib):
         err = (np.abs(out["Pl"][sel, l] - Pl[sel, l]) / scale).max()
         print(f"l={2 * l}: max |dP|/(|P0|+Pshot) = {err:.2e}")
         assert err <= 1e-5
+
+
+@pytest.mark.parametrize("Ng", [256, 2048])
+def test_lattice_shot_noise_fast_and_plain(torch_cuda, lib, Ng):
+    """Closed form at up to the largest grid (Ng = 2048, 68.7 GB per FP64
+    grid): particles on a cubic sublattice of spacing s = 8 cells (offset
+    0.3 cell) make a density field of period s cells, whose Fourier modes
+    vanish except at multiples of Ng/s.  Below that wavenumber every mode's
+    power is -Pshot, and since the shot noise is subtracted per mode before
+    the Legendre weighting (src/power.c:18-24, 59-67), bin b holds
+    P_l = -Pshot (2l+1) <L_l(mu)> over the integer lattice vectors of the
+    shell b <= |n| < b+1 (numpy enumeration, first 30 shells; the rest:
+    fast path against the FP64 form).  Both forms must give that, and the
+    same mode counts."""
+    import fftgal_amd as fa
+    torch = torch_cuda
+    L, s = 4000.0, 8
+    H = L / Ng
+    n = Ng // s
+    g1 = (np.arange(n) * s + 0.3) * H
+    x, y, z = (a.ravel().astype(np.float32) for a in np.meshgrid(g1, g1, g1, indexing="ij"))
+    Np = x.size
+    V = L ** 3
+    dK = 2 * np.pi / L
+    Pshot = V / Np
+    kmax = n - 2
+    tx, ty, tz = _dev(torch, x, y, z)
+    p = fa.PowerPlan(Ng, L, interlace=True, dK=dK, los=(0, 0, 1), Nmu=4, Np_max=Np)
+    out = p.run(tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), Np, V)
+    p.close()
+    torch.cuda.empty_cache()
+    dd = [t.double() for t in (tx, ty, tz)]
+    del tx, ty, tz
+    torch.cuda.synchronize()
+    g = fa.Grid(Ng, L, resident=True)
+    g.lib.fft_p2k_device(g.ptr, dd[0].data_ptr(), dd[1].data_ptr(), dd[2].data_ptr(), Np, V)
+    g.lib.P_params(g.ptr, Np, V, 0)
+    K, N, Pl = g.Pl_bins((0, 0, 1), dK)
+    g.free()
+    del dd
+    torch.cuda.empty_cache()
+    assert np.array_equal(out["N"], N)
+    # closed form of the first shells
+    nb = 30
+    r = np.arange(-nb, nb + 1, dtype=np.float64)
+    fi, fj, fk = np.meshgrid(r, r, r, indexing="ij")
+    kr = np.sqrt(fi * fi + fj * fj + fk * fk).ravel()
+    mu = np.divide(fk.ravel(), kr, out=np.zeros_like(kr), where=kr > 0)
+    ib = np.floor(kr).astype(np.int64)
+    use = (kr > 0) & (ib < nb)
+    cnt = np.bincount(ib[use], minlength=nb)
+    m2 = mu[use] ** 2
+    leg = (np.ones_like(m2), 1.5 * m2 - 0.5, (4.375 * m2 - 3.75) * m2 + 0.375)
+    exact = np.stack([-Pshot * (4 * l + 1) * np.bincount(ib[use], weights=leg[l],
+                                                         minlength=nb) / np.maximum(cnt, 1)
+                      for l in range(3)], axis=1)
+    assert np.array_equal(N[1:nb], cnt[1:nb])
+    # the fast path's FP32 FFT leaks O(eps) amplitude into the empty modes,
+    # O(eps^2) in power; the FP64 form O(eps_64^2)
+    for Pv, tol in ((out["Pl"], 1e-8), (Pl, 1e-12)):
+        assert (np.abs(Pv[1:nb, :3] - exact[1:nb]) / Pshot).max() <= tol
+        assert (np.abs(Pv[1:kmax, 0] + Pshot) / Pshot).max() <= tol
+    assert (np.abs(out["Pl"][1:kmax, :3] - Pl[1:kmax, :3]) / Pshot).max() <= 1e-8
