@@ -97,7 +97,11 @@ API = {
     "fftgal_power_spectra": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p),
                                        C.POINTER(C.c_void_p)]),
     "fftgal_power_tuning": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
+    "fftgal_power_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_long, C.c_double, C.c_int, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fftgal_power_use_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "fftgal_power_fft_generic": (C.c_int, [C.c_void_p, C.c_int]),
     "fftgal_power_stream": (C.c_void_p, [C.c_void_p]),
@@ -381,9 +385,11 @@ class PowerPlan:
         return a.value, b.value
 
     def tuning(self):
-        t, c, r = C.c_int(), C.c_int(), C.c_int()
-        self.lib.fftgal_power_tuning(self.h, C.byref(t), C.byref(c), C.byref(r))
-        return {"paint_tile": t.value, "fft_cols": c.value, "fft_rows": r.value}
+        t, c, r, g, k = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self.lib.fftgal_power_tuning(self.h, C.byref(t), C.byref(c), C.byref(r), C.byref(g),
+                                     C.byref(k))
+        return {"paint_tile": t.value, "fft_cols": c.value, "fft_rows": r.value,
+                "fft_generic": g.value, "bin_bk": k.value}
 
     def use_graph(self, on):
         self.lib.fftgal_power_use_graph(self.h, 1 if on else 0)
@@ -422,6 +428,30 @@ class PowerPlan:
             self.close()
         except Exception:
             pass
+
+
+def autotune(Ng, L, x, y, z, Np, V, wisdom, interlace=True, dK=None, los=(0.0, 0.0, 1.0),
+             Nmu=0, reps=2):
+    """fftgal_power_autotune: measure the paint tile, FFT kernels and bin
+    block on this catalog (device float32 positions by address), store the
+    fastest in the wisdom file and return it."""
+    lib = load()
+    d = PowerDesc()
+    d.Ng = int(Ng)
+    d.L = float(L)
+    d.interlace = 1 if interlace else 0
+    d.dK = float(dK if dK is not None else 2 * np.pi / L)
+    d.los = (C.c_double * 3)(*[float(v) for v in los])
+    d.Nmu = int(Nmu)
+    d.Np_max = int(Np)
+    d.paint_tile = 0
+    d.wisdom = wisdom.encode()
+    t, g, k = C.c_int(), C.c_int(), C.c_int()
+    rc = lib.fftgal_power_autotune(C.byref(d), x, y, z, int(Np), float(V), int(reps),
+                                   C.byref(t), C.byref(g), C.byref(k))
+    if rc:
+        raise RuntimeError(f"fftgal_power_autotune failed ({rc}): {last_error()}")
+    return {"paint_tile": t.value, "fft_generic": g.value, "bin_bk": k.value}
 
 
 class JobDesc(C.Structure):

@@ -158,6 +158,8 @@ typedef struct fgk_binspec {
     double los[3];      /* already normalised */
     int do_pl;          /* accumulate the Legendre sums */
     int geometry;       /* also accumulate K sums and counts */
+    int bk;             /* fused kernel: k extent of a workgroup's block of
+                           1024 modes, 16, 32, 64 or 128 (0: 64) */
 } fgk_binspec;
 size_t fgk_bins_doubles(const fgk_binspec *b);
 size_t fgk_bins_counts(const fgk_binspec *b);

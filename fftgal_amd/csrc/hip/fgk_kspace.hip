@@ -120,9 +120,13 @@ extern "C" int fgk_P_f64(double *f, int Ng, double Vinv, double Pshot, fgk_strea
 // ------------------------------------------------------------- binning
 
 #define BIN_THREADS 256
-#define BI 4
-#define BJ 4
-#define BK 64
+// block of modes per workgroup of the bin kernel: BI x BJ x BK (i, j, k),
+// 1024 modes; BK is the tunable (fgk_binspec.bk), 64 by default
+template <int BKT> struct BinBlock {
+    static constexpr int BK = BKT;
+    static constexpr int BI = BKT >= 64 ? 4 : 8;
+    static constexpr int BJ = 1024 / BKT / BI;
+};
 #define BIN_LDS_BYTES (12 * 1024)
 
 extern "C" size_t fgk_bins_doubles(const fgk_binspec *b) {
@@ -394,7 +398,7 @@ __device__ __forceinline__ BinOut make_out(const BinArgs &a, double *sh, int blo
         npart = J.npart;                   \
     } while (0)
 
-template <int MODE>
+template <int MODE, int BKT>
 __global__ void __launch_bounds__(BIN_THREADS)
 k_bin(const double *__restrict__ fP, const float *__restrict__ A, const float *__restrict__ B,
       BinArgs a, const double *__restrict__ ph, const double *__restrict__ winv, double norm,
@@ -403,6 +407,7 @@ k_bin(const double *__restrict__ fP, const float *__restrict__ A, const float *_
 #pragma clang fp contract(off)
     extern __shared__ double sh[];
     if (MODE == 1 && jobs) BIN_JOB(jobs);
+    constexpr int BI = BinBlock<BKT>::BI, BJ = BinBlock<BKT>::BJ, BK = BinBlock<BKT>::BK;
     const int Ng = a.Ng;
     const int Mh = Ng / 2;
     const int Nk = MODE == 1 ? Mh : Mh + 1;      // k columns walked
@@ -519,7 +524,12 @@ k_bin_plane(const float *__restrict__ A, const float *__restrict__ B, BinArgs a,
     window_flush(a, o);
 }
 
+static int bin_bk(const fgk_binspec *b) {
+    return (b->bk == 16 || b->bk == 32 || b->bk == 128) ? b->bk : 64;
+}
+
 static BinArgs make_args(const fgk_binspec *b) {
+    const int BK = bin_bk(b);
     BinArgs a;
     a.Ng = b->Ng;
     a.NK = b->NK;
@@ -542,10 +552,33 @@ static BinArgs make_args(const fgk_binspec *b) {
 }
 
 // Nk: k columns walked (Ng/2 + 1 padded layout, Ng/2 packed)
-static unsigned bin_blocks(int Ng, int Nk) {
+template <int BKT> static unsigned bin_blocks(int Ng, int Nk) {
+    constexpr int BI = BinBlock<BKT>::BI, BJ = BinBlock<BKT>::BJ, BK = BinBlock<BKT>::BK;
     long n = (long)((Ng + BI - 1) / BI) * ((Ng + BJ - 1) / BJ) * ((Nk + BK - 1) / BK);
     return (unsigned)n;
 }
+
+// launch k_bin<MODE, bk> with the block count of that shape
+#define BIN_LAUNCH(MODE, bk, Ng, Nk, grid_y, lds, s, ...)                                    \
+    do {                                                                                      \
+        switch (bk) {                                                                         \
+        case 16:                                                                              \
+            hipLaunchKernelGGL((k_bin<MODE, 16>), dim3(bin_blocks<16>(Ng, Nk), grid_y),        \
+                               dim3(BIN_THREADS), lds, s, __VA_ARGS__);                       \
+            break;                                                                            \
+        case 32:                                                                              \
+            hipLaunchKernelGGL((k_bin<MODE, 32>), dim3(bin_blocks<32>(Ng, Nk), grid_y),        \
+                               dim3(BIN_THREADS), lds, s, __VA_ARGS__);                       \
+            break;                                                                            \
+        case 128:                                                                             \
+            hipLaunchKernelGGL((k_bin<MODE, 128>), dim3(bin_blocks<128>(Ng, Nk), grid_y),      \
+                               dim3(BIN_THREADS), lds, s, __VA_ARGS__);                       \
+            break;                                                                            \
+        default:                                                                              \
+            hipLaunchKernelGGL((k_bin<MODE, 64>), dim3(bin_blocks<64>(Ng, Nk), grid_y),        \
+                               dim3(BIN_THREADS), lds, s, __VA_ARGS__);                       \
+        }                                                                                     \
+    } while (0)
 
 static size_t bin_lds(const BinArgs &a) {
     return (size_t)a.wmax * a.nvals * 8 + (size_t)a.wmax * a.ncnt * 4 + 16;
@@ -553,8 +586,10 @@ static size_t bin_lds(const BinArgs &a) {
 
 extern "C" int fgk_bin_pgrid_f64(const double *f, const fgk_binspec *b, double *bins_dev,
                                  int64_t *counts_dev, fgk_stream_t s) {
-    BinArgs a = make_args(b);
-    hipLaunchKernelGGL(k_bin<0>, dim3(bin_blocks(b->Ng, b->Ng / 2 + 1)), dim3(BIN_THREADS),
+    fgk_binspec b64 = *b;
+    b64.bk = 64;             // MODE 0 and 2 run the default block
+    BinArgs a = make_args(&b64);
+    hipLaunchKernelGGL((k_bin<0, 64>), dim3(bin_blocks<64>(b->Ng, b->Ng / 2 + 1)), dim3(BIN_THREADS),
                        bin_lds(a), (hipStream_t)s, f, (const float *)nullptr,
                        (const float *)nullptr, a, (const double *)nullptr,
                        (const double *)nullptr, 0.0, 0.0, 0.0, 0.0, bins_dev,
@@ -569,8 +604,10 @@ extern "C" int fgk_bin_geometry(const fgk_binspec *b, double *bins_dev, int64_t 
         fgk_set_error("fgk_bin_geometry: binspec without geometry");
         return -2;
     }
-    BinArgs a = make_args(b);
-    hipLaunchKernelGGL(k_bin<2>, dim3(bin_blocks(b->Ng, b->Ng / 2 + 1)), dim3(BIN_THREADS),
+    fgk_binspec b64 = *b;
+    b64.bk = 64;
+    BinArgs a = make_args(&b64);
+    hipLaunchKernelGGL((k_bin<2, 64>), dim3(bin_blocks<64>(b->Ng, b->Ng / 2 + 1)), dim3(BIN_THREADS),
                        bin_lds(a), (hipStream_t)s, (const double *)nullptr,
                        (const float *)nullptr, (const float *)nullptr, a,
                        (const double *)nullptr, (const double *)nullptr, 0.0, 0.0, 0.0, 0.0,
@@ -589,10 +626,9 @@ extern "C" int fgk_fused_bin_f32(const float *A, const float *B, const fgk_binsp
     }
     BinArgs a = make_args(b);
     const int Ng = b->Ng;
-    hipLaunchKernelGGL(k_bin<1>, dim3(bin_blocks(Ng, Ng / 2)), dim3(BIN_THREADS), bin_lds(a),
-                       (hipStream_t)s, (const double *)nullptr, A, B, a, phase_dev, winv_dev,
-                       norm, Vinv, Pshot, npart, bins_dev, (unsigned long long *)nullptr,
-                       (const fgk_binjob *)nullptr);
+    BIN_LAUNCH(1, bin_bk(b), Ng, Ng / 2, 1u, bin_lds(a), (hipStream_t)s, (const double *)nullptr,
+               A, B, a, phase_dev, winv_dev, norm, Vinv, Pshot, npart, bins_dev,
+               (unsigned long long *)nullptr, (const fgk_binjob *)nullptr);
     FGK_LAUNCHED();
     const unsigned nplane = (unsigned)(((Ng + PI - 1) / PI) * ((Ng + PJ - 1) / PJ));
     hipLaunchKernelGGL(k_bin_plane, dim3(nplane), dim3(PI * PJ), bin_lds(a), (hipStream_t)s, A, B,
@@ -617,10 +653,10 @@ extern "C" int fgk_fused_bin_f32_batch(const fgk_binspec *b, const fgk_binjob *j
     bb.NK = NK_max;          // the LDS window size does not depend on NK
     BinArgs a = make_args(&bb);
     const int Ng = b->Ng;
-    hipLaunchKernelGGL(k_bin<1>, dim3(bin_blocks(Ng, Ng / 2), (unsigned)njobs), dim3(BIN_THREADS),
-                       bin_lds(a), (hipStream_t)s, (const double *)nullptr, (const float *)nullptr,
-                       (const float *)nullptr, a, phase_dev, winv_dev, 0.0, 0.0, 0.0, 0.0,
-                       (double *)nullptr, (unsigned long long *)nullptr, jobs_dev);
+    BIN_LAUNCH(1, bin_bk(b), Ng, Ng / 2, (unsigned)njobs, bin_lds(a), (hipStream_t)s,
+               (const double *)nullptr, (const float *)nullptr, (const float *)nullptr, a,
+               phase_dev, winv_dev, 0.0, 0.0, 0.0, 0.0, (double *)nullptr,
+               (unsigned long long *)nullptr, jobs_dev);
     FGK_LAUNCHED();
     const unsigned nplane = (unsigned)(((Ng + PI - 1) / PI) * ((Ng + PJ - 1) / PJ));
     hipLaunchKernelGGL(k_bin_plane, dim3(nplane, (unsigned)njobs), dim3(PI * PJ), bin_lds(a),

@@ -212,6 +212,7 @@ int fftgal_power_create(fftgal_power_t **out, const fftgal_power_desc *desc) {
     STEP(fgk_stream_create(&p->s));
     STEP(fgk_fft_plan_create(&p->plan, Ng, 4));
     if (!rc) STEP(fgk_fft_set_tuning(p->plan, p->tune.fft_cols, p->tune.fft_rows));
+    if (!rc && p->tune.fft_generic) STEP(fgk_fft_force_generic(p->plan, 1));
     if (!rc && fgk_paint_nstages() != NPAINT) {
         fgk_set_error("internal: paint stage count mismatch");
         rc = -3;
@@ -238,6 +239,7 @@ int fftgal_power_create(fftgal_power_t **out, const fftgal_power_desc *desc) {
         p->spec.los[2] = p->los[2];
         p->spec.do_pl = 1;
         p->spec.geometry = 0;
+        p->spec.bk = p->tune.bin_bk;
         p->nbins = fgk_bins_doubles(&p->spec);
     }
     STEP(fgk_malloc((void **)&p->d_bins, p->nbins * sizeof(double)));
@@ -450,10 +452,149 @@ int fftgal_power_spectra(fftgal_power_t *p, void **A, void **B) {
     return 0;
 }
 
-int fftgal_power_tuning(const fftgal_power_t *p, int *paint_tile, int *fft_cols, int *fft_rows) {
+int fftgal_power_tuning(const fftgal_power_t *p, int *paint_tile, int *fft_cols, int *fft_rows,
+                        int *fft_generic, int *bin_bk) {
     if (paint_tile) *paint_tile = fgk_paint_plan_tile(p->paint);
     if (fft_cols) *fft_cols = p->tune.fft_cols;
     if (fft_rows) *fft_rows = p->tune.fft_rows;
+    if (fft_generic) *fft_generic = p->tune.fft_generic || !fgk_fft_f32_specialised(p->d.Ng);
+    if (bin_bk) *bin_bk = p->spec.bk ? p->spec.bk : 64;
+    return 0;
+}
+
+/* ---- autotuner ----
+ * Times the candidates of each knob on the caller's catalog with eager
+ * profiled runs (HIP events per stage), one knob at a time in pipeline
+ * order: paint tile side (paint stages), FFT kernels (FFT stages), bin
+ * block (fused bin stage); the fastest of `reps` runs of each candidate
+ * counts.  The winner is stored in the wisdom file (tune.h) for
+ * (Ng, interlace), where fftgal_power_create and fftgal_batch_create find
+ * it. */
+
+static float stage_sum(const float *ms, int a, int b) {
+    float t = 0.0f;
+    for (int i = a; i < b; ++i) t += ms[i];
+    return t;
+}
+
+/* fastest of reps profiled runs: paint, fft and bin stage times */
+static int tune_measure(fftgal_power_t *p, const float *x, const float *y, const float *z,
+                        long Np, double V, int reps, float best[3]) {
+    float ms[NSTAGES];
+    best[0] = best[1] = best[2] = 1e30f;
+    for (int r = 0; r <= reps; ++r) {   /* the first run warms up */
+        TRY(fftgal_power_profile(p, x, y, z, Np, V, ms));
+        if (r == 0) continue;
+        const float t[3] = {stage_sum(ms, 1, 1 + NPAINT), stage_sum(ms, 1 + NPAINT, 4 + NPAINT),
+                            ms[4 + NPAINT]};
+        for (int k = 0; k < 3; ++k)
+            if (t[k] < best[k]) best[k] = t[k];
+    }
+    return 0;
+}
+
+/* switch a plan's FFT kernels: family (generic or specialised) and the
+ * generic kernels' columns / rows per workgroup */
+static int tune_fft(fftgal_power_t *p, int generic, int cols, int rows) {
+    TRY(fgk_stream_sync(p->s));
+    if (p->graph) {
+        fgk_graph_destroy(p->graph);
+        p->graph = NULL;
+    }
+    TRY(fgk_fft_set_tuning(p->plan, cols, rows));
+    TRY(fgk_fft_force_generic(p->plan, generic));
+    p->tune.fft_generic = generic;
+    p->tune.fft_cols = cols;
+    p->tune.fft_rows = rows;
+    return 0;
+}
+
+int fftgal_power_autotune(const fftgal_power_desc *desc, const float *x, const float *y,
+                          const float *z, long Np, double V, int reps, int *paint_tile,
+                          int *fft_generic, int *bin_bk) {
+    if (!desc) { fgk_set_error("autotune: NULL descriptor"); return -2; }
+    if (reps < 1) reps = 1;
+    const int Ng = desc->Ng;
+    fftgal_tuning best;
+    fftgal_tune_lookup(NULL, Ng, desc->interlace, &best);
+    /* 1. paint tile side */
+    static const int kTiles[] = {16, 12, 8, 4};
+    float tbest = 1e30f;
+    for (int i = 0; i < 4; ++i) {
+        const int T = kTiles[i];
+        if (T > Ng) continue;
+        fftgal_power_desc d = *desc;
+        d.wisdom = NULL;
+        d.paint_tile = T;
+        fftgal_power_t *p = NULL;
+        TRY(fftgal_power_create(&p, &d));
+        float m[3];
+        int rc = tune_measure(p, x, y, z, Np, V, reps, m);
+        fftgal_power_destroy(p);
+        if (rc) return rc;
+        if (m[0] < tbest) {
+            tbest = m[0];
+            best.paint_tile = T;
+        }
+    }
+    /* 2. FFT kernels and 3. bin block, on a plan with the chosen tile */
+    fftgal_power_desc d = *desc;
+    d.wisdom = NULL;
+    d.paint_tile = best.paint_tile;
+    fftgal_power_t *p = NULL;
+    TRY(fftgal_power_create(&p, &d));
+    int rc = 0;
+    float fbest = 1e30f;
+    /* candidates: specialised kernels (where they exist), generic kernels with
+     * the default launch shape, with half the columns, with half the rows */
+    const int spec = fgk_fft_f32_specialised(Ng);
+    int cand[4][3] = {{0, 0, 0}, {1, 0, 0}, {1, -1, 0}, {1, 0, -1}};
+    for (int c = spec ? 0 : 1; c < 4 && !rc; ++c) {
+        int cols = cand[c][1], rows = cand[c][2];
+        if (cols < 0) {   /* half the default columns per workgroup */
+            int C = 16;
+            while (C > 1 && (size_t)2 * Ng * C * 8 > 64 * 1024) C >>= 1;
+            if (C < 2) continue;
+            cols = C / 2;
+        }
+        if (rows < 0) {   /* half the default rows per workgroup */
+            int P = 1;
+            while (P * (Ng / 2) < 1024 && (size_t)2 * P * 2 * (Ng / 2) * 8 <= 64 * 1024) P *= 2;
+            if (P < 2) continue;
+            rows = P / 2;
+        }
+        rc = tune_fft(p, cand[c][0], cols, rows);
+        float m[3];
+        if (!rc) rc = tune_measure(p, x, y, z, Np, V, reps, m);
+        if (!rc && m[1] < fbest) {
+            fbest = m[1];
+            best.fft_generic = cand[c][0];
+            best.fft_cols = cols;
+            best.fft_rows = rows;
+        }
+    }
+    if (!rc) rc = tune_fft(p, best.fft_generic, best.fft_cols, best.fft_rows);
+    float bbest = 1e30f;
+    static const int kBk[] = {64, 32, 16, 128};
+    for (int i = 0; i < 4 && !rc; ++i) {
+        if (kBk[i] != 64 && kBk[i] > Ng / 2) continue;
+        p->spec.bk = kBk[i];
+        float m[3];
+        rc = tune_measure(p, x, y, z, Np, V, reps, m);
+        if (!rc && m[2] < bbest) {
+            bbest = m[2];
+            best.bin_bk = kBk[i];
+        }
+    }
+    fftgal_power_destroy(p);
+    if (rc) return rc;
+    if (fftgal_tune_store(desc->wisdom, Ng, desc->interlace, &best)) {
+        fgk_set_error("autotune: cannot write the wisdom file %s", desc->wisdom);
+        return -1;
+    }
+    if (paint_tile) *paint_tile = best.paint_tile;
+    if (fft_generic) *fft_generic = best.fft_generic;
+    if (bin_bk) *bin_bk = best.bin_bk;
     return 0;
 }
 
@@ -582,6 +723,7 @@ static int group_setup(fftgal_batch_t *b, batch_group *G, const char *wisdom) {
     fftgal_tune_lookup(wisdom, Ng, G->interlace, &t);
     TRY(fgk_fft_plan_create(&G->plan, Ng, 4));
     TRY(fgk_fft_set_tuning(G->plan, t.fft_cols, t.fft_rows));
+    if (t.fft_generic) TRY(fgk_fft_force_generic(G->plan, 1));
     TRY(fgk_malloc((void **)&G->g, (size_t)G->n * NG * cells * sizeof(int64_t)));
     TRY(fgk_malloc((void **)&G->f, (size_t)G->n * NG * cells * sizeof(float)));
     TRY(fgk_fill_zero(G->g, (size_t)G->n * NG * cells * sizeof(int64_t), b->s));
@@ -612,6 +754,7 @@ static int group_setup(fftgal_batch_t *b, batch_group *G, const char *wisdom) {
     G->spec.los[2] = b->los[2];
     G->spec.do_pl = 1;
     G->spec.geometry = 0;
+    G->spec.bk = t.bin_bk;
     for (int k = 0; k < G->n; ++k) {
         batch_job *J = &b->job[G->jobs[k]];
         fgk_binjob *h = &G->hjob[k];

@@ -10,6 +10,8 @@ static void defaults(int Ng, fftgal_tuning *t) {
     t->paint_tile = Ng >= 16 ? 16 : Ng;
     t->fft_cols = 0;
     t->fft_rows = 0;
+    t->fft_generic = 0;
+    t->bin_bk = 0;
 }
 
 static int usable(const char *path) {
@@ -24,13 +26,15 @@ int fftgal_tune_lookup(const char *path, int Ng, int interlace, fftgal_tuning *t
     char line[256];
     int found = 0;
     while (fgets(line, sizeof line, fp)) {
-        int n, il, a, b, c;
+        int n, il, a, b, c, g = 0, k = 0;
         if (line[0] == '#') continue;
-        if (sscanf(line, "%d %d %d %d %d", &n, &il, &a, &b, &c) == 5 && n == Ng &&
+        if (sscanf(line, "%d %d %d %d %d %d %d", &n, &il, &a, &b, &c, &g, &k) >= 5 && n == Ng &&
             il == (interlace ? 1 : 0)) {
             t->paint_tile = a;
             t->fft_cols = b;
             t->fft_rows = c;
+            t->fft_generic = g;
+            t->bin_bk = k;
             found = 1;
         }
     }
@@ -60,10 +64,11 @@ int fftgal_tune_store(const char *path, int Ng, int interlace, const fftgal_tuni
     fp = fopen(path, "w");
     int rc = fp ? 0 : -1;
     if (fp) {
-        if (nkeep == 0) fprintf(fp, "# Ng interlace paint_tile fft_cols fft_rows\n");
+        if (nkeep == 0)
+            fprintf(fp, "# Ng interlace paint_tile fft_cols fft_rows fft_generic bin_bk\n");
         for (size_t i = 0; i < nkeep; ++i) fputs(keep[i], fp);
-        fprintf(fp, "%d %d %d %d %d\n", Ng, interlace ? 1 : 0, t->paint_tile, t->fft_cols,
-                t->fft_rows);
+        fprintf(fp, "%d %d %d %d %d %d %d\n", Ng, interlace ? 1 : 0, t->paint_tile, t->fft_cols,
+                t->fft_rows, t->fft_generic, t->bin_bk);
         fclose(fp);
     }
     for (size_t i = 0; i < nkeep; ++i) free(keep[i]);

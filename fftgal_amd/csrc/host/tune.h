@@ -4,8 +4,11 @@
  * (src/fft.c:26-45, named ${Ng}.wsdm in src/app/job_P.sh:13): fft_init's and
  * fftgal_power_desc's `wisdom` string names it, "FFTW_ESTIMATE" or NULL means
  * built-in defaults.  Text format, one line per configuration:
- *     Ng interlace paint_tile fft_cols fft_rows
- * (0 in the fft fields = automatic choice inside the FFT plan). */
+ *     Ng interlace paint_tile fft_cols fft_rows fft_generic bin_bk
+ * (0 in fft_cols / fft_rows = automatic choice inside the FFT plan; the
+ * last two fields may be absent: 0 = specialised FFT kernels where the
+ * grid side has them, 0 = default bin block).  fftgal_power_autotune
+ * (batch.c) measures the candidates and writes the line. */
 #ifndef FFTGAL_TUNE_H
 #define FFTGAL_TUNE_H
 
@@ -13,6 +16,8 @@ typedef struct fftgal_tuning {
     int paint_tile;
     int fft_cols;
     int fft_rows;
+    int fft_generic;     /* 1: generic LDS Stockham kernels even where specialised exist */
+    int bin_bk;          /* fused bin block k extent (0: default) */
 } fftgal_tuning;
 
 /* defaults, overridden by a matching line of the cache file if present;

@@ -76,8 +76,23 @@ int fftgal_power_fixed_bits(const fftgal_power_t *p);
  * delta(i,j,0) = (Z[i,j] + conj Z[-i,-j]) / 2 and
  * delta(i,j,Ng/2) = (Z[i,j] - conj Z[-i,-j]) / 2i */
 int fftgal_power_spectra(fftgal_power_t *p, void **A, void **B);
-/* tuning in use */
-int fftgal_power_tuning(const fftgal_power_t *p, int *paint_tile, int *fft_cols, int *fft_rows);
+/* tuning in use: paint tile side, generic FFT kernels' columns / rows per
+ * workgroup (0: automatic), 1 if the generic FFT kernels run, k extent of
+ * the fused bin kernel's block of modes */
+int fftgal_power_tuning(const fftgal_power_t *p, int *paint_tile, int *fft_cols, int *fft_rows,
+                        int *fft_generic, int *bin_bk);
+/* Autotuner: times, on the caller's catalog (device float32 positions, as
+ * enqueue takes them), the paint tile sides 16, 12, 8, 4, the FFT kernels
+ * (compile-time specialised where the grid side has them, generic LDS
+ * Stockham with its default, halved-column and halved-row launch shapes)
+ * and the fused bin block (k extent 64, 32, 16, 128), one knob at a time in
+ * pipeline order, the fastest of `reps` eager runs per candidate.  Stores
+ * the winner for (desc->Ng, desc->interlace) in the wisdom file
+ * desc->wisdom, the role FFTW's wisdom has for fft_init (src/fft.c:26-45),
+ * and returns it.  Plans and batches created with that wisdom use it. */
+int fftgal_power_autotune(const fftgal_power_desc *desc, const float *x, const float *y,
+                          const float *z, long Np, double V, int reps, int *paint_tile,
+                          int *fft_generic, int *bin_bk);
 /* use the hipGraph (1, default) or plain stream launches (0) */
 int fftgal_power_use_graph(fftgal_power_t *p, int on);
 /* FFT kernels: on = 1 forces the generic LDS Stockham kernels, on = 0
