@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 21)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (default: tune each grid side on the workload "
+                         "first, timed separately)")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed graph loop (for rocprofv3 runs)")
     a = ap.parse_args()
@@ -225,8 +228,21 @@ def main_batch(args):
             jobs.append({"Ng": Ng, "L": Lg, "Np_max": Np, "dK": 2 * np.pi / Lg})
             cats.append((d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), Np, L ** 3))
     Ntot = sum(c[3] for c in cats)
+    wisdom, tuned, t_tune = None, {}, 0.0
+    if not args.no_tune:
+        import tempfile
+        wisdom = os.path.join(tempfile.mkdtemp(prefix="fftgal_tune_"), "batch.wsdm")
+        t0 = time.perf_counter()
+        for n in sorted({sp[2] for sp in specs}):
+            q = next(i for i, sp in enumerate(specs) if sp[2] == n)
+            s_, L_, _, Np_ = specs[q]
+            d = dev[q]
+            tuned[n] = fa.autotune(n, L_, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), Np_,
+                                   L_ ** 3, wisdom, Nmu=nmu, reps=2)
+        t_tune = time.perf_counter() - t0
+        print(f"# autotune: {tuned} in {t_tune:.1f}s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    b = fa.BatchPlan(jobs, los=(0, 0, 1), Nmu=nmu)
+    b = fa.BatchPlan(jobs, los=(0, 0, 1), Nmu=nmu, wisdom=wisdom)
     t_plan = time.perf_counter() - t0
     print(f"# batch plan: {len(jobs)} catalogs in {t_plan:.1f}s", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
@@ -320,8 +336,9 @@ def main_batch(args):
         "error_vs_plain_fp64": {"max_rel_dP_below_kNyq": worst,
                                 "relative_to": "max(|P_l|, |P_0| + Pshot) of the bin",
                                 "counts_equal": counts_equal, "worst": where},
-        "timing": {"mock_s": round(t_mock, 1), "plan_s": round(t_plan, 2),
-                   "wall_s": round(wall, 3)},
+        "timing": {"mock_s": round(t_mock, 1), "autotune_s": round(t_tune, 2),
+                   "plan_s": round(t_plan, 2), "wall_s": round(wall, 3)},
+        "tuning": {str(k): v for k, v in tuned.items()},
     }
     print(json.dumps(line))
 
@@ -357,7 +374,16 @@ def main():
     ptr = [d.data_ptr() for d in dev]
 
     t0 = time.perf_counter()
-    plan = fa.PowerPlan(Ng, Lg, interlace=True, dK=dK, los=(0, 0, 1), Nmu=nmu, Np_max=Np)
+    wisdom, tuned, t_tune = None, None, 0.0
+    if not args.no_tune:
+        import tempfile
+        wisdom = os.path.join(tempfile.mkdtemp(prefix="fftgal_tune_"), "bench.wsdm")
+        tuned = fa.autotune(Ng, Lg, *ptr, Np, V, wisdom, dK=dK, Nmu=nmu, reps=2)
+        t_tune = time.perf_counter() - t0
+        print(f"# autotune: {tuned} in {t_tune:.1f}s", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+    plan = fa.PowerPlan(Ng, Lg, interlace=True, dK=dK, los=(0, 0, 1), Nmu=nmu, Np_max=Np,
+                        wisdom=wisdom)
     t_plan = time.perf_counter() - t0
     print(f"# plan: {t_plan:.2f}s", file=sys.stderr, flush=True)
     stream = torch.cuda.ExternalStream(plan.stream())
@@ -421,6 +447,7 @@ def main():
     pipe = {"B_alg_bytes": B, "GBps": round(B / (ms_per_step * 1e-3) / 1e9, 1),
             "frac_of_peak": round(B / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "stage_sum_ms": round(sum(acc.values()), 4)}
+    plan_tuning = plan.tuning()
     plan.close()
 
     # plain FP64 GPU form on the same catalog
@@ -500,7 +527,9 @@ def main():
         "pipeline": pipe,
         "stages": stages,
         "event_ms_per_step": round(ev_ms, 4),
-        "setup_s": {"mock": round(t_mock, 1), "plan": round(t_plan, 2)},
+        "setup_s": {"mock": round(t_mock, 1), "plan": round(t_plan, 2),
+                    "autotune": round(t_tune, 2)},
+        "tuning": plan_tuning,
         "device": name,
     }
     print(json.dumps(line))
