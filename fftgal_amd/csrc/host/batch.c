@@ -527,7 +527,7 @@ int fftgal_power_autotune(const fftgal_power_desc *desc, const float *x, const f
         d.wisdom = NULL;
         d.paint_tile = T;
         fftgal_power_t *p = NULL;
-        TRY(fftgal_power_create(&p, &d));
+        if (fftgal_power_create(&p, &d)) continue;   /* side not plannable here (tiles, memory) */
         float m[3];
         int rc = tune_measure(p, x, y, z, Np, V, reps, m);
         fftgal_power_destroy(p);
@@ -536,6 +536,12 @@ int fftgal_power_autotune(const fftgal_power_desc *desc, const float *x, const f
             tbest = m[0];
             best.paint_tile = T;
         }
+    }
+    if (!(tbest < 1e30f)) {
+        char why[256];
+        snprintf(why, sizeof why, "%s", fgk_last_error());
+        fgk_set_error("autotune: no paint tile side could be planned for Ng=%d (%s)", Ng, why);
+        return -2;
     }
     /* 2. FFT kernels and 3. bin block, on a plan with the chosen tile */
     fftgal_power_desc d = *desc;
