@@ -158,6 +158,11 @@ extern "C" int fgk_event_record(fgk_event_t e, fgk_stream_t s) {
     return 0;
 }
 
+extern "C" int fgk_event_sync(fgk_event_t e) {
+    FGK_HIP(hipEventSynchronize((hipEvent_t)e));
+    return 0;
+}
+
 extern "C" int fgk_event_elapsed_ms(float *ms, fgk_event_t a, fgk_event_t b) {
     FGK_HIP(hipEventSynchronize((hipEvent_t)b));
     FGK_HIP(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));

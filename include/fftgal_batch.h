@@ -160,6 +160,16 @@ int fftgal_batch_use_graph(fftgal_batch_t *b, int on);
 int fftgal_batch_tile(const fftgal_batch_t *b, int job);
 void *fftgal_batch_stream(fftgal_batch_t *b);
 
+/* ---- input ----
+ * A QPM catalog file (the format of gal_loadqpm_cubic, include/io.h)
+ * straight into device float32 arrays x, y, z of at least Np_max elements:
+ * threads parse the lines a piece at a time into pinned host buffers, and
+ * each piece's copy to the device overlaps the parsing of the next.  Values
+ * are those of the drop-in loader rounded to float32.  *Np receives the
+ * number of galaxies; more than Np_max lines is an error. */
+int fftgal_qpm_load_device(const char *file, long Np_max, float *x, float *y, float *z,
+                           long *Np);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,22 @@
+/* catalog input -- drop-in for eelregit/fftgal src/io/io.h.
+ * Implemented in fftgal_amd/csrc/host/qpm.c (multithreaded, memory-mapped). */
+#ifndef FFTGAL_IO_H
+#define FFTGAL_IO_H
+
+#include "gal.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* load QPM galaxies from periodic boxes: one galaxy per line, the columns
+ * x y z vx vy vz, then halo mass, satellite flag and halo id, which are
+ * skipped (src/io/io.h:4-6, src/io/qpm.c:9-34).  V = L^3; velocities are
+ * allocated and filled.  Aborts with a message on an unreadable file or a
+ * line that does not start with six numbers, as the reference asserts. */
+gal_t *gal_loadqpm_cubic(char *file, double L);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

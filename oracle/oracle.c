@@ -28,6 +28,36 @@ static inline double quart(double x) { double s = x * x; return s * s; } /* gsl_
 
 /* ---------------------------------------------------------------- catalog */
 
+/* src/io/qpm.c:9-34: count the '\n' characters, then read six numbers per
+ * line with fscanf, skipping the halo mass, satellite flag and halo id.
+ * Returns NULL (where the reference asserts) if the file cannot be opened or
+ * a line does not match. */
+ogal_t *oracle_loadqpm_cubic(const char *file, double L) {
+    FILE *fp = fopen(file, "r");
+    if (!fp) return NULL;
+    long Np = 0;
+    int c;
+    while ((c = fgetc(fp)) != EOF)
+        if (c == '\n') ++Np;
+    rewind(fp);
+    ogal_t *part = oracle_gal_init(Np, L * L * L);
+    size_t n = Np > 0 ? (size_t)Np : 1;
+    part->vx = (double *)malloc(n * sizeof(double));
+    part->vy = (double *)malloc(n * sizeof(double));
+    part->vz = (double *)malloc(n * sizeof(double));
+    for (long i = 0; i < Np; ++i) {
+        int r = fscanf(fp, "%lf %lf %lf %lf %lf %lf %*f %*d %*d\n", part->x + i, part->y + i,
+                       part->z + i, part->vx + i, part->vy + i, part->vz + i);
+        if (r != 6) {
+            fclose(fp);
+            oracle_gal_free(part);
+            return NULL;
+        }
+    }
+    fclose(fp);
+    return part;
+}
+
 /* src/gal.h:19-26 */
 void oracle_hat(double vec[3]) {
     double amp = sqrt(vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2]);

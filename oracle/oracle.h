@@ -24,6 +24,9 @@
  *   oracle_P / _Pl / _Pmu  closed forms, brute-force mode counts, sum N = Ng^3-1,
  *                       planted isotropic and planted-Legendre fields
  *   oracle_gal_rsd, oracle_hat  closed forms
+ *   oracle_loadqpm_cubic  every value equals Python's float() of its text
+ *                       field (correctly rounded), line count = '\n' count
+ *                       (a last line without '\n' is not read), velocities
  */
 #ifndef FFTGAL_ORACLE_H
 #define FFTGAL_ORACLE_H
@@ -103,6 +106,7 @@ ogal_t *oracle_gal_init(long Np, double V);
 void oracle_gal_free(ogal_t *self);
 void oracle_gal_wrap(ogal_t *self, double L);
 ogal_t *oracle_gal_rsd(const ogal_t *self, double los[3], double aH);
+ogal_t *oracle_loadqpm_cubic(const char *file, double L);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,7 @@
 """GPU parity of the drop-in API (include/fft.h, power.h) against the FP64
 oracle, through the C library.  Mirrors the reference's own test
 (src/tests/octet.c) and extends it to every hot-path function."""
+import ctypes as C
 import os
 
 import numpy as np
@@ -206,3 +207,89 @@ def test_c_octet_program(torch_cuda, lib):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "test PASSED" in out.stdout
+
+
+class _Ptr:
+    def __init__(self, p):
+        self.ptr = p
+
+
+def test_P_driver_flow_vs_oracle(torch_cuda, lib, orc, tmp_path):
+    """The reference's P.c flow (src/app/P.c:19-97) as a C program on the
+    unchanged API (tests/c/test_pflow.c: QPM load, six lines of sight in
+    redshift space, then real space reusing one transform through
+    fft_exportf / fft_importf), against the same sequence on the oracle:
+    all 24 text files with identical headers and counts, values to the 7
+    printed digits."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "c", "test_pflow")
+    assert os.path.exists(exe), "tests/c/test_pflow not built"
+    rng = np.random.default_rng(17)
+    n, L, Ng, fold, dK, a = 20000, 500.0, 32, 2, 0.02, 0.6
+    pos = rng.random((n, 3)) * L
+    vel = rng.standard_normal((n, 3)) * 400.0
+    path = str(tmp_path / "mock.qpm")
+    with open(path, "w") as fp:
+        for p, v in zip(pos, vel):
+            fp.write(f"{p[0]:.5f} {p[1]:.5f} {p[2]:.5f} {v[0]:.3f} {v[1]:.3f} {v[2]:.3f} "
+                     f"2.5e+12 1 7\n")
+    gdir, odir = tmp_path / "gpu", tmp_path / "orc"
+    gdir.mkdir()
+    odir.mkdir()
+    out = subprocess.run([exe, str(Ng), str(L), str(fold), str(dK), path, str(a), str(gdir)],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    # the same flow on the oracle
+    o = orc.load()
+    part = _Ptr(o.oracle_loadqpm_cubic(path.encode(), L))
+    g = orc.Grid(Ng, L / fold)
+    aH = a * 100 * np.sqrt(0.29 / a ** 3 + 0.71)
+    los_all = [(i, j, k) for i in (0, 1) for j in (0, 1) for k in (0, 1)
+               if 0 < i + j + k < 3]
+    # one los array per line of sight through gal_rsd, Pl and Pmu, each of
+    # which normalises it in place (src/gal.c:64, src/power.c:33,107), as
+    # P.c passes it: a second normalisation can change its last bit, and
+    # with it the wedge of modes with |mu| = 1/3 or 2/3 exactly
+    dp = C.POINTER(C.c_double)
+    for los in los_all:
+        lv = np.array(los, dtype=np.float64)
+        rsd = _Ptr(o.oracle_gal_rsd(part.ptr, lv.ctypes.data_as(dp), aH))
+        g.p2k(rsd)
+        g.P(rsd)
+        tag = "%d%d%d" % los
+        o.oracle_Pl(g.ptr, rsd.ptr, lv.ctypes.data_as(dp), dK,
+                    str(odir / f"Pl_rsd1_los{tag}_fd{fold}.txt").encode())
+        o.oracle_Pmu(g.ptr, rsd.ptr, lv.ctypes.data_as(dp), dK,
+                     str(odir / f"Pmu_rsd1_los{tag}_fd{fold}.txt").encode())
+        o.oracle_gal_free(rsd.ptr)
+    g.p2k(part)
+    fk = o.oracle_fft_exportf(g.ptr)
+    for los in los_all:
+        o.oracle_fft_importf(g.ptr, fk)
+        g.P(part)
+        tag = "%d%d%d" % los
+        lv = np.array(los, dtype=np.float64)
+        o.oracle_Pl(g.ptr, part.ptr, lv.ctypes.data_as(dp), dK,
+                    str(odir / f"Pl_rsd0_los{tag}_fd{fold}.txt").encode())
+        o.oracle_Pmu(g.ptr, part.ptr, lv.ctypes.data_as(dp), dK,
+                     str(odir / f"Pmu_rsd0_los{tag}_fd{fold}.txt").encode())
+    import ctypes
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+    libc.free(ctypes.cast(fk, ctypes.c_void_p))
+    o.oracle_gal_free(part.ptr)
+    names = sorted(os.listdir(odir))
+    assert len(names) == 24 and sorted(os.listdir(gdir)) == names
+    for name in names:
+        ho, to = _read_table(str(odir / name))
+        hg, tg = _read_table(str(gdir / name))
+        assert ho == hg, name
+        assert to.shape == tg.shape, name
+        assert np.array_equal(np.isnan(to), np.isnan(tg)), name
+        m = ~np.isnan(to)
+        # 7 printed digits; values that cancel to near zero (a wedge's P
+        # after the shot-noise subtraction) differ by the FP64 summation
+        # order, far below the column's scale
+        scale = np.abs(to[m]).max()
+        bad = ~np.isclose(tg[m], to[m], rtol=2e-6, atol=1e-12 * scale)
+        assert not bad.any(), (name, tg[m][bad][:5], to[m][bad][:5])

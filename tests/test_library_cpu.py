@@ -113,3 +113,76 @@ def test_no_gpu_means_clean_error():
                          timeout=120)
     assert out.returncode == 0, out.stderr
     assert "ERR" in out.stdout and "no HIP device" in out.stdout
+
+
+def _write_qpm(path, n, seed, trailing_newline=True):
+    """A QPM-format file: x y z vx vy vz halo_mass sat_flag halo_id."""
+    rng = np.random.default_rng(seed)
+    pos = rng.random((n, 3)) * 1000.0
+    vel = rng.standard_normal((n, 3)) * 300.0
+    mass = 10 ** rng.uniform(11, 15, n)
+    sat = rng.integers(0, 2, n)
+    hid = rng.integers(0, 10 ** 9, n)
+    lines = [f"{p[0]:.6f} {p[1]:.6f} {p[2]:.6f} {v[0]:.4f} {v[1]:.4f} {v[2]:.4f} {m:.6e} {s} {h}"
+             for p, v, m, s, h in zip(pos, vel, mass, sat, hid)]
+    text = "\n".join(lines) + ("\n" if trailing_newline and lines else "")
+    with open(path, "w") as fp:
+        fp.write(text)
+    return lines
+
+
+def _gal_arrays(ptr):
+    g = ptr.contents
+    n = g.Np
+    return [np.ctypeslib.as_array(getattr(g, k), shape=(n,)).copy() if n else np.zeros(0)
+            for k in ("x", "y", "z", "vx", "vy", "vz")]
+
+
+@pytest.mark.parametrize("n,trailing,threads,chunk", [(0, True, None, None), (1, True, None, None),
+                                                      (1000, False, None, None),
+                                                      (120000, True, None, None),
+                                                      (120000, True, "1", None),
+                                                      (120000, True, "3", None),
+                                                      (5000, False, "4", "4096")])
+def test_loadqpm_matches_oracle(lib, orc, tmp_path, monkeypatch, n, trailing, threads, chunk):
+    """gal_loadqpm_cubic (src/io/qpm.c:9-34), multithreaded over 8 MiB
+    chunks, against the oracle's stdio restatement and against Python's
+    float() of every field: bit-identical values, the reference's line count
+    (a last line without '\\n' is not read), V = L^3."""
+    import fftgal_amd as fa
+    if threads:
+        monkeypatch.setenv("FFTGAL_IO_THREADS", threads)
+    if chunk:
+        monkeypatch.setenv("FFTGAL_QPM_CHUNK_BYTES", chunk)
+    path = str(tmp_path / "mock.qpm")
+    lines = _write_qpm(path, n, seed=n, trailing_newline=trailing)
+    L = 1000.0
+    g = lib.gal_loadqpm_cubic(path.encode(), L)
+    o = orc.load().oracle_loadqpm_cubic(path.encode(), L)
+    assert o
+    nread = n if trailing else n - 1
+    assert g.contents.Np == nread == o.contents.Np
+    assert g.contents.V == L ** 3
+    got, ref = _gal_arrays(g), _gal_arrays(o)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+    cols = np.array([[float(v) for v in ln.split()[:6]] for ln in lines[:nread]]).reshape(-1, 6)
+    for k in range(6):
+        assert np.array_equal(got[k], cols[:, k])
+    lib.gal_free(g)
+    orc.load().oracle_gal_free(o)
+
+
+def test_loadqpm_rejects_bad_line(lib, tmp_path):
+    """A line that does not start with six numbers aborts with a message, as
+    the reference's assert does (run in a child process)."""
+    import subprocess
+    import sys
+    path = tmp_path / "bad.qpm"
+    path.write_text("1 2 3 4 5 6 7 0 1\n1 2 three 4 5 6 7 0 1\n")
+    code = ("import fftgal_amd as fa; lib = fa.load(); "
+            f"lib.gal_loadqpm_cubic({str(path)!r}.encode(), 10.0)")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                       cwd=os.path.dirname(INC), timeout=120)
+    assert r.returncode != 0
+    assert "line 2" in r.stderr
