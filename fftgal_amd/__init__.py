@@ -110,6 +110,8 @@ API = {
     "fftgal_power_stage_name": (C.c_char_p, [C.c_int]),
     "fftgal_power_profile": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_long, C.c_double, C.POINTER(C.c_float)]),
+    "fft_dl_filter": (None, [C.POINTER(FftT), C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int,
+                             C.c_double, C.POINTER(C.c_double)]),
     # include/io.h
     "gal_loadqpm_cubic": (C.POINTER(GalT), [C.c_char_p, C.c_double]),
     "fftgal_qpm_load_device": (C.c_int, [C.c_char_p, C.c_long, C.c_void_p, C.c_void_p,

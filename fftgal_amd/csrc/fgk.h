@@ -141,6 +141,13 @@ int  fgk_paint_batch_tile(const fgk_paintbatch *pb, int job);
 int  fgk_offset_phase_f64(double *f, int Ng, const double *phase_dev, fgk_stream_t s);
 int  fgk_deconv_f64(double *f, int Ng, const double *winv_dev, fgk_stream_t s);
 int  fgk_P_f64(double *f, int Ng, double Vinv, double Pshot, fgk_stream_t s);
+/* super-sample k filter (kind 0: sub box, tables wamp[Ng], wph[3*Ng][2];
+ * kind 1: sub sphere) times L_ell(mu), and the gather of the Nsub^3
+ * sub-volume corners of the configuration-space grid divided by bias */
+int  fgk_dl_filter_f64(double *f, int Ng, int kind, int Nsub, const double los[3], int ell,
+                       const double *wamp_dev, const double *wph_dev, fgk_stream_t s);
+int  fgk_dl_gather_f64(const double *f, int Ng, int Nsub, double bias, double *out_dev,
+                       fgk_stream_t s);
 
 /* binning.  Bins are accumulated into bins_dev (doubles) laid out as
  *   [NK][4]         Pl sums (l = 0,2,4,6)      at offset 0

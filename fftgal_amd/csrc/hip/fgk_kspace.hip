@@ -86,6 +86,59 @@ __global__ void k_P_f64(double *f, int Ng, double Vinv, double Pshot) {
     }
 }
 
+// Super-sample filter of src/app/DLsb.c:66-88 (kind 0: sub-box window with
+// its phase, tables wamp[Ng] and wph[3 Ng][2] built on the host as the
+// reference builds them) and src/app/DLss.c:62-82 (kind 1: spherical top
+// hat of K R = Kamp pi sqrt(3) / Nsub), times L_ell(mu) for ell = 0, 2, 4;
+// Re of the DC mode set to 0 and the DC mode otherwise untouched.  The
+// frequencies carry the Nyquist index as -Ng/2 (the reference's Kval table).
+__global__ void k_dl_filter_f64(double *f, int Ng, int kind, int Nsub, double l0, double l1,
+                                double l2, int ell, const double *__restrict__ wamp,
+                                const double *__restrict__ wph) {
+#pragma clang fp contract(off)
+    const long nmodes = (long)Ng * Ng * (Ng / 2 + 1);
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < nmodes; m += stride) {
+        if (m == 0) {
+            f[0] = 0;
+            continue;
+        }
+        int i, j, k;
+        mode_ijk(m, Ng, i, j, k);
+        const double Ki = 2 * i < Ng ? i : i - Ng, Kj = 2 * j < Ng ? j : j - Ng,
+                     Kk = 2 * k < Ng ? k : k - Ng;
+        const double Kamp = sqrt(Ki * Ki + Kj * Kj + Kk * Kk);
+        const double q = (Ki * l0 + Kj * l1 + Kk * l2) / Kamp;
+        const double mu2 = q * q;
+        const double leg = ell == 0 ? 1.0
+                         : ell == 2 ? 1.5 * mu2 - 0.5
+                                    : (4.375 * mu2 - 3.75) * mu2 + 0.375;
+        double *c = f + 2 * m;
+        const double fr = c[0], fi = c[1];
+        if (kind == 0) {
+            const double w3 = wamp[i] * wamp[j] * wamp[k];
+            const double wr = wph[2 * (i + j + k)] * w3, wi = wph[2 * (i + j + k) + 1] * w3;
+            c[0] = leg * (fr * wr - fi * wi);
+            c[1] = leg * (fr * wi + fi * wr);
+        } else {
+            const double x = Kamp * M_PI * 1.73205080756887729352744634151 / Nsub;
+            const double W = 3 * (sin(x) - x * cos(x)) / (x * x * x);
+            c[0] = fr * (leg * W);
+            c[1] = fi * (leg * W);
+        }
+    }
+}
+
+// out[(a*Nsub + b)*Nsub + c] = f[(a*Ngsub, b*Ngsub, c*Ngsub)] / bias (padded layout)
+__global__ void k_dl_gather(const double *f, int Ng, int Nsub, double bias, double *out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Nsub * Nsub * Nsub) return;
+    const int a = t / (Nsub * Nsub), b = (t / Nsub) % Nsub, c = t % Nsub;
+    const int gs = Ng / Nsub;
+    const long Ng_pad = 2 * (Ng / 2 + 1);
+    out[t] = f[((long)a * gs * Ng + (long)b * gs) * Ng_pad + (long)c * gs] / bias;
+}
+
 // workgroups for a grid-stride walk over n elements with 256 threads
 static unsigned stride_blocks(long n) {
     long b = (n + 255) / 256;
@@ -105,6 +158,25 @@ extern "C" int fgk_deconv_f64(double *f, int Ng, const double *winv_dev, fgk_str
     long n = (long)Ng * Ng * (Ng / 2 + 1);
     hipLaunchKernelGGL(k_deconv_f64, dim3(stride_blocks(n)), dim3(256), 0, (hipStream_t)s, f,
                        Ng, winv_dev);
+    FGK_LAUNCHED();
+    return 0;
+}
+
+extern "C" int fgk_dl_filter_f64(double *f, int Ng, int kind, int Nsub, const double los[3],
+                                 int ell, const double *wamp_dev, const double *wph_dev,
+                                 fgk_stream_t s) {
+    const long n = (long)Ng * Ng * (Ng / 2 + 1);
+    hipLaunchKernelGGL(k_dl_filter_f64, dim3(stride_blocks(n)), dim3(256), 0, (hipStream_t)s, f,
+                       Ng, kind, Nsub, los[0], los[1], los[2], ell, wamp_dev, wph_dev);
+    FGK_LAUNCHED();
+    return 0;
+}
+
+extern "C" int fgk_dl_gather_f64(const double *f, int Ng, int Nsub, double bias, double *out_dev,
+                                 fgk_stream_t s) {
+    const int n = Nsub * Nsub * Nsub;
+    hipLaunchKernelGGL(k_dl_gather, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)s, f, Ng,
+                       Nsub, bias, out_dev);
     FGK_LAUNCHED();
     return 0;
 }

@@ -289,6 +289,43 @@ void fft_k2x(fft_t *self, int offset_phase_off) {
     }
 }
 
+/* super-sample modes of sub-volumes: the k-space loop, inverse transform
+ * and corner reads of src/app/DLsb.c:66-96 (kind 0, with the window tables
+ * of DLsb.c:42-49) and src/app/DLss.c:62-89 (kind 1) on the device */
+void fft_dl_filter(fft_t *self, int kind, int Nsub, const double los[3], int ell, double bias,
+                   double *delta) {
+    const int Ng = self->Ng;
+    DI_ASSERT(kind == 0 || kind == 1, "fft_dl_filter: kind must be 0 (sub box) or 1 (sub sphere)");
+    DI_ASSERT(Nsub >= 1 && Ng % Nsub == 0, "fft_dl_filter: Nsub must divide Ng");
+    DI_ASSERT(ell == 0 || ell == 2 || ell == 4, "fft_dl_filter: ell must be 0, 2 or 4");
+    fft_dev *d = fft_dev_of(self);
+    const int Ngsub = Ng / Nsub, n = Nsub * Nsub * Nsub;
+    const size_t ntab = (size_t)Ng + (size_t)3 * Ng * 2;
+    double *tab = (double *)malloc(ntab * sizeof(double));
+    DI_ASSERT(tab != NULL, "fft_dl_filter: out of memory");
+    double *wamp = tab, *wph = tab + Ng;
+    wamp[0] = 1;
+    for (int i = 1; i < Ng; ++i) wamp[i] = sin(M_PI * i / Nsub) / sin(M_PI * i / Ng) / Ngsub;
+    for (int i = 0; i < 3 * Ng; ++i) {
+        wph[2 * i] = cos(M_PI * i * (Ngsub - 1) / Ng);
+        wph[2 * i + 1] = sin(M_PI * i * (Ngsub - 1) / Ng);
+    }
+    double *dtab = NULL, *dout = NULL;
+    DI_CHECK(fgk_malloc((void **)&dtab, ntab * sizeof(double)));
+    DI_CHECK(fgk_malloc((void **)&dout, (size_t)n * sizeof(double)));
+    DI_CHECK(fgk_h2d(dtab, tab, ntab * sizeof(double), d->s));
+    fftgal_dev_pull(self);
+    DI_CHECK(fgk_dl_filter_f64(d->d_f, Ng, kind, Nsub, los, ell, dtab, dtab + Ng, d->s));
+    DI_CHECK(fgk_fft_c2r_f64(d->plan, d->d_f, 1 / cube(self->L), d->s));   /* fft_k2x(grid, 0) */
+    DI_CHECK(fgk_dl_gather_f64(d->d_f, Ng, Nsub, bias, dout, d->s));
+    DI_CHECK(fgk_d2h(delta, dout, (size_t)n * sizeof(double), d->s));
+    DI_CHECK(fgk_stream_sync(d->s));
+    fftgal_dev_push(self);
+    fgk_free(dtab);
+    fgk_free(dout);
+    free(tab);
+}
+
 /* src/fft.c:196-233 */
 void fft_offset_phase(fft_t *self, int onoff) {
     DI_ASSERT(onoff != 0, "fft_offset_phase: onoff must be nonzero");

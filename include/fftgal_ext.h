@@ -42,6 +42,19 @@ int Pl_bins(fft_t *grid, double los[3], double dK, double *K, long *N, double *P
 /* Pmu with Nmu wedges as arrays: K, N, P each [NK*Nmu]; returns NK */
 int Pmu_bins(fft_t *grid, double los[3], double dK, int Nmu, double *K, long *N, double *P);
 
+/* Super-sample modes of sub-volumes (the k-space loop of src/app/DLsb.c:66-96
+ * and DLss.c:62-89 on the device).  On the grid's half spectrum from
+ * fft_x2k (restored e.g. by fft_importf): Re of the DC mode set to 0, every
+ * other mode times W(k) L_ell(mu), with W the sub-box window with its phase
+ * (kind 0, DLsb) or the spherical top hat of K R = |n| pi sqrt(3) / Nsub
+ * (kind 1, DLss); then fft_k2x(grid, 0), and the Nsub^3 corners
+ * f(a Ng/Nsub, b Ng/Nsub, c Ng/Nsub) / bias into delta[(a*Nsub + b)*Nsub + c].
+ * los as the reference passes it (already normalised by hat()); ell = 0, 2
+ * or 4; Nsub divides Ng.  grid->f holds the filtered configuration-space
+ * grid afterwards, as after fft_k2x. */
+void fft_dl_filter(fft_t *grid, int kind, int Nsub, const double los[3], int ell, double bias,
+                   double *delta);
+
 #ifdef __cplusplus
 }
 #endif

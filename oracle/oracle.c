@@ -445,6 +445,62 @@ int oracle_Pmu_bins(const offt_t *g, double los[3], double dK, int Nmu,
     return NK;
 }
 
+/* src/app/DLsb.c:42-49,66-96 (kind 0) and src/app/DLss.c:18-20,62-89
+ * (kind 1): Re of the DC mode to 0, every other mode times the window and
+ * L_ell(mu) (ell = 0, 2, 4 is the reference's iL = 0, 1, 2), fft_k2x(g, 0),
+ * then the sub-volume corners over bias */
+void oracle_dl_filter(offt_t *g, int kind, int Nsub, const double los[3], int ell, double bias,
+                      double *delta) {
+    const int Ng = g->Ng, Ngsub = Ng / Nsub;
+    double *Wamp = (double *)malloc((size_t)Ng * sizeof(double));
+    double *Wph = (double *)malloc((size_t)3 * Ng * 2 * sizeof(double));
+    double *Kval = (double *)malloc((size_t)Ng * sizeof(double));
+    assert(Wamp && Wph && Kval);
+    Wamp[0] = 1;
+    for (int i = 1; i < Ng; ++i) Wamp[i] = sin(M_PI * i / Nsub) / sin(M_PI * i / Ng) / Ngsub;
+    for (int i = 0; i < 3 * Ng; ++i) {
+        Wph[2 * i] = cos(M_PI * i * (Ngsub - 1) / Ng);
+        Wph[2 * i + 1] = sin(M_PI * i * (Ngsub - 1) / Ng);
+    }
+    Kval[0] = 0;
+    for (int i = 1; i <= Ng / 2; ++i) {
+        Kval[i] = i;
+        Kval[Ng - i] = -i;
+    }
+    const int iL = ell / 2;
+    g->f[0] = 0;
+    for (int i = 0; i < Ng; ++i)
+        for (int j = 0; j < Ng; ++j)
+            for (int k = (i == 0 && j == 0); k <= Ng / 2; ++k) {
+                double Kamp = sqrt(sq(Kval[i]) + sq(Kval[j]) + sq(Kval[k]));
+                double mu2 = sq((Kval[i] * los[0] + Kval[j] * los[1] + Kval[k] * los[2]) / Kamp);
+                double Legendre[3] = {1, 1.5 * mu2 - 0.5, (4.375 * mu2 - 3.75) * mu2 + 0.375};
+                double *c = g->f + FIDX(g, i, j, 2 * k);
+                if (kind == 0) {
+                    double Wamp3 = Wamp[i] * Wamp[j] * Wamp[k];
+                    double W_re = Wph[2 * (i + j + k)] * Wamp3;
+                    double W_im = Wph[2 * (i + j + k) + 1] * Wamp3;
+                    double f_re = c[0], f_im = c[1];
+                    c[0] = Legendre[iL] * (f_re * W_re - f_im * W_im);
+                    c[1] = Legendre[iL] * (f_re * W_im + f_im * W_re);
+                } else {
+                    double KR = Kamp * M_PI * ORC_SQRT3 / Nsub;
+                    double W = 3 * (sin(KR) - KR * cos(KR)) / cube(KR);
+                    c[0] *= Legendre[iL] * W;
+                    c[1] *= Legendre[iL] * W;
+                }
+            }
+    oracle_fft_k2x(g, 0);
+    for (int a = 0; a < Nsub; ++a)
+        for (int b = 0; b < Nsub; ++b)
+            for (int cc = 0; cc < Nsub; ++cc)
+                delta[(a * Nsub + b) * Nsub + cc] =
+                    g->f[FIDX(g, a * Ngsub, b * Ngsub, cc * Ngsub)] / bias;
+    free(Wamp);
+    free(Wph);
+    free(Kval);
+}
+
 /* src/power.c:78-99 */
 void oracle_Pl(offt_t *g, const ogal_t *part, double los[3], double dK, const char *file) {
     int NK = oracle_NK(g->Ng, g->L, dK);

@@ -24,6 +24,10 @@
  *   oracle_P / _Pl / _Pmu  closed forms, brute-force mode counts, sum N = Ng^3-1,
  *                       planted isotropic and planted-Legendre fields
  *   oracle_gal_rsd, oracle_hat  closed forms
+ *   oracle_dl_filter    the sub-box window against its defining sum (the
+ *                       mean of e^{ik.x} over a sub box, brute force), the
+ *                       top hat against its series at small KR, and
+ *                       uniform-field and Legendre-plane closed forms
  *   oracle_loadqpm_cubic  every value equals Python's float() of its text
  *                       field (correctly rounded), line count = '\n' count
  *                       (a last line without '\n' is not read), velocities
@@ -107,6 +111,8 @@ void oracle_gal_free(ogal_t *self);
 void oracle_gal_wrap(ogal_t *self, double L);
 ogal_t *oracle_gal_rsd(const ogal_t *self, double los[3], double aH);
 ogal_t *oracle_loadqpm_cubic(const char *file, double L);
+void oracle_dl_filter(offt_t *g, int kind, int Nsub, const double los[3], int ell, double bias,
+                      double *delta);
 
 #ifdef __cplusplus
 }

@@ -74,6 +74,7 @@ def load():
         "oracle_gal_wrap": (None, [P, C.c_double]),
         "oracle_gal_rsd": (P, [P, dptr, C.c_double]),
         "oracle_loadqpm_cubic": (P, [C.c_char_p, C.c_double]),
+        "oracle_dl_filter": (None, [G, C.c_int, C.c_int, dptr, C.c_int, C.c_double, dptr]),
         "offt64_r2c_3d": (None, [G]),
         "offt64_c2r_3d": (None, [G]),
     }

@@ -262,3 +262,65 @@ def test_gal_rsd_closed_form(orc):
         got = np.ctypeslib.as_array(getattr(r.contents, name), shape=(50,))
         assert np.allclose(got, p0 + s * lh[a], rtol=0, atol=1e-12)
     orc.load().oracle_gal_free(r)
+
+
+def _dl(orc, g, kind, Nsub, los, ell):
+    import ctypes as C
+    l = np.array(los, dtype=np.float64)
+    out = np.zeros(Nsub ** 3)
+    dp = C.POINTER(C.c_double)
+    orc.load().oracle_dl_filter(g.ptr, kind, Nsub, l.ctypes.data_as(dp), ell, 1.0,
+                                out.ctypes.data_as(dp))
+    return out.reshape(Nsub, Nsub, Nsub)
+
+
+def test_dl_subbox_monopole_is_the_subbox_mean(orc):
+    """src/app/DLsb.c: the sub-box window W(k) = mean over s < Ng/Nsub of
+    e^{2 pi i k s / Ng} per axis, so the monopole at a corner is the mean of
+    the field (less its grid mean, the DC mode being zeroed) over the sub box
+    starting at that corner."""
+    Ng, Nsub, L = 24, 4, 100.0
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((Ng, Ng, Ng))
+    g = orc.Grid(Ng, L)
+    g.f[:, :, :Ng] = x
+    g.x2k(0)
+    got = _dl(orc, g, 0, Nsub, (0, 0, 1), 0)
+    s = Ng // Nsub
+    ref = (x - x.mean()).reshape(Nsub, s, Nsub, s, Nsub, s).mean(axis=(1, 3, 5))
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+    g.free()
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("ell", [0, 2, 4])
+def test_dl_plane_wave(orc, kind, ell):
+    """One plane wave cos(2 pi n.x / Ng) (n with n_z > 0): the filtered field
+    at every corner is L_ell(mu_n) Re[W(n) e^{2 pi i n.x / Ng}], with W the
+    sub-box window from its defining sum (kind 0) or the spherical top hat of
+    radius sqrt(3) L / (2 Nsub) from a quadrature of its defining integral
+    (kind 1), and L_ell from numpy."""
+    from numpy.polynomial import legendre
+    from scipy.integrate import quad
+    Ng, Nsub, L = 24, 3, 300.0
+    n = np.array([2, -1, 3])
+    ax = np.arange(Ng)
+    X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
+    phase = 2 * np.pi * (n[0] * X + n[1] * Y + n[2] * Z) / Ng
+    g = orc.Grid(Ng, L)
+    g.f[:, :, :Ng] = np.cos(phase)
+    g.x2k(0)
+    los = np.array([1.0, 2.0, 2.0]) / 3.0
+    got = _dl(orc, g, kind, Nsub, los, ell)
+    s = Ng // Nsub
+    if kind == 0:
+        W = np.prod([np.exp(2j * np.pi * k * np.arange(s) / Ng).mean() for k in n % Ng])
+    else:
+        kR = np.linalg.norm(n) * np.pi * np.sqrt(3) / Nsub     # |k| R with R = sqrt(3) L / (2 Nsub)
+        W = 3 * quad(lambda r: np.sin(kR * r) / (kR * r) * r * r, 0, 1, epsabs=1e-15)[0]
+    mu = n @ los / np.linalg.norm(n)
+    Lell = legendre.legval(mu, [0] * ell + [1])
+    corners = phase[::s, ::s, ::s]
+    ref = Lell * np.real(W * np.exp(1j * corners))
+    assert np.abs(got - ref).max() <= 1e-12, (np.abs(got - ref).max())
+    g.free()
