@@ -112,6 +112,9 @@ API = {
                                        C.c_long, C.c_double, C.POINTER(C.c_float)]),
     "fft_dl_filter": (None, [C.POINTER(FftT), C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int,
                              C.c_double, C.POINTER(C.c_double)]),
+    "gal_subbox": (C.POINTER(GalT), [C.POINTER(GalT), C.POINTER(C.c_double), C.c_double]),
+    "gal_subsphere": (C.POINTER(GalT), [C.POINTER(GalT), C.POINTER(C.c_double), C.c_double]),
+    "fft_subvolume_deconv": (None, [C.POINTER(FftT), C.c_int, C.c_int]),
     # include/io.h
     "gal_loadqpm_cubic": (C.POINTER(GalT), [C.c_char_p, C.c_double]),
     "fftgal_qpm_load_device": (C.c_int, [C.c_char_p, C.c_long, C.c_void_p, C.c_void_p,

@@ -148,6 +148,9 @@ int  fgk_dl_filter_f64(double *f, int Ng, int kind, int Nsub, const double los[3
                        const double *wamp_dev, const double *wph_dev, fgk_stream_t s);
 int  fgk_dl_gather_f64(const double *f, int Ng, int Nsub, double bias, double *out_dev,
                        fgk_stream_t s);
+/* sub-volume window deconvolution of a real-space grid (kind 0: sub box of
+ * Ng/Nsub cells; kind 1: sub sphere of diameter sqrt(3) Ng/Nsub cells) */
+int  fgk_subvolume_winv_f64(double *f, int Ng, int kind, int Nsub, fgk_stream_t s);
 
 /* binning.  Bins are accumulated into bins_dev (doubles) laid out as
  *   [NK][4]         Pl sums (l = 0,2,4,6)      at offset 0

@@ -139,6 +139,57 @@ __global__ void k_dl_gather(const double *f, int Ng, int Nsub, double bias, doub
     out[t] = f[((long)a * gs * Ng + (long)b * gs) * Ng_pad + (long)c * gs] / bias;
 }
 
+// Window deconvolution of a sub-volume correlation function on the real
+// grid (padded layout), the host loops of src/app/Psb.c:74-83 (kind 0:
+// lags |i|,|j|,|k| < gs, factor gs^3 / prod(gs - |lag|)) and
+// src/app/Pss.c:71-84 (kind 1: lags |i|,|j|,|k| < ceil(sqrt(3) gs) with
+// i^2 + j^2 + k^2 < 0.99 * 3 gs^2, factor 1 / ((1 - d)^2 (1 + d/2)),
+// d = |lag| / (sqrt(3) gs)).  A lag reaches cell (i + Ng) % Ng; when two
+// lags of one axis reach the same cell (the sphere's range exceeds Ng),
+// the cell takes both factors, in the reference's loop order.
+__global__ void k_subvolume_winv_f64(double *f, int Ng, int kind, int gs) {
+#pragma clang fp contract(off)
+    const long ncell = (long)Ng * Ng * Ng;
+    const long stride = (long)gridDim.x * blockDim.x;
+    const long Ng_pad = 2 * (Ng / 2 + 1);
+    const double D2 = 3.0 * gs * gs;
+    const int bnd = kind == 0 ? gs : (int)ceil(sqrt(D2));
+    for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < ncell; m += stride) {
+        const int x = (int)(m / ((long)Ng * Ng)), y = (int)((m / Ng) % Ng), z = (int)(m % Ng);
+        // lags in (-bnd, bnd) congruent to the cell index, ascending
+        int li[4], lj[4], lk[4], ni = 0, nj = 0, nk = 0;
+        for (int q = -2; q <= 2; ++q) {
+            const int a = x + q * Ng, b = y + q * Ng, c = z + q * Ng;
+            if (a > -bnd && a < bnd) li[ni++] = a;
+            if (b > -bnd && b < bnd) lj[nj++] = b;
+            if (c > -bnd && c < bnd) lk[nk++] = c;
+        }
+        if (!ni || !nj || !nk) continue;
+        double *cell = f + ((long)x * Ng + y) * Ng_pad + z;
+        double v = *cell;
+        for (int ia = 0; ia < ni; ++ia)
+            for (int ib = 0; ib < nj; ++ib)
+                for (int ic = 0; ic < nk; ++ic) {
+                    const int i = li[ia], j = lj[ib], k = lk[ic];
+                    if (kind == 0) {
+                        const double g3 = (double)gs * gs * gs;
+                        const double w = g3 / ((double)(gs - abs(i)) * (gs - abs(j)) *
+                                               (gs - abs(k)));
+                        v *= w;
+                    } else {
+                        const double d2 = (double)i * i + (double)j * j + (double)k * k;
+                        if (d2 < 0.99 * D2) {
+                            const double dD = sqrt(d2 / D2);
+                            v *= 1 / ((1 - dD) * (1 - dD) * (1 + 0.5 * dD));
+                        }
+                    }
+                }
+        *cell = v;
+    }
+}
+
+extern "C" int fgk_subvolume_winv_f64(double *f, int Ng, int kind, int Nsub, fgk_stream_t s);
+
 // workgroups for a grid-stride walk over n elements with 256 threads
 static unsigned stride_blocks(long n) {
     long b = (n + 255) / 256;
@@ -168,6 +219,14 @@ extern "C" int fgk_dl_filter_f64(double *f, int Ng, int kind, int Nsub, const do
     const long n = (long)Ng * Ng * (Ng / 2 + 1);
     hipLaunchKernelGGL(k_dl_filter_f64, dim3(stride_blocks(n)), dim3(256), 0, (hipStream_t)s, f,
                        Ng, kind, Nsub, los[0], los[1], los[2], ell, wamp_dev, wph_dev);
+    FGK_LAUNCHED();
+    return 0;
+}
+
+extern "C" int fgk_subvolume_winv_f64(double *f, int Ng, int kind, int Nsub, fgk_stream_t s) {
+    const long n = (long)Ng * Ng * Ng;
+    hipLaunchKernelGGL(k_subvolume_winv_f64, dim3(stride_blocks(n)), dim3(256), 0,
+                       (hipStream_t)s, f, Ng, kind, Ng / Nsub);
     FGK_LAUNCHED();
     return 0;
 }

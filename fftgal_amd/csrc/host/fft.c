@@ -326,6 +326,26 @@ void fft_dl_filter(fft_t *self, int kind, int Nsub, const double los[3], int ell
     free(tab);
 }
 
+/* the window deconvolution of src/app/Psb.c:73-85 (kind 0) and
+ * src/app/Pss.c:70-87 (kind 1): fft_k2x(grid, 0), the correlation function
+ * divided by the sub-volume's window autocorrelation, fft_x2k(grid, 0) */
+void fft_subvolume_deconv(fft_t *self, int kind, int Nsub) {
+    const int Ng = self->Ng;
+    DI_ASSERT(kind == 0 || kind == 1, "fft_subvolume_deconv: kind must be 0 (box) or 1 (sphere)");
+    DI_ASSERT(Nsub >= 1 && Ng % Nsub == 0, "fft_subvolume_deconv: Nsub must divide Ng");
+    double t = fftgal_now();
+    fft_dev *d = fft_dev_of(self);
+    fftgal_dev_pull(self);
+    DI_CHECK(fgk_fft_c2r_f64(d->plan, d->d_f, 1 / cube(self->L), d->s));
+    DI_CHECK(fgk_subvolume_winv_f64(d->d_f, Ng, kind, Nsub, d->s));
+    DI_CHECK(fgk_fft_r2c_f64(d->plan, d->d_f, cube(self->L / Ng), d->s));
+    fftgal_dev_push(self);
+    if (fftgal_verbose()) {
+        fgk_stream_sync(d->s);
+        fprintf(stderr, "fft_subvolume_deconv() %.2fs\n", fftgal_now() - t);
+    }
+}
+
 /* src/fft.c:196-233 */
 void fft_offset_phase(fft_t *self, int onoff) {
     DI_ASSERT(onoff != 0, "fft_offset_phase: onoff must be nonzero");

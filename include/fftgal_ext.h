@@ -55,6 +55,17 @@ int Pmu_bins(fft_t *grid, double los[3], double dK, int Nmu, double *K, long *N,
 void fft_dl_filter(fft_t *grid, int kind, int Nsub, const double los[3], int ell, double bias,
                    double *delta);
 
+/* Window deconvolution of a sub-volume power spectrum (the loops of
+ * src/app/Psb.c:73-85 and Pss.c:70-87 on the device).  The grid holds P(k)
+ * from P() of a sub-volume catalog (gal_subbox / gal_subsphere); it is
+ * transformed back (fft_k2x(grid, 0)), the correlation function at lag
+ * (i, j, k) cells is divided by the window's autocorrelation -- kind 0, a
+ * sub box of gs = Ng/Nsub cells: prod(gs - |lag|) / gs^3 for |lags| < gs;
+ * kind 1, a sub sphere of diameter D = sqrt(3) gs cells:
+ * (1 - d)^2 (1 + d/2), d = |lag| / D, for |lag|^2 < 0.99 D^2 -- and
+ * transformed forward (fft_x2k(grid, 0)), ready for Pl / Pmu. */
+void fft_subvolume_deconv(fft_t *grid, int kind, int Nsub);
+
 #ifdef __cplusplus
 }
 #endif

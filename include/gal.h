@@ -7,10 +7,10 @@
  * Differences from the reference header:
  *  - pdist2() and hat() are `static inline` (src/gal.h:10,19 declare plain
  *    `inline`, which links only when the compiler inlines every call);
- *  - gal_subbox()/gal_subsphere() (src/gal.h:48-53) are not provided: they
- *    belong to the sub-volume-with-randoms workflow (src/app/Psb.c, Pss.c),
- *    which this library does not cover.  gal_t.rand is still honoured by
- *    fft_p2g() and P()/Pl()/Pmu().
+ *  - gal_subbox()/gal_subsphere() draw their randoms from the generator
+ *    the reference gets from GSL (gsl_rng_mt19937 set up from the
+ *    environment: GSL_RNG_SEED, default seed 0 -> 4357), written in this
+ *    library, so that the same catalogs come out without linking GSL.
  */
 #ifndef GAL_H
 #define GAL_H
@@ -59,6 +59,14 @@ void gal_wrap(gal_t *self, double L);
 /* plane-parallel redshift-space shift along los, new catalog without
  * velocities or randoms (src/gal.c:63-77); normalises los in place */
 gal_t *gal_rsd(gal_t *self, double los[3], double aH);
+
+/* particles inside the periodic sub-box box = {x0, x1, y0, y1, z0, z1, L},
+ * V = its volume, with lround(Np_sub / alpha) uniform randoms in it as
+ * ->rand (src/gal.h:46-49, src/gal.c:78-118) */
+gal_t *gal_subbox(gal_t *self, double box[7], double alpha);
+/* the same for the periodic sub-sphere sphere = {x0, y0, z0, R, L}
+ * (src/gal.h:50-53, src/gal.c:121-158) */
+gal_t *gal_subsphere(gal_t *self, double sphere[5], double alpha);
 
 #ifdef __cplusplus
 }

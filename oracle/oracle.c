@@ -58,6 +58,129 @@ ogal_t *oracle_loadqpm_cubic(const char *file, double L) {
     return part;
 }
 
+/* MT19937 (Matsumoto and Nishimura 1998, 2002 seeding), the generator behind
+ * gsl_rng_default in src/gal.c:101-106,142-147: seed 0 means 4357, and
+ * gsl_rng_uniform is the 32-bit output over 2^32.  Written in the original
+ * paper's two-loop form. */
+typedef struct orc_mt {
+    unsigned long mt[624];
+    int mti;
+} orc_mt;
+
+static void orc_mt_seed(orc_mt *r, unsigned long s) {
+    if (s == 0) s = 4357;
+    r->mt[0] = s & 0xffffffffUL;
+    for (int i = 1; i < 624; ++i)
+        r->mt[i] = (1812433253UL * (r->mt[i - 1] ^ (r->mt[i - 1] >> 30)) + i) & 0xffffffffUL;
+    r->mti = 624;
+}
+
+static double orc_mt_uniform(orc_mt *r) {
+    static const unsigned long mag01[2] = {0x0UL, 0x9908b0dfUL};
+    unsigned long y;
+    if (r->mti >= 624) {
+        int kk;
+        for (kk = 0; kk < 624 - 397; ++kk) {
+            y = (r->mt[kk] & 0x80000000UL) | (r->mt[kk + 1] & 0x7fffffffUL);
+            r->mt[kk] = r->mt[kk + 397] ^ (y >> 1) ^ mag01[y & 0x1UL];
+        }
+        for (; kk < 623; ++kk) {
+            y = (r->mt[kk] & 0x80000000UL) | (r->mt[kk + 1] & 0x7fffffffUL);
+            r->mt[kk] = r->mt[kk + (397 - 624)] ^ (y >> 1) ^ mag01[y & 0x1UL];
+        }
+        y = (r->mt[623] & 0x80000000UL) | (r->mt[0] & 0x7fffffffUL);
+        r->mt[623] = r->mt[396] ^ (y >> 1) ^ mag01[y & 0x1UL];
+        r->mti = 0;
+    }
+    y = r->mt[r->mti++];
+    y ^= (y >> 11);
+    y ^= (y << 7) & 0x9d2c5680UL;
+    y ^= (y << 15) & 0xefc60000UL;
+    y ^= (y >> 18);
+    return (double)(y & 0xffffffffUL) / 4294967296.0;
+}
+
+static unsigned long orc_env_seed(void) {
+    const char *e = getenv("GSL_RNG_SEED");
+    return e ? strtoul(e, NULL, 0) : 0;
+}
+
+static double orc_pdist2(double x1, double y1, double z1, double x0, double y0, double z0,
+                         double L) {
+    double dx = remainder(x1 - x0, L), dy = remainder(y1 - y0, L), dz = remainder(z1 - z0, L);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+/* src/gal.c:78-118 */
+ogal_t *oracle_gal_subbox(const ogal_t *g, double box[7], double alpha) {
+    static orc_mt rng;
+    static int seeded = 0;
+    double x0 = box[0], x1 = box[1], y0 = box[2], y1 = box[3], z0 = box[4], z1 = box[5];
+    double L = box[6];
+    x1 = x0 + remainder(x1 - x0, L);
+    y1 = y0 + remainder(y1 - y0, L);
+    z1 = z0 + remainder(z1 - z0, L);
+    double Vsub = (x1 - x0) * (y1 - y0) * (z1 - z0);
+    ogal_t *sub = oracle_gal_init(g->Np, Vsub);
+    long n = 0;
+    for (long p = 0; p < g->Np; ++p)
+        if (remainder(g->x[p] - x0, L) >= 0 && remainder(g->x[p] - x1, L) < 0 &&
+            remainder(g->y[p] - y0, L) >= 0 && remainder(g->y[p] - y1, L) < 0 &&
+            remainder(g->z[p] - z0, L) >= 0 && remainder(g->z[p] - z1, L) < 0) {
+            sub->x[n] = g->x[p];
+            sub->y[n] = g->y[p];
+            sub->z[n] = g->z[p];
+            ++n;
+        }
+    sub->Np = n;
+    long nr = lround(n / alpha);
+    ogal_t *rand = oracle_gal_init(nr, Vsub);
+    if (!seeded) {
+        orc_mt_seed(&rng, orc_env_seed());
+        seeded = 1;
+    }
+    for (long p = 0; p < nr; ++p) {
+        rand->x[p] = x0 + (x1 - x0) * orc_mt_uniform(&rng);
+        rand->y[p] = y0 + (y1 - y0) * orc_mt_uniform(&rng);
+        rand->z[p] = z0 + (z1 - z0) * orc_mt_uniform(&rng);
+    }
+    sub->rand = rand;
+    return sub;
+}
+
+/* src/gal.c:121-158 */
+ogal_t *oracle_gal_subsphere(const ogal_t *g, double sphere[5], double alpha) {
+    static orc_mt rng;
+    static int seeded = 0;
+    double x0 = sphere[0], y0 = sphere[1], z0 = sphere[2], R = sphere[3], L = sphere[4];
+    double R2 = R * R, Vsub = 4 * M_PI / 3 * R2 * R;
+    ogal_t *sub = oracle_gal_init(g->Np, Vsub);
+    long n = 0;
+    for (long p = 0; p < g->Np; ++p)
+        if (orc_pdist2(g->x[p], g->y[p], g->z[p], x0, y0, z0, L) < R2) {
+            sub->x[n] = g->x[p];
+            sub->y[n] = g->y[p];
+            sub->z[n] = g->z[p];
+            ++n;
+        }
+    sub->Np = n;
+    long nr = lround(n / alpha);
+    ogal_t *rand = oracle_gal_init(nr, Vsub);
+    if (!seeded) {
+        orc_mt_seed(&rng, orc_env_seed());
+        seeded = 1;
+    }
+    for (long p = 0; p < nr; ++p) {
+        do {
+            rand->x[p] = x0 + R * (2 * orc_mt_uniform(&rng) - 1);
+            rand->y[p] = y0 + R * (2 * orc_mt_uniform(&rng) - 1);
+            rand->z[p] = z0 + R * (2 * orc_mt_uniform(&rng) - 1);
+        } while (orc_pdist2(rand->x[p], rand->y[p], rand->z[p], x0, y0, z0, L) >= R2);
+    }
+    sub->rand = rand;
+    return sub;
+}
+
 /* src/gal.h:19-26 */
 void oracle_hat(double vec[3]) {
     double amp = sqrt(vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2]);
@@ -499,6 +622,36 @@ void oracle_dl_filter(offt_t *g, int kind, int Nsub, const double los[3], int el
     free(Wamp);
     free(Wph);
     free(Kval);
+}
+
+/* src/app/Psb.c:73-85 (kind 0) and src/app/Pss.c:70-87 (kind 1) */
+void oracle_subvolume_deconv(offt_t *g, int kind, int Nsub) {
+    const int Ng = g->Ng, Ngsub = Ng / Nsub;
+    oracle_fft_k2x(g, 0);
+    if (kind == 0) {
+        double Ngsub3 = (double)Ngsub * Ngsub * Ngsub;
+        for (int i = -Ngsub + 1; i < Ngsub; ++i)
+            for (int j = -Ngsub + 1; j < Ngsub; ++j)
+                for (int k = -Ngsub + 1; k < Ngsub; ++k) {
+                    double Winv = Ngsub3 / ((double)(Ngsub - abs(i)) * (Ngsub - abs(j)) *
+                                            (Ngsub - abs(k)));
+                    g->f[FIDX(g, (i + Ng) % Ng, (j + Ng) % Ng, (k + Ng) % Ng)] *= Winv;
+                }
+    } else {
+        double D2 = 3.0 * Ngsub * Ngsub;
+        int Ngbnd = (int)ceil(sqrt(D2));
+        for (int i = -Ngbnd + 1; i < Ngbnd; ++i)
+            for (int j = -Ngbnd + 1; j < Ngbnd; ++j)
+                for (int k = -Ngbnd + 1; k < Ngbnd; ++k) {
+                    double d2 = (double)i * i + (double)j * j + (double)k * k;
+                    if (d2 < 0.99 * D2) {
+                        double dD = sqrt(d2 / D2);
+                        double Winv = 1 / ((1 - dD) * (1 - dD) * (1 + 0.5 * dD));
+                        g->f[FIDX(g, (i + Ng) % Ng, (j + Ng) % Ng, (k + Ng) % Ng)] *= Winv;
+                    }
+                }
+    }
+    oracle_fft_x2k(g, 0);
 }
 
 /* src/power.c:78-99 */

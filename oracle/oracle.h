@@ -28,6 +28,11 @@
  *                       mean of e^{ik.x} over a sub box, brute force), the
  *                       top hat against its series at small KR, and
  *                       uniform-field and Legendre-plane closed forms
+ *   oracle_gal_subbox/_subsphere  selection against a numpy mask, randoms
+ *                       against numpy's MT19937 (legacy seeding) raw outputs
+ *   oracle_subvolume_deconv  window autocorrelations against brute-force
+ *                       overlap counts of a digitised box / quadrature of the
+ *                       sphere lens volume
  *   oracle_loadqpm_cubic  every value equals Python's float() of its text
  *                       field (correctly rounded), line count = '\n' count
  *                       (a last line without '\n' is not read), velocities
@@ -113,6 +118,9 @@ ogal_t *oracle_gal_rsd(const ogal_t *self, double los[3], double aH);
 ogal_t *oracle_loadqpm_cubic(const char *file, double L);
 void oracle_dl_filter(offt_t *g, int kind, int Nsub, const double los[3], int ell, double bias,
                       double *delta);
+ogal_t *oracle_gal_subbox(const ogal_t *g, double box[7], double alpha);
+ogal_t *oracle_gal_subsphere(const ogal_t *g, double sphere[5], double alpha);
+void oracle_subvolume_deconv(offt_t *g, int kind, int Nsub);
 
 #ifdef __cplusplus
 }

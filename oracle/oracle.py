@@ -75,6 +75,9 @@ def load():
         "oracle_gal_rsd": (P, [P, dptr, C.c_double]),
         "oracle_loadqpm_cubic": (P, [C.c_char_p, C.c_double]),
         "oracle_dl_filter": (None, [G, C.c_int, C.c_int, dptr, C.c_int, C.c_double, dptr]),
+        "oracle_gal_subbox": (P, [P, dptr, C.c_double]),
+        "oracle_gal_subsphere": (P, [P, dptr, C.c_double]),
+        "oracle_subvolume_deconv": (None, [G, C.c_int, C.c_int]),
         "offt64_r2c_3d": (None, [G]),
         "offt64_c2r_3d": (None, [G]),
     }

@@ -332,3 +332,55 @@ def test_dl_filter_vs_oracle(torch_cuda, lib, orc, kind, Nsub):
             assert np.abs(dg - do).max() <= 1e-12 * sc, (los, ell)
             assert np.abs(gg.f[:, :, :Ng] - go.f[:, :, :Ng]).max() <= \
                 1e-12 * np.abs(go.f[:, :, :Ng]).max()
+
+
+@pytest.mark.parametrize("kind,Nsub", [(0, 2), (1, 2), (1, 4)])
+def test_subvolume_flow_vs_oracle(torch_cuda, lib, orc, kind, Nsub):
+    """The sub-volume flow of src/app/Psb.c:54-96 and Pss.c:57-96 for one
+    sub-volume: gal_subbox / gal_subsphere with randoms, fft_p2k, P (with
+    alpha and the randoms' shot noise), the window deconvolution on the
+    device (fft_subvolume_deconv), Pl / Pmu bins; against the oracle:
+    identical catalogs, counts exact, sums to FP64 rounding, and the whole
+    deconvolved grid."""
+    import fftgal_amd as fa
+    Ng, L = 32, 200.0
+    rng = np.random.default_rng(23)
+    x, y, z = rng.random((3, 30000)) * L
+    co, cg = _catalogs(orc, fa, x, y, z, L ** 3)
+    dp = C.POINTER(C.c_double)
+    gs = L / Nsub
+    if kind == 0:
+        v = np.array([gs, 2 * gs, 0.0, gs, gs, 2 * gs, L])
+        so = _Ptr(orc.load().oracle_gal_subbox(co.ptr, v.ctypes.data_as(dp), 0.05))
+        sg = _Ptr(lib.gal_subbox(cg.ptr, v.ctypes.data_as(dp), 0.05))
+    else:
+        v = np.array([gs, 0.0, gs, gs * np.sqrt(3) / 2, L])
+        so = _Ptr(orc.load().oracle_gal_subsphere(co.ptr, v.ctypes.data_as(dp), 0.05))
+        sg = _Ptr(lib.gal_subsphere(cg.ptr, v.ctypes.data_as(dp), 0.05))
+    a, b = so.ptr.contents, sg.ptr.contents
+    assert a.Np == b.Np and a.rand.contents.Np == b.rand.contents.Np
+    n, nr = b.Np, b.rand.contents.Np
+    assert np.array_equal(np.ctypeslib.as_array(a.x, (n,)), np.ctypeslib.as_array(b.x, (n,)))
+    assert np.array_equal(np.ctypeslib.as_array(a.rand.contents.z, (nr,)),
+                          np.ctypeslib.as_array(b.rand.contents.z, (nr,)))
+    go, gg = orc.Grid(Ng, L), fa.Grid(Ng, L)
+    go.p2k(so)
+    gg.p2k(sg)
+    go.P(so)
+    gg.P(sg)
+    orc.load().oracle_subvolume_deconv(go.ptr, kind, Nsub)
+    lib.fft_subvolume_deconv(gg.ptr, kind, Nsub)
+    assert np.abs(gg.f - go.f).max() <= 1e-10 * np.abs(go.f).max()
+    dK = 2 * np.pi / L
+    for los in ((0, 0, 1), (0, 1, 1)):
+        Ko, No, Po = go.Pl_bins(los, dK)
+        Kg, Ng_, Pg = gg.Pl_bins(los, dK)
+        assert np.array_equal(No, Ng_)
+        ok = No > 0
+        assert np.abs(Pg[ok] - Po[ok]).max() <= 1e-9 * np.abs(Po[ok, 0]).max()
+        Ko, No, Po = go.Pmu_bins(los, dK, 3)
+        Kg, Ng_, Pg = gg.Pmu_bins(los, dK, 3)
+        assert np.array_equal(No, Ng_)
+        assert np.abs(Pg - Po).max() <= 1e-9 * np.abs(Po).max()
+    orc.load().oracle_gal_free(so.ptr)
+    lib.gal_free(sg.ptr)

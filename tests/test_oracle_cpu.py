@@ -324,3 +324,151 @@ def test_dl_plane_wave(orc, kind, ell):
     ref = Lell * np.real(W * np.exp(1j * corners))
     assert np.abs(got - ref).max() <= 1e-12, (np.abs(got - ref).max())
     g.free()
+
+
+_SUB_SCRIPT = r"""
+import sys, json, ctypes as C
+sys.path.insert(0, {root!r})
+import numpy as np
+import oracle, fftgal_amd as fa
+o = oracle.load()
+lib = fa.load()
+rng = np.random.default_rng(1)
+L = 100.0
+x, y, z = rng.random((3, 4000)) * L
+co = oracle.Catalog(x, y, z, L ** 3)
+cg = fa.Catalog(x, y, z, L ** 3)
+dp = C.POINTER(C.c_double)
+out = []
+for kind, vol, alpha in {calls!r}:
+    v = np.array(vol, dtype=np.float64)
+    if kind == "box":
+        a = o.oracle_gal_subbox(co.ptr, v.ctypes.data_as(dp), alpha)
+        b = lib.gal_subbox(cg.ptr, v.ctypes.data_as(dp), alpha)
+    else:
+        a = o.oracle_gal_subsphere(co.ptr, v.ctypes.data_as(dp), alpha)
+        b = lib.gal_subsphere(cg.ptr, v.ctypes.data_as(dp), alpha)
+    rec = {{}}
+    for tag, s in (("orc", a.contents), ("lib", b.contents)):
+        r = s.rand.contents
+        rec[tag] = {{"Np": s.Np, "V": s.V, "Nr": r.Np, "Vr": r.V,
+                    "x": [s.x[i] for i in range(s.Np)], "y": [s.y[i] for i in range(s.Np)],
+                    "z": [s.z[i] for i in range(s.Np)],
+                    "rx": [r.x[i] for i in range(r.Np)], "ry": [r.y[i] for i in range(r.Np)],
+                    "rz": [r.z[i] for i in range(r.Np)]}}
+    out.append(rec)
+print(json.dumps({{"x": list(x), "y": list(y), "z": list(z), "calls": out}}))
+"""
+
+
+def _run_sub(calls, seed=None):
+    import json
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ)
+    env.pop("GSL_RNG_SEED", None)
+    if seed is not None:
+        env["GSL_RNG_SEED"] = str(seed)
+    r = subprocess.run([sys.executable, "-c", _SUB_SCRIPT.format(root=root, calls=calls)],
+                       capture_output=True, text=True, env=env, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout)
+
+
+@pytest.mark.parametrize("seed", [None, 12345])
+def test_subbox_subsphere_randoms_and_selection(seed):
+    """gal_subbox / gal_subsphere (src/gal.c:78-158), library and oracle, in a
+    fresh process each (their generators live for the process): selections
+    against a numpy mask of the periodic sub-volume; randoms against numpy's
+    MT19937 with the legacy integer seeding (GSL's seeding: GSL_RNG_SEED,
+    0 -> 4357), u = output / 2^32, drawn x, y, z per random in sequence and
+    continuing across calls; library and oracle bit-identical."""
+    L = 100.0
+    calls = [("box", [0.0, 50.0, 50.0, 100.0, 80.0, 120.0, L], 0.05),
+             ("box", [10.0, 60.0, 0.0, 50.0, 0.0, 50.0, L], 0.1),
+             ("sphere", [5.0, 95.0, 50.0, 20.0, L], 0.05),
+             ("sphere", [50.0, 50.0, 50.0, 30.0, L], 0.08)]
+    res = _run_sub(calls, seed)
+    x, y, z = (np.array(res[k]) for k in "xyz")
+    gens = {"box": np.random.RandomState(seed or 4357)._bit_generator,
+            "sphere": np.random.RandomState(seed or 4357)._bit_generator}
+
+    def u(kind):
+        return float(gens[kind].random_raw()) / 4294967296.0
+
+    for (kind, vol, alpha), rec in zip(calls, res["calls"]):
+        assert rec["orc"] == rec["lib"], kind        # bit-identical, same order
+        r = rec["lib"]
+        if kind == "box":
+            x0, x1, y0, y1, z0, z1, _ = vol
+            x1, y1, z1 = (a0 + np.remainder(a1 - a0, L) if a1 - a0 != L else a1
+                          for a0, a1 in ((x0, x1), (y0, y1), (z0, z1)))
+            inside = ((np.mod(x - x0, L) < x1 - x0) & (np.mod(y - y0, L) < y1 - y0) &
+                      (np.mod(z - z0, L) < z1 - z0))
+            V = (x1 - x0) * (y1 - y0) * (z1 - z0)
+        else:
+            x0, y0, z0, R, _ = vol
+            d = [np.remainder(a - a0, L) for a, a0 in ((x, x0), (y, y0), (z, z0))]
+            d = [np.where(t >= L / 2, t - L, t) for t in d]
+            inside = d[0] ** 2 + d[1] ** 2 + d[2] ** 2 < R * R
+            V = 4 * np.pi / 3 * R ** 3
+        assert np.array_equal(np.array(r["x"]), x[inside])
+        assert np.array_equal(np.array(r["z"]), z[inside])
+        assert r["Nr"] == round(inside.sum() / alpha) and r["V"] == r["Vr"]
+        assert abs(r["V"] - V) <= 1e-12 * V
+        for i in range(r["Nr"]):
+            if kind == "box":
+                exp = (x0 + (x1 - x0) * u(kind), y0 + (y1 - y0) * u(kind),
+                       z0 + (z1 - z0) * u(kind))
+            else:
+                while True:
+                    exp = (x0 + R * (2 * u(kind) - 1), y0 + R * (2 * u(kind) - 1),
+                           z0 + R * (2 * u(kind) - 1))
+                    dd = [np.remainder(e - e0, L) for e, e0 in zip(exp, (x0, y0, z0))]
+                    dd = [t - L if t > L / 2 else t for t in dd]
+                    if dd[0] ** 2 + dd[1] ** 2 + dd[2] ** 2 < R * R:
+                        break
+            assert (r["rx"][i], r["ry"][i], r["rz"][i]) == exp, (kind, i)
+
+
+@pytest.mark.parametrize("kind,Ng,Nsub", [(0, 16, 2), (0, 24, 4), (1, 16, 4), (1, 24, 2)])
+def test_subvolume_deconv_window(orc, kind, Ng, Nsub):
+    """src/app/Psb.c:73-85 / Pss.c:70-87 on a field whose correlation
+    function is 1 everywhere: after the deconvolution each lag holds the
+    product of the inverse window autocorrelations of the lags that reach it
+    -- for the box, the overlap count of a digitised gs^3 box with itself
+    shifted by the lag (brute force); for the sphere, the lens volume of two
+    spheres of diameter D = sqrt(3) gs at the lag's distance over the sphere
+    volume, (4R + r)(2R - r)^2 / (16 R^3), for |lag|^2 < 0.99 D^2."""
+    L = 50.0
+    g = orc.Grid(Ng, L)
+    g.f[:, :, :Ng] = 1.0
+    g.x2k(0)
+    orc.load().oracle_subvolume_deconv(g.ptr, kind, Nsub)
+    g.k2x(0)
+    got = g.f[:, :, :Ng].copy()
+    gs = Ng // Nsub
+    exp = np.ones((Ng, Ng, Ng))
+    if kind == 0:
+        box = np.zeros((Ng, Ng, Ng), bool)
+        box[:gs, :gs, :gs] = True
+        for i in range(-gs + 1, gs):
+            for j in range(-gs + 1, gs):
+                for k in range(-gs + 1, gs):
+                    ov = (box & np.roll(box, (i, j, k), axis=(0, 1, 2))).sum()
+                    exp[i % Ng, j % Ng, k % Ng] *= gs ** 3 / ov
+    else:
+        D = np.sqrt(3.0) * gs
+        R = D / 2
+        b = int(np.ceil(D))
+        for i in range(-b + 1, b):
+            for j in range(-b + 1, b):
+                for k in range(-b + 1, b):
+                    r2 = float(i * i + j * j + k * k)
+                    if r2 < 0.99 * D * D:
+                        r = np.sqrt(r2)
+                        frac = (4 * R + r) * (2 * R - r) ** 2 / (16 * R ** 3)
+                        exp[i % Ng, j % Ng, k % Ng] /= frac
+    assert np.abs(got - exp).max() <= 1e-11 * np.abs(exp).max()
+    g.free()
