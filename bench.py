@@ -438,7 +438,9 @@ def main():
     ach = stage_bytes(dom, Np, Ng) / (dom_ms * 1e-3) / 1e9
     traffic = None
     tr = load_traffic()
-    if tr and dom in tr.get("stages", {}):
+    # the counter profile counts only for the workload it was taken on
+    wl = (tr or {}).get("workload", {})
+    if tr and dom in tr.get("stages", {}) and wl.get("Np") == Np and wl.get("Ng") == Ng:
         traffic = tr["stages"][dom].get("bytes")
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,

@@ -11,14 +11,15 @@
 //    src/fft.c:257-276).  Particles are first grouped by paint tile (T^3
 //    cells, by the plain grid's PCS base cell) with a two-level MSD
 //    counting sort -- coarse bucket (<= 256 per grid) then tile within the
-//    bucket (<= 128) -- whose scatters stage each 4096-particle chunk in LDS
+//    bucket (<= 128); grids with more tiles take 2048 x 1024 digits -- whose
+//    scatters stage each 4096-particle chunk in LDS
 //    ordered by digit, so that global writes are contiguous runs.  Ranks
 //    come from a wave-level match of equal digits (one LDS atomic per
 //    distinct digit per wave), so heavily clustered catalogs, where most
 //    particles share a few buckets, do not serialise on one LDS counter.
-//    Positions travel as 64-bit words: 7-bit tile-within-bucket digit and a
-//    19-bit tile-local fixed-point coordinate per axis (2^-15 cell for
-//    T = 16).  The paint then runs one workgroup per work item (<= cap
+//    Positions travel as 64-bit words: a 7-bit tile-within-bucket digit
+//    and a 19-bit tile-local fixed-point coordinate per axis (2^-15 cell
+//    for T = 16); the wide sort's 10-bit digit leaves 18 bits.  The paint then runs one workgroup per work item (<= cap
 //    particles of one tile): the 4x4x4 stencils of both grids go into LDS
 //    footprints ((T+3)^3 plain, (T+4)^3 dual cells, rows padded to a stride
 //    of 20 so that the 64 offsets of a stencil fall in 64 distinct banks) as
@@ -34,9 +35,11 @@
 //    below half a unit always round down: a particle paints on average about
 //    1 - 2^-(S+1) of its unit mass, a deficit the k-space kernel removes by
 //    normalising each grid by its painted total (its DC mode).
-//    Grids with more tiles than the two levels can address fall back to a
-//    single-level sort whose ranks come from global atomics.
+//    Grids with more tiles than even the wide two levels address (over
+//    2^21) fall back to a single-level sort whose ranks come from global
+//    atomics.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <math.h>
 #include <stdlib.h>
 #include "../fgk.h"
@@ -154,13 +157,18 @@ extern "C" int fgk_widen_f32_f64(double *dst, const float *src, long n, fgk_stre
 #define RB 32                            // sort chunks per scan block
 #define MAX_D1 256                       // coarse buckets
 #define MAX_D2 128                       // tiles per coarse bucket
+#define WIDE_D1 2048                     // the same for grids with more tiles
+#define WIDE_D2 1024
+// packed particle word: tile-within-bucket digit in the top bits, then
+// three PB-bit tile-local positions; PB = 19 with a 7-bit digit (two levels
+// of MAX_D1 x MAX_D2), 18 with a 10-bit digit (WIDE_D1 x WIDE_D2)
 #define POS_BITS 19
-#define POS_MASK ((1u << POS_BITS) - 1u)
-#define FINE_SHIFT 57
+#define POS_BITS_WIDE 18
 
 struct KeyGeo {
     double Hinv;
     int Ng, T, nt, F;
+    int pb, fs;          // position bits per axis, shift of the level-2 digit (3 pb)
     int cg0, cg1, cg2;   // tiles per coarse bucket per axis (powers of two)
     int nc0, nc1, nc2;   // coarse buckets per axis
     int lg0, lg1, lg2;   // log2 of cg0, cg1, cg2
@@ -170,6 +178,8 @@ struct KeyGeo {
 struct fgk_paintplan {
     int Ng, T, nt, interlace, F, S, cap;
     int two_level;
+    int wide;            // two levels of WIDE_D1 x WIDE_D2 digits
+    uint32_t *bitem, *ibase, *bscratch;   // [nb1+1] wide mode: items per bucket, their scan
     KeyGeo g;            // Hinv filled per call
     int nb1, nb2, bits1, bits2;
     long nslots;         // nb1 * nb2 tile slots
@@ -240,8 +250,8 @@ __device__ __forceinline__ void particle_key(float xv, float yv, float zv, const
     coarse = (uint32_t)((cx * g.nc1 + cy) * g.nc2 + cz);
     const uint32_t fine = (uint32_t)((((tx & (g.cg0 - 1)) << g.lg1 | (ty & (g.cg1 - 1))) << g.lg2) |
                                      (tz & (g.cg2 - 1)));
-    pk = ((uint64_t)fine << FINE_SHIFT) | ((uint64_t)qx << (2 * POS_BITS)) |
-         ((uint64_t)qy << POS_BITS) | (uint64_t)qz;
+    pk = ((uint64_t)fine << g.fs) | ((uint64_t)qx << (2 * g.pb)) | ((uint64_t)qy << g.pb) |
+         (uint64_t)qz;
 }
 
 // Rank of this lane among the valid lanes of its wave holding the same
@@ -270,10 +280,15 @@ __device__ __forceinline__ uint32_t wave_rank(uint32_t d, bool valid, uint32_t *
     return base + below;
 }
 
-// exclusive scan of hist[0..n) (n <= 256) into off[], by the first wave
+// exclusive scan of hist[0..n) (n <= NMAX <= 2048) into off[], by the first
+// wave.  NMAX <= 256: each lane holds 4 entries in registers; larger: each
+// lane sums a contiguous run of NMAX/64 entries, the wave scans the sums,
+// and each lane writes its run's offsets
+template <int NMAX>
 __device__ __forceinline__ void small_exclusive_scan(const uint32_t *hist, uint32_t *off, int n) {
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
+    if (threadIdx.x >= 64) return;
+    const int lane = threadIdx.x;
+    if constexpr (NMAX <= 256) {
         uint32_t v[4], s = 0;
         for (int q = 0; q < 4; ++q) {
             const int i = lane * 4 + q;
@@ -290,6 +305,22 @@ __device__ __forceinline__ void small_exclusive_scan(const uint32_t *hist, uint3
             const int i = lane * 4 + q;
             if (i < n) off[i] = run;
             run += v[q];
+        }
+    } else {
+        constexpr int PER = NMAX / 64;
+        const int i0 = lane * PER, i1 = i0 + PER < n ? i0 + PER : n;
+        uint32_t s = 0;
+        for (int i = i0; i < i1; ++i) s += hist[i];
+        uint32_t incl = s;
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        uint32_t run = incl - s;
+        for (int i = i0; i < i1; ++i) {
+            const uint32_t c = hist[i];
+            off[i] = run;
+            run += c;
         }
     }
 }
@@ -316,19 +347,21 @@ __device__ __forceinline__ bool chunk_range2(long j, const uint32_t *chunk0, con
 // Batched level 1 (jobs != nullptr): chunk j belongs to the job whose chunk
 // range jc1[job] <= j < jc1[job+1] holds it, whose positions and geometry
 // replace x, y, z, Np and g.
-template <int LEVEL, int COUNT>
+// NB: digit bits (8 / 7 for MAX_D1 / MAX_D2 digits, 11 / 10 for WIDE_D1 /
+// WIDE_D2); nd <= 2^NB digits in use.
+template <int LEVEL, int COUNT, int NB>
 __global__ void __launch_bounds__(SORT_THREADS)
 k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
        long Np, KeyGeo g, const uint64_t *__restrict__ in, const uint32_t *__restrict__ chunk0,
        const uint32_t *__restrict__ base1, const uint32_t *__restrict__ T1, int nb1, int nd,
        uint32_t *__restrict__ H, const uint32_t *__restrict__ O, uint64_t *__restrict__ out,
        const PaintJob *__restrict__ jobs, const long *__restrict__ jc1, int njobs) {
-    constexpr int NB = LEVEL == 1 ? 8 : 7;       // digit bits: MAX_D1, MAX_D2 digits
-    __shared__ uint32_t hist[MAX_D1];
-    __shared__ uint32_t off[MAX_D1];
-    __shared__ uint32_t dbase[COUNT ? 1 : MAX_D1];
+    constexpr int ND = 1 << NB;
+    using dig_t = typename std::conditional<(NB > 8), uint16_t, uint8_t>::type;
+    __shared__ uint32_t hist[ND];
+    __shared__ uint32_t off[ND];                 // staged offsets, then global bases
     __shared__ uint64_t stage[COUNT ? 1 : SORT_CHUNK];
-    __shared__ uint8_t sdig[COUNT ? 1 : SORT_CHUNK];
+    __shared__ dig_t sdig[COUNT ? 1 : SORT_CHUNK];
     __shared__ long s_start;
     __shared__ int s_n;
     const long j = blockIdx.x;
@@ -389,7 +422,7 @@ k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__
                 particle_key(xs[k], ys[k], zs[k], g, d, pk);
             } else {
                 pk = vk[k];
-                d = (uint32_t)(pk >> FINE_SHIFT);
+                d = (uint32_t)(pk >> g.fs);
             }
         }
         rk[k] = wave_rank<NB, !COUNT>(d, valid, hist);
@@ -401,22 +434,23 @@ k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__
         for (int i = threadIdx.x; i < nd; i += blockDim.x) H[j * nd + i] = hist[i];
         return;
     }
-    small_exclusive_scan(hist, off, nd);
+    small_exclusive_scan<ND>(hist, off, nd);
     __syncthreads();
-    // global destination of staged position pos with digit d: dbase[d] + pos
-    for (int i = threadIdx.x; i < nd; i += blockDim.x) dbase[i] = O[j * nd + i] - off[i];
 #pragma unroll
     for (int k = 0; k < SORT_PER_THREAD; ++k) {
         const int i = k * SORT_THREADS + threadIdx.x;
         if (i < n) {
             const uint32_t pos = off[dk[k]] + rk[k];
             stage[pos] = vk[k];
-            sdig[pos] = (uint8_t)dk[k];
+            sdig[pos] = (dig_t)dk[k];
         }
     }
     __syncthreads();
+    // global destination of staged position pos with digit d: off[d] + pos
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) off[i] = O[j * nd + i] - off[i];
+    __syncthreads();
     for (int pos = threadIdx.x; pos < n; pos += blockDim.x)
-        out[dbase[sdig[pos]] + pos] = stage[pos];
+        out[off[sdig[pos]] + pos] = stage[pos];
 }
 
 // S[b][d] = sum of H[j][d] over the RB chunks j of block b
@@ -547,6 +581,61 @@ __global__ void k_colscan(const uint32_t *__restrict__ H, const uint32_t *__rest
         O[j * nd + d] = run;
         run += H[j * nd + d];
     }
+}
+
+// Wide two-level sort, level-2 slots: per coarse bucket b (one workgroup),
+// exclusive scans of its tile counts T2[b*nd + s], s < nd, into tilebase
+// (from the bucket's first sorted position base1[b]) and paint items
+// (ceil(count / cap), from 0), and the bucket's item total into bitem[b]
+__global__ void __launch_bounds__(SCAN_THREADS)
+k_bucket_slots(const uint32_t *__restrict__ T2, const uint32_t *__restrict__ base1, int nb1,
+               int nd, int cap, uint32_t *__restrict__ tilebase, uint32_t *__restrict__ items,
+               uint32_t *__restrict__ bitem) {
+    __shared__ uint32_t sa[SCAN_THREADS], sb[SCAN_THREADS];
+    const int b = blockIdx.x;
+    const uint32_t *v = T2 + (long)b * nd;
+    uint32_t *base = tilebase + (long)b * nd, *aux = items + (long)b * nd;
+    const uint32_t b0 = base1[b];
+    const int per = (nd + SCAN_THREADS - 1) / SCAN_THREADS;
+    const int i0 = threadIdx.x * per, i1 = i0 + per < nd ? i0 + per : nd;
+    uint32_t ca = 0, cb = 0;
+    for (int i = i0; i < i1; ++i) {
+        const uint32_t c = v[i];
+        ca += c;
+        cb += (c + cap - 1) / cap;
+    }
+    sa[threadIdx.x] = ca;
+    sb[threadIdx.x] = cb;
+    __syncthreads();
+    for (int o = 1; o < SCAN_THREADS; o <<= 1) {
+        const uint32_t va = threadIdx.x >= o ? sa[threadIdx.x - o] : 0;
+        const uint32_t vb = threadIdx.x >= o ? sb[threadIdx.x - o] : 0;
+        __syncthreads();
+        sa[threadIdx.x] += va;
+        sb[threadIdx.x] += vb;
+        __syncthreads();
+    }
+    uint32_t ra = b0 + sa[threadIdx.x] - ca, rb = sb[threadIdx.x] - cb;
+    for (int i = i0; i < i1; ++i) {
+        base[i] = ra;
+        aux[i] = rb;
+        const uint32_t c = v[i];
+        ra += c;
+        rb += (c + cap - 1) / cap;
+    }
+    if (threadIdx.x == SCAN_THREADS - 1) {
+        bitem[b] = sb[SCAN_THREADS - 1];
+        if (b == nb1 - 1) tilebase[(long)nb1 * nd] = b0 + sa[SCAN_THREADS - 1];
+    }
+}
+
+// items[e] += ibase[e / nd] for the nb1 * nd slots; items[nb1 * nd] = total
+__global__ void k_add_item_base(uint32_t *__restrict__ items, const uint32_t *__restrict__ ibase,
+                                int nb1, int nd) {
+    const long n = (long)nb1 * nd;
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e < n) items[e] += ibase[e / nd];
+    if (e == n) items[n] = ibase[nb1];
 }
 
 // the single level-1 segment: blocks [0, nblk)
@@ -735,21 +824,22 @@ __device__ __forceinline__ void unpack_axis(uint32_t q, int F, float fsc, int &l
     d = (float)(q & ((1u << F) - 1u)) * fsc;
 }
 
+template <int PB>
 __device__ __forceinline__ uint32_t pk_axis(uint64_t pk, int a) {
-    return (uint32_t)(pk >> ((2 - a) * POS_BITS)) & POS_MASK;
+    return (uint32_t)(pk >> ((2 - a) * PB)) & ((1u << PB) - 1u);
 }
 
 // one grid's stencil of one packed particle word into an LDS footprint:
 // DUAL = 0: plain grid, nodes li-1..li+2, footprint origin tile origin - 1;
 // DUAL = 1: dual grid (offset 1/2), base li or li+1, origin tile origin - 2
-template <int P, int ROT, int DUAL>
+template <int P, int ROT, int DUAL, int PB>
 __device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, int F, float fsc,
                                            float wscale, int lane) {
     int li[3];
     float d[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        unpack_axis(pk_axis(pk, a), F, fsc, li[a], d[a]);
+        unpack_axis(pk_axis<PB>(pk, a), F, fsc, li[a], d[a]);
         if (DUAL) {
             if (d[a] >= 0.5f) { d[a] -= 0.5f; li[a] += 1; }
             else d[a] += 0.5f;
@@ -766,12 +856,12 @@ __device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, in
 
 // LDS word index mod 64 of a particle's stencil base in one footprint: the
 // bank of its first stencil offset
-template <int P, int DUAL>
+template <int P, int DUAL, int PB>
 __device__ __forceinline__ uint32_t stencil_bank(uint64_t pk, int F) {
     uint32_t li[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const uint32_t q = pk_axis(pk, a);
+        const uint32_t q = pk_axis<PB>(pk, a);
         li[a] = (q >> F) + (DUAL ? (q >> (F - 1)) & 1u : 0u);
     }
     return ((li[0] * P + li[1]) * P + li[2]) & 63u;
@@ -785,14 +875,14 @@ __device__ __forceinline__ uint32_t stencil_bank(uint64_t pk, int F) {
 // bank of their stencil base in one footprint, so that a wave's lanes
 // mostly share that bank; with the lanes' 64 rotated stencil orders their
 // atomics then hit 64 distinct banks at every step
-template <int P, int DUAL>
+template <int P, int DUAL, int PB>
 __device__ __forceinline__ void bank_sort(uint64_t pk, bool valid, int F, uint32_t *hist,
                                           uint32_t *off, uint64_t *stage) {
-    const uint32_t key = stencil_bank<P, DUAL>(pk, F);
+    const uint32_t key = stencil_bank<P, DUAL, PB>(pk, F);
     __syncthreads();                           // stage and hist free
     const uint32_t rank = wave_rank<6, true>(key, valid, hist);
     __syncthreads();
-    small_exclusive_scan(hist, off, 64);
+    small_exclusive_scan<64>(hist, off, 64);
     __syncthreads();
     if (valid) stage[off[key] + rank] = pk;
     if (threadIdx.x < 64) hist[threadIdx.x] = 0;
@@ -808,7 +898,7 @@ __device__ __forceinline__ void bank_sort(uint64_t pk, bool valid, int F, uint32
 // batch; the job owning it (ji0[job] <= item < ji0[job+1], ja <= job < jb)
 // supplies the geometry, grids and tile tables (tilebase, items + job * sl,
 // MAX_D2 slots per coarse bucket).
-template <int T>
+template <int T, int PB>
 __global__ void __launch_bounds__(PAINT_THREADS, 8)
 k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ tilebase,
               const uint32_t *__restrict__ items, long nslots, KeyGeo g, int nb2, float wscale,
@@ -854,17 +944,18 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
         const bool valid = p < p1;
         if (!PAINT_BANKSORT) {
             if (valid) {
-                paint_grid<P, PAINT_ROT, 0>(fa, pk, F, fsc, wscale, lane);
-                if (interlace) paint_grid<P, PAINT_ROT, 1>(fb, pk, F, fsc, wscale, lane);
+                paint_grid<P, PAINT_ROT, 0, PB>(fa, pk, F, fsc, wscale, lane);
+                if (interlace) paint_grid<P, PAINT_ROT, 1, PB>(fb, pk, F, fsc, wscale, lane);
             }
             continue;
         }
         const bool mine = threadIdx.x < p1 - c0;
-        bank_sort<P, 0>(pk, valid, F, hist, off, stage);
-        if (mine) paint_grid<P, PAINT_ROT, 0>(fa, stage[threadIdx.x], F, fsc, wscale, lane);
+        bank_sort<P, 0, PB>(pk, valid, F, hist, off, stage);
+        if (mine) paint_grid<P, PAINT_ROT, 0, PB>(fa, stage[threadIdx.x], F, fsc, wscale, lane);
         if (interlace) {
-            bank_sort<P, 1>(pk, valid, F, hist, off, stage);
-            if (mine) paint_grid<P, PAINT_ROT, 1>(fb, stage[threadIdx.x], F, fsc, wscale, lane);
+            bank_sort<P, 1, PB>(pk, valid, F, hist, off, stage);
+            if (mine)
+                paint_grid<P, PAINT_ROT, 1, PB>(fb, stage[threadIdx.x], F, fsc, wscale, lane);
         }
     }
     __syncthreads();
@@ -918,26 +1009,34 @@ static int fixed_bits(int cap) {
 }
 
 // Tile geometry of a grid of side Ng painted in tiles of side T: coarse
-// buckets group up to MAX_D2 tiles, doubling the group along the axis with
-// the most buckets, so that small grids get few, large buckets (few level-2
-// segments, long level-1 runs).  Two levels when at most MAX_D1 buckets
-// remain, else every tile is its own bucket (single level).
-static KeyGeo tile_geometry(int Ng, int T, int &two_level) {
+// buckets group up to d2 tiles, doubling the group along the axis with the
+// most buckets, so that small grids get few, large buckets (few level-2
+// segments, long level-1 runs).  mode 1: two levels when at most MAX_D1
+// buckets of MAX_D2 tiles cover the grid; mode 2 (allow_wide): two levels of
+// WIDE_D1 x WIDE_D2; mode 0: every tile is its own bucket (single level).
+static KeyGeo tile_geometry(int Ng, int T, int allow_wide, int &mode) {
     const int nt = (Ng + T - 1) / T;
     int cg[3] = {1, 1, 1}, nc[3] = {nt, nt, nt};
-    while (cg[0] * cg[1] * cg[2] < MAX_D2 && (nc[0] > 1 || nc[1] > 1 || nc[2] > 1)) {
-        int ax = 2;
-        if (nc[1] > nc[ax]) ax = 1;
-        if (nc[0] > nc[ax]) ax = 0;
-        cg[ax] *= 2;
-        nc[ax] = (nt + cg[ax] - 1) / cg[ax];
+    mode = 0;
+    for (int m = 1; m <= (allow_wide ? 2 : 1) && !mode; ++m) {
+        const int d1 = m == 1 ? MAX_D1 : WIDE_D1, d2 = m == 1 ? MAX_D2 : WIDE_D2;
+        cg[0] = cg[1] = cg[2] = 1;
+        nc[0] = nc[1] = nc[2] = nt;
+        while (cg[0] * cg[1] * cg[2] < d2 && (nc[0] > 1 || nc[1] > 1 || nc[2] > 1)) {
+            int ax = 2;
+            if (nc[1] > nc[ax]) ax = 1;
+            if (nc[0] > nc[ax]) ax = 0;
+            cg[ax] *= 2;
+            nc[ax] = (nt + cg[ax] - 1) / cg[ax];
+        }
+        if ((long)nc[0] * nc[1] * nc[2] <= d1) mode = m;
     }
-    two_level = (long)nc[0] * nc[1] * nc[2] <= MAX_D1;
-    if (!two_level) {
+    if (!mode) {
         cg[0] = cg[1] = cg[2] = 1;
         nc[0] = nc[1] = nc[2] = nt;
     }
-    KeyGeo g = {0.0,   Ng,    T,     nt,    POS_BITS - ceil_log2(T),
+    const int pb = mode == 2 ? POS_BITS_WIDE : POS_BITS;
+    KeyGeo g = {0.0,   Ng,    T,     nt,    pb - ceil_log2(T), pb, 3 * pb,
                 cg[0], cg[1], cg[2], nc[0], nc[1],
                 nc[2], ceil_log2(cg[0]), ceil_log2(cg[1]), ceil_log2(cg[2]),
                 ((1ull << 32) + (uint64_t)T - 1) / (uint64_t)T};
@@ -959,7 +1058,10 @@ extern "C" int fgk_paint_plan_create(fgk_paintplan **out, int Ng, int tile, long
     pp->cap = 8192;
     pp->S = fixed_bits(pp->cap);
     pp->Np_max = Np_max;
-    pp->g = tile_geometry(Ng, T, pp->two_level);
+    int mode = 0;
+    pp->g = tile_geometry(Ng, T, 1, mode);
+    pp->two_level = mode != 0;
+    pp->wide = mode == 2;
     pp->nt = pp->g.nt;
     pp->F = pp->g.F;
     if (!pp->two_level && (long)pp->nt * pp->nt * pp->nt >= (1l << 24)) {
@@ -999,6 +1101,11 @@ extern "C" int fgk_paint_plan_create(fgk_paintplan **out, int Ng, int tile, long
         rc |= fgk_malloc((void **)&pp->base1, (size_t)(pp->nb1 + 1) * u);
         rc |= fgk_malloc((void **)&pp->chunk0, (size_t)(pp->nb1 + 1) * u);
         rc |= fgk_malloc((void **)&pp->blk0, (size_t)(pp->nb1 + 1) * u);
+        if (pp->wide) {
+            rc |= fgk_malloc((void **)&pp->bitem, (size_t)(pp->nb1 + 1) * u);
+            rc |= fgk_malloc((void **)&pp->ibase, (size_t)(pp->nb1 + 1) * u);
+            rc |= fgk_malloc((void **)&pp->bscratch, (size_t)(pp->nb1 + 1) * u);
+        }
     } else {
         rc |= fgk_malloc((void **)&pp->rank32, (size_t)Np_max * u);
     }
@@ -1019,7 +1126,7 @@ extern "C" void fgk_paint_plan_destroy(fgk_paintplan *pp) {
     if (!pp) return;
     uint32_t *bufs[] = {pp->H1, pp->O1, pp->S1, pp->B1, pp->T1, pp->H2, pp->O2, pp->S2, pp->B2,
                         pp->seg1, pp->base1, pp->chunk0, pp->blk0, pp->T2, pp->tilebase,
-                        pp->items, pp->rank32};
+                        pp->items, pp->rank32, pp->bitem, pp->ibase, pp->bscratch};
     for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; ++i) fgk_free(bufs[i]);
     fgk_free(pp->sorted1);
     fgk_free(pp->sorted2);
@@ -1065,6 +1172,22 @@ static int level_scans(hipStream_t s, const uint32_t *H, long nw, int nd, uint32
     return 0;
 }
 
+// k_sort of one level over nw chunks with the plan's digit width (standard:
+// 8 / 7 bits, wide: 11 / 10 bits)
+#define SORT_LAUNCH(LEVEL, COUNT, nw, ...)                                                    \
+    do {                                                                                      \
+        constexpr int nbs_ = LEVEL == 1 ? 8 : 7, nbw_ = LEVEL == 1 ? 11 : 10;                 \
+        if (pp->wide)                                                                         \
+            hipLaunchKernelGGL((k_sort<LEVEL, COUNT, nbw_>), dim3((unsigned)(nw)),            \
+                               dim3(SORT_THREADS), 0, s, __VA_ARGS__,                         \
+                               (const PaintJob *)nullptr, (const long *)nullptr, 0);          \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_sort<LEVEL, COUNT, nbs_>), dim3((unsigned)(nw)),            \
+                               dim3(SORT_THREADS), 0, s, __VA_ARGS__,                         \
+                               (const PaintJob *)nullptr, (const long *)nullptr, 0);          \
+        FGK_LAUNCHED();                                                                       \
+    } while (0)
+
 extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, const float *y,
                                const float *z, long Np, double Hinv, int64_t *gridA,
                                int64_t *gridB, fgk_stream_t st) {
@@ -1108,12 +1231,10 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
     } else {
         switch (stage) {
         case 0:   // level 1 histograms
-            hipLaunchKernelGGL((k_sort<1, 1>), dim3((unsigned)nw1), dim3(SORT_THREADS), 0, s, x, y,
-                               z, Np, g, (const uint64_t *)nullptr, (const uint32_t *)nullptr,
-                               (const uint32_t *)nullptr, (const uint32_t *)nullptr, pp->nb1,
-                               pp->nb1, pp->H1, (const uint32_t *)nullptr,
-                               (uint64_t *)nullptr, (const PaintJob *)nullptr, (const long *)nullptr, 0);
-            FGK_LAUNCHED();
+            SORT_LAUNCH(1, 1, nw1, x, y, z, Np, g, (const uint64_t *)nullptr,
+                        (const uint32_t *)nullptr, (const uint32_t *)nullptr,
+                        (const uint32_t *)nullptr, pp->nb1, pp->nb1, pp->H1,
+                        (const uint32_t *)nullptr, (uint64_t *)nullptr);
             return 0;
         case 1:   // level 1 offsets and the level 2 chunk layout
             hipLaunchKernelGGL(k_set_seg1, dim3(1), dim3(64), 0, s, pp->seg1,
@@ -1123,36 +1244,55 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
                                pp->base1, 1, pp->cap, pp->chunk0, pp->blk0, pp->O1, pp->nb1, 1,
                                (long)pp->nb1);
         case 2:
-            hipLaunchKernelGGL((k_sort<1, 0>), dim3((unsigned)nw1), dim3(SORT_THREADS), 0, s, x, y,
-                               z, Np, g, (const uint64_t *)nullptr, (const uint32_t *)nullptr,
-                               (const uint32_t *)nullptr, (const uint32_t *)nullptr, pp->nb1,
-                               pp->nb1, (uint32_t *)nullptr, pp->O1, pp->sorted1, (const PaintJob *)nullptr, (const long *)nullptr, 0);
-            FGK_LAUNCHED();
+            SORT_LAUNCH(1, 0, nw1, x, y, z, Np, g, (const uint64_t *)nullptr,
+                        (const uint32_t *)nullptr, (const uint32_t *)nullptr,
+                        (const uint32_t *)nullptr, pp->nb1, pp->nb1, (uint32_t *)nullptr, pp->O1,
+                        pp->sorted1);
             return 0;
         case 3: { // level 2 histograms (every capacity chunk; empty ones write zeros)
             const long nw2 = nw1 + (long)pp->nb1 * RB;
-            hipLaunchKernelGGL((k_sort<2, 1>), dim3((unsigned)nw2), dim3(SORT_THREADS), 0, s,
-                               (const float *)nullptr, (const float *)nullptr,
-                               (const float *)nullptr, Np, g, pp->sorted1, pp->chunk0, pp->base1,
-                               pp->T1, pp->nb1, pp->nb2, pp->H2,
-                               (const uint32_t *)nullptr, (uint64_t *)nullptr, (const PaintJob *)nullptr, (const long *)nullptr, 0);
-            FGK_LAUNCHED();
+            SORT_LAUNCH(2, 1, nw2, (const float *)nullptr, (const float *)nullptr,
+                        (const float *)nullptr, Np, g, pp->sorted1, pp->chunk0, pp->base1, pp->T1,
+                        pp->nb1, pp->nb2, pp->H2, (const uint32_t *)nullptr, (uint64_t *)nullptr);
             return 0;
         }
         case 4: {
             const long nw2 = nw1 + (long)pp->nb1 * RB;
-            return level_scans(s, pp->H2, nw2, pp->nb2, pp->S2, pp->B2, pp->blk0, pp->nb1, pp->T2,
-                               pp->tilebase, 2, pp->cap, pp->items, (uint32_t *)nullptr, pp->O2,
-                               pp->nslots, pp->nb1, 0l);
+            if (!pp->wide)
+                return level_scans(s, pp->H2, nw2, pp->nb2, pp->S2, pp->B2, pp->blk0, pp->nb1,
+                                   pp->T2, pp->tilebase, 2, pp->cap, pp->items,
+                                   (uint32_t *)nullptr, pp->O2, pp->nslots, pp->nb1, 0l);
+            // wide: the tile scan runs per bucket in parallel, then the
+            // buckets' item bases are added
+            const long nblk = nw2 / RB;
+            hipLaunchKernelGGL(k_blocksum, dim3(blocks_for(nblk * pp->nb2, 256)), dim3(256), 0, s,
+                               pp->H2, nblk, pp->nb2, pp->S2);
+            FGK_LAUNCHED();
+            hipLaunchKernelGGL(k_segscan, dim3(blocks_for((long)pp->nb1 * pp->nb2, 256)),
+                               dim3(256), 0, s, pp->S2, pp->blk0, pp->nb1, pp->nb2, pp->B2, pp->T2);
+            FGK_LAUNCHED();
+            hipLaunchKernelGGL(k_bucket_slots, dim3((unsigned)pp->nb1), dim3(SCAN_THREADS), 0, s,
+                               pp->T2, pp->base1, pp->nb1, pp->nb2, pp->cap, pp->tilebase,
+                               pp->items, pp->bitem);
+            FGK_LAUNCHED();
+            hipLaunchKernelGGL(k_digitscan, dim3(1), dim3(SCAN_THREADS), 0, s, pp->bitem,
+                               (long)pp->nb1, 2, pp->cap, pp->ibase, pp->bscratch,
+                               (uint32_t *)nullptr);
+            FGK_LAUNCHED();
+            hipLaunchKernelGGL(k_add_item_base, dim3(blocks_for(pp->nslots + 1, 256)), dim3(256), 0,
+                               s, pp->items, pp->ibase, pp->nb1, pp->nb2);
+            FGK_LAUNCHED();
+            hipLaunchKernelGGL(k_colscan, dim3(blocks_for(nblk * pp->nb2, 256)), dim3(256), 0, s,
+                               pp->H2, pp->blk0, pp->nb1, nblk, pp->nb2, pp->tilebase, pp->B2,
+                               pp->O2, pp->nb1, 0l);
+            FGK_LAUNCHED();
+            return 0;
         }
         case 5: {
             const long nw2 = nw1 + (long)pp->nb1 * RB;
-            hipLaunchKernelGGL((k_sort<2, 0>), dim3((unsigned)nw2), dim3(SORT_THREADS), 0, s,
-                               (const float *)nullptr, (const float *)nullptr,
-                               (const float *)nullptr, Np, g, pp->sorted1, pp->chunk0, pp->base1,
-                               pp->T1, pp->nb1, pp->nb2, (uint32_t *)nullptr, pp->O2,
-                               pp->sorted2, (const PaintJob *)nullptr, (const long *)nullptr, 0);
-            FGK_LAUNCHED();
+            SORT_LAUNCH(2, 0, nw2, (const float *)nullptr, (const float *)nullptr,
+                        (const float *)nullptr, Np, g, pp->sorted1, pp->chunk0, pp->base1, pp->T1,
+                        pp->nb1, pp->nb2, (uint32_t *)nullptr, pp->O2, pp->sorted2);
             return 0;
         }
         default:
@@ -1166,9 +1306,18 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
         const float ws = (float)ldexp(1.0, pp->S);
         unsigned long long *A = (unsigned long long *)gridA, *B = (unsigned long long *)gridB;
 #define PAINT_T(TT)                                                                           \
-    hipLaunchKernelGGL(k_paint_tiles<TT>, grid, block, lds, s, pp->sorted2, pp->tilebase,     \
-                       pp->items, pp->nslots, g, pp->nb2, ws, pp->cap, pp->interlace, A, B,    \
-                       (const PaintJob *)nullptr, (const long *)nullptr, 0, 0, 0l)
+    do {                                                                                      \
+        if (pp->wide)                                                                         \
+            hipLaunchKernelGGL((k_paint_tiles<TT, POS_BITS_WIDE>), grid, block, lds, s,       \
+                               pp->sorted2, pp->tilebase, pp->items, pp->nslots, g, pp->nb2,  \
+                               ws, pp->cap, pp->interlace, A, B, (const PaintJob *)nullptr,   \
+                               (const long *)nullptr, 0, 0, 0l);                              \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_paint_tiles<TT, POS_BITS>), grid, block, lds, s,            \
+                               pp->sorted2, pp->tilebase, pp->items, pp->nslots, g, pp->nb2,  \
+                               ws, pp->cap, pp->interlace, A, B, (const PaintJob *)nullptr,   \
+                               (const long *)nullptr, 0, 0, 0l);                              \
+    } while (0)
         switch (pp->T) {
         case 16: PAINT_T(16); break;
         case 12: PAINT_T(12); break;
@@ -1344,7 +1493,7 @@ extern "C" int fgk_paint_batch_create(fgk_paintbatch **out, int njobs,
         const fgk_paintjob_desc &d = jobs[pb->perm[q]];
         const int T = paint_side(d.Ng, d.tile);
         int two_level = 0;
-        KeyGeo g = tile_geometry(d.Ng, T, two_level);
+        KeyGeo g = tile_geometry(d.Ng, T, 0, two_level);
         if (!two_level) {
             fgk_set_error("paint batch: catalog %d, Ng=%d with tiles of side %d has more tiles "
                           "than a batch addresses (%d x %d); paint it with its own plan",
@@ -1477,7 +1626,7 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
     const KeyGeo g0 = pb->hjobs[0].g;   // unused by batched launches
     switch (stage) {
     case 0:
-        hipLaunchKernelGGL((k_sort<1, 1>), dim3((unsigned)pb->NW1), dim3(SORT_THREADS), 0, s,
+        hipLaunchKernelGGL((k_sort<1, 1, 8>), dim3((unsigned)pb->NW1), dim3(SORT_THREADS), 0, s,
                            (const float *)nullptr, (const float *)nullptr,
                            (const float *)nullptr, 0l, g0, (const uint64_t *)nullptr,
                            (const uint32_t *)nullptr, (const uint32_t *)nullptr,
@@ -1504,7 +1653,7 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
         return 0;
     }
     case 2:
-        hipLaunchKernelGGL((k_sort<1, 0>), dim3((unsigned)pb->NW1), dim3(SORT_THREADS), 0, s,
+        hipLaunchKernelGGL((k_sort<1, 0, 8>), dim3((unsigned)pb->NW1), dim3(SORT_THREADS), 0, s,
                            (const float *)nullptr, (const float *)nullptr,
                            (const float *)nullptr, 0l, g0, (const uint64_t *)nullptr,
                            (const uint32_t *)nullptr, (const uint32_t *)nullptr,
@@ -1513,7 +1662,7 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
         FGK_LAUNCHED();
         return 0;
     case 3:
-        hipLaunchKernelGGL((k_sort<2, 1>), dim3((unsigned)pb->NW2), dim3(SORT_THREADS), 0, s,
+        hipLaunchKernelGGL((k_sort<2, 1, 7>), dim3((unsigned)pb->NW2), dim3(SORT_THREADS), 0, s,
                            (const float *)nullptr, (const float *)nullptr,
                            (const float *)nullptr, 0l, g0, pb->sorted1, pb->chunk0, pb->base1,
                            pb->T1, (int)nseg2, MAX_D2, pb->H2, (const uint32_t *)nullptr,
@@ -1540,7 +1689,7 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
         return 0;
     }
     case 5:
-        hipLaunchKernelGGL((k_sort<2, 0>), dim3((unsigned)pb->NW2), dim3(SORT_THREADS), 0, s,
+        hipLaunchKernelGGL((k_sort<2, 0, 7>), dim3((unsigned)pb->NW2), dim3(SORT_THREADS), 0, s,
                            (const float *)nullptr, (const float *)nullptr,
                            (const float *)nullptr, 0l, g0, pb->sorted1, pb->chunk0, pb->base1,
                            pb->T1, (int)nseg2, MAX_D2, (uint32_t *)nullptr, pb->O2, pb->sorted2,
@@ -1556,7 +1705,8 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
             const size_t lds = (size_t)paint_lds_bytes(pb->side[i], pb->interlace_any);
             const dim3 grid((unsigned)nitems), block(PAINT_THREADS);
 #define PAINT_TB(TT)                                                                          \
-    hipLaunchKernelGGL(k_paint_tiles<TT>, grid, block, lds, s, pb->sorted2, pb->tilebase,     \
+    hipLaunchKernelGGL((k_paint_tiles<TT, POS_BITS>), grid, block, lds, s, pb->sorted2,      \
+                       pb->tilebase,                                                          \
                        pb->items, 0l, g0, MAX_D2, ws, pb->cap, 0,                             \
                        (unsigned long long *)nullptr, (unsigned long long *)nullptr,          \
                        (const PaintJob *)pb->djobs, (const long *)pb->di0, ja, jb, BATCH_SL)

@@ -48,12 +48,15 @@ def test_fixed_point_paint_vs_oracle(torch_cuda, lib, orc, Ng, tile):
     g.p2g(c, (0.5, 0.5, 0.5))
     rb = g.f[:, :, :Ng] / unit
     # Stated tolerance per stencil term: weight rounding 2^-(S+1), the packed
-    # tile-local position (F = 19 - log2(T) fraction bits, so at most
+    # tile-local position (F = PB - log2(T) fraction bits, PB = 19, or 18 for
+    # grids of more than 32^3 tiles, which take the wide sort; so at most
     # 2^-(F+1) cell away, moving a weight product by <= 0.67 * 2^-(F+1)),
     # and FP32 weight evaluation (1e-7).  The oracle sees the same float32
     # positions.  A cell's bound is that times the number of particles
     # whose 4x4x4 stencil covers it (counted here with numpy).
-    F = 19 - int(np.ceil(np.log2(min(tile, Ng))))
+    T = min(tile, Ng)
+    nt = -(-Ng // T)
+    F = (19 if nt ** 3 <= 32768 else 18) - int(np.ceil(np.log2(T)))
     term = 2.0 ** -(S + 1) + 0.34 * 2.0 ** -F + 1e-7
     for grid, ref, off in ((a, ra, 0.0), (b, rb, 0.5)):
         cnt = np.zeros((Ng, Ng, Ng))
@@ -211,22 +214,24 @@ def test_config2_clustered_vs_oracle(torch_cuda, lib, orc):
     p.close()
 
 
-def test_single_level_sort_pipeline(torch_cuda, lib, orc):
-    """A grid with more paint tiles than the two-level sort addresses
-    (48^3 tiles of side 2 at Ng = 96) takes the single-level sort; its
-    bins against the FP64 oracle."""
+@pytest.mark.parametrize("Ng,tile", [(96, 2), (160, 1)])
+def test_large_tile_count_sorts(torch_cuda, lib, orc, Ng, tile):
+    """Grids with more paint tiles than 256 x 128: 48^3 tiles of side 2 at
+    Ng = 96 take the wide two-level sort (up to 2048 x 1024 digits), 160^3
+    tiles of side 1 at Ng = 160 (over 2^21) the single-level sort; bins
+    against the FP64 oracle."""
     import fftgal_amd as fa
     from fftgal_amd import mockgen
     torch = torch_cuda
-    Ng, Np, L = 96, 200000, 600.0
+    Np, L = 200000, 600.0
     (x, y, z), _ = mockgen.lognormal(Np, L, seed=11, Ngs=64)
     V = L ** 3
     dK = 2 * np.pi / L
     ref = _oracle_bins(orc, x, y, z, V, Ng, L, True, (0, 0, 1), dK, 5)
     tx, ty, tz = _dev(torch, x, y, z)
     p = fa.PowerPlan(Ng, L, interlace=True, dK=dK, los=(0, 0, 1), Nmu=5, Np_max=Np,
-                     paint_tile=2)
-    assert p.tuning()["paint_tile"] == 2
+                     paint_tile=tile)
+    assert p.tuning()["paint_tile"] == tile
     out = p.run(tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), Np, V)
     assert np.array_equal(out["N"], ref["N"])
     assert np.array_equal(out["Nmu"], ref["Nmu"])
