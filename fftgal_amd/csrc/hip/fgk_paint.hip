@@ -19,15 +19,16 @@
 //    particles share a few buckets, do not serialise on one LDS counter.
 //    Positions travel as 64-bit words: a 7-bit tile-within-bucket digit
 //    and a 19-bit tile-local fixed-point coordinate per axis (2^-15 cell
-//    for T = 16); the wide sort's 10-bit digit leaves 18 bits.  The paint then runs one workgroup per work item (<= cap
-//    particles of one tile): the 4x4x4 stencils of both grids go into LDS
-//    footprints ((T+3)^3 plain, (T+4)^3 dual cells, rows padded to a stride
-//    of 20 so that the 64 offsets of a stencil fall in 64 distinct banks) as
-//    int32 fixed point.  Each lane walks its stencil in one of 64 rotated
-//    orders (the lane's six low bits rotate a, b and c), so lanes holding
-//    particles of the same dense cell never hit the same address together.
-//    Non-zero footprint cells are flushed into the int64 grids with global
-//    integer atomics.
+//    for T = 16); the wide sort's 10-bit digit leaves 18 bits.  The paint
+//    then runs one workgroup per work item (<= cap particles of one tile):
+//    the 4x4x4 stencils of both grids go into LDS footprints ((T+3)^3
+//    plain, (T+4)^3 dual cells) as int32 fixed point.  The footprints are
+//    stored in 4x4x4 bricks, so a cell's LDS bank is fixed by its
+//    coordinates mod 4, and each lane walks its stencil in the residue
+//    order given by its six low bits: at every step the 64 lanes' atomics
+//    fall in 64 distinct banks (and so never on one address) whatever
+//    particles the wave holds.  Non-zero footprint cells are flushed into
+//    the int64 grids with global integer atomics.
 //    Integer addition is associative, so the grids are bit-identical run to
 //    run whatever the scheduling.  Each weight is rounded once to 2^-S, S
 //    chosen so that cap particles cannot overflow int32.  Round to nearest
@@ -685,10 +686,6 @@ __device__ __forceinline__ void pcs_w_f32(float d, float w[4]) {
     w[3] = d2 * d * (1.0f / 6.0f);
 }
 
-__device__ __forceinline__ float sel4(const float w[4], int a) {
-    return (a & 2) ? ((a & 1) ? w[3] : w[2]) : ((a & 1) ? w[1] : w[0]);
-}
-
 // o[q] = w[(q + r) & 3] as a two-level barrel shift with static indices
 // (a select on a lane-dependent index would be lowered to a scratch array)
 __device__ __forceinline__ void rot4(const float w[4], int r, float o[4]) {
@@ -707,16 +704,25 @@ __device__ __forceinline__ int fixed_rne(float a, float b) {
     return __float_as_int(__builtin_fmaf(a, b, 12582912.0f)) - 0x4B400000;
 }
 
-// Flush an LDS footprint of side^3 cells (cell (i,j,k) at (i*P + j)*P + k)
-// whose cell (0,0,0) sits at grid cell (oi, oj, ok) into the int64 grid
-// with one integer atomic per non-zero cell.  Each wave takes groups of
-// 64/side whole rows along k; a lane's k offset is fixed, so the periodic
-// wrap needs no division (footprints extend at most 2 cells below 0 and
-// side cells past the grid's last index).
-template <int SIDE, int P>
+// LDS footprints are stored in 4x4x4 bricks: cell (i,j,k) of a footprint
+// with NBK bricks per side sits at word
+//   (((i>>2)*NBK + (j>>2))*NBK + (k>>2))*64 + 16(i&3) + 4(j&3) + (k&3),
+// so its LDS bank (word mod 64) is 16(i&3) + 4(j&3) + (k&3): the 64 cells
+// of any 4x4x4 stencil lie in 64 distinct banks, wherever its base.
+constexpr int brick_n(int side) { return (side + 3) / 4; }
+constexpr int brick_words(int side) { return brick_n(side) * brick_n(side) * brick_n(side) * 64; }
+
+// Flush a bricked LDS footprint of side^3 cells whose cell (0,0,0) sits at
+// grid cell (oi, oj, ok) into the int64 grid with one integer atomic per
+// non-zero cell.  Each wave takes groups of 64/side whole rows along k (so
+// a wave's atomics cover runs of side consecutive grid cells); a lane's k
+// offset is fixed, so the periodic wrap needs no division (footprints
+// extend at most 2 cells below 0 and side cells past the grid's last index).
+template <int SIDE>
 __device__ void flush_footprint(const int *__restrict__ fp, int oi, int oj, int ok, int Ng,
                                 unsigned long long *__restrict__ G) {
     constexpr int RPG = 64 / SIDE;             // rows per wave step
+    constexpr int NBK = brick_n(SIDE);
     const int lane = __lane_id();
     const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const int ro = lane / SIDE, fk = lane - ro * SIDE;
@@ -729,9 +735,10 @@ __device__ void flush_footprint(const int *__restrict__ fp, int oi, int oj, int 
         return v;
     };
     const int gk = wrap(ok + fk);
+    const int wk = (fk >> 2) * 64 + (fk & 3);
     for (int r = wv * RPG + ro; r < SIDE * SIDE; r += nwv * RPG) {
         const int fi = r / SIDE, fj = r - fi * SIDE;
-        const int v = fp[(fi * P + fj) * P + fk];
+        const int v = fp[((fi >> 2) * NBK + (fj >> 2)) * NBK * 64 + (fi & 3) * 16 + (fj & 3) * 4 + wk];
         if (!v) continue;
         const int gi = wrap(oi + fi), gj = wrap(oj + fj);
         atomicAdd(G + ((long)gi * Ng + gj) * Ng + gk, (unsigned long long)(long long)v);
@@ -755,65 +762,45 @@ __device__ __forceinline__ bool item_range(long it, const uint32_t *__restrict__
     return true;
 }
 
-#ifndef PAINT_ROT
-#define PAINT_ROT 2
-#endif
+// Offsets of the 4 cells of one stencil axis, in the order a lane visits
+// them: step q goes to the cell of residue r = (q + rl) & 3 (rl: the lane's
+// 2 bits of this axis), i.e. the first cell >= base congruent to r, which
+// lies in the base's brick when r >= base & 3 and in the next one otherwise.
+// w[] is rotated to match: out[q] = w[(r - base) & 3].
+__device__ __forceinline__ void brick_axis(int base, int rl, int stride, int unit,
+                                           const float w[4], int off[4], float out[4]) {
+    const int b0 = (base >> 2) * stride, s = base & 3;
+    rot4(w, (rl - base) & 3, out);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = (q + rl) & 3;
+        off[q] = b0 + (r < s ? stride : 0) + r * unit;
+    }
+}
 
-// LDS row stride of the footprints: with P = 20 (12 for small tiles) the 64
-// stencil offsets a*P*P + b*P + c fall in 64 distinct LDS banks, as do the
-// cells of any 4x4x4 block
-constexpr int paint_stride(int T) { return T >= 12 ? 20 : 12; }
-
-// One particle's 4x4x4 stencil into an LDS footprint with strides P*P, P
-// (compile-time, so row offsets are immediates).  Lanes walk the stencil
-// in rotated orders so that lanes holding particles of one dense cell hit
-// different addresses: ROT 0 rotates a by two lane bits (groups of 16
-// lanes), ROT 1 rotates a and b by lane bits 0-1 and 2-3, ROT 2 also c by
-// bits 4-5 (64 orders; c offsets then come from registers).
-template <int P, int ROT>
+// One particle's 4x4x4 stencil with base (bi,bj,bk) into a bricked LDS
+// footprint of NBK bricks per side.  Lane l = 16ra + 4rb + rc adds at step
+// (aa,bb,cc) to the stencil cell with residues ((aa+ra)&3, (bb+rb)&3,
+// (cc+rc)&3): the bank 16((aa+ra)&3) + 4((bb+rb)&3) + ((cc+rc)&3) is a
+// bijection of the lane's bits, so every step's 64 atomics fall in 64
+// distinct banks, for any mix of particles in the wave.
+template <int NBK>
 __device__ __forceinline__ void stencil_lds(int *__restrict__ fp, int bi, int bj, int bk,
                                             const float wx[4], const float wy[4],
                                             const float wzs[4], int lane) {
-    int *base = fp + (bi * P + bj) * P + bk;
-    if (ROT == 0) {
-        const int ph = (lane >> 4) & 3;
-#pragma unroll
-        for (int aa = 0; aa < 4; ++aa) {
-            const int a = (aa + ph) & 3;
-            const float wa = sel4(wx, a);
-            int *pa = base + a * P * P;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const float wab = wa * wy[b];
-#pragma unroll
-                for (int c = 0; c < 4; ++c) atomicAdd(pa + b * P + c, fixed_rne(wab, wzs[c]));
-            }
-        }
-        return;
-    }
-    const int ra = lane & 3, rb = (lane >> 2) & 3, rc = (lane >> 4) & 3;
     float xr[4], yr[4], zr[4];
-    int oa[4], ob[4], oc[4];
-    rot4(wx, ra, xr);
-    rot4(wy, rb, yr);
-    if (ROT == 2) rot4(wzs, rc, zr);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        oa[q] = ((q + ra) & 3) * P * P;
-        ob[q] = ((q + rb) & 3) * P;
-        oc[q] = (q + rc) & 3;
-    }
+    int ox[4], oy[4], oz[4];
+    brick_axis(bi, (lane >> 4) & 3, NBK * NBK * 64, 16, wx, ox, xr);
+    brick_axis(bj, (lane >> 2) & 3, NBK * 64, 4, wy, oy, yr);
+    brick_axis(bk, lane & 3, 64, 1, wzs, oz, zr);
 #pragma unroll
     for (int aa = 0; aa < 4; ++aa) {
 #pragma unroll
         for (int bb = 0; bb < 4; ++bb) {
             const float wab = xr[aa] * yr[bb];
-            int *row = base + oa[aa] + ob[bb];
+            int *row = fp + ox[aa] + oy[bb];
 #pragma unroll
-            for (int cc = 0; cc < 4; ++cc) {
-                if (ROT == 2) atomicAdd(row + oc[cc], fixed_rne(wab, zr[cc]));
-                else atomicAdd(row + cc, fixed_rne(wab, wzs[cc]));
-            }
+            for (int cc = 0; cc < 4; ++cc) atomicAdd(row + oz[cc], fixed_rne(wab, zr[cc]));
         }
     }
 }
@@ -832,7 +819,7 @@ __device__ __forceinline__ uint32_t pk_axis(uint64_t pk, int a) {
 // one grid's stencil of one packed particle word into an LDS footprint:
 // DUAL = 0: plain grid, nodes li-1..li+2, footprint origin tile origin - 1;
 // DUAL = 1: dual grid (offset 1/2), base li or li+1, origin tile origin - 2
-template <int P, int ROT, int DUAL, int PB>
+template <int NBK, int DUAL, int PB>
 __device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, int F, float fsc,
                                            float wscale, int lane) {
     int li[3];
@@ -851,49 +838,13 @@ __device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, in
     pcs_w_f32(d[2], wz);
 #pragma unroll
     for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * wscale;
-    stencil_lds<P, ROT>(fp, li[0], li[1], li[2], wx, wy, wzs, lane);
+    stencil_lds<NBK>(fp, li[0], li[1], li[2], wx, wy, wzs, lane);
 }
 
-// LDS word index mod 64 of a particle's stencil base in one footprint: the
-// bank of its first stencil offset
-template <int P, int DUAL, int PB>
-__device__ __forceinline__ uint32_t stencil_bank(uint64_t pk, int F) {
-    uint32_t li[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const uint32_t q = pk_axis<PB>(pk, a);
-        li[a] = (q >> F) + (DUAL ? (q >> (F - 1)) & 1u : 0u);
-    }
-    return ((li[0] * P + li[1]) * P + li[2]) & 63u;
-}
-
-#ifndef PAINT_BANKSORT
-#define PAINT_BANKSORT 1
-#endif
-
-// stage[] = this chunk's particles (pk, valid per thread) ordered by the
-// bank of their stencil base in one footprint, so that a wave's lanes
-// mostly share that bank; with the lanes' 64 rotated stencil orders their
-// atomics then hit 64 distinct banks at every step
-template <int P, int DUAL, int PB>
-__device__ __forceinline__ void bank_sort(uint64_t pk, bool valid, int F, uint32_t *hist,
-                                          uint32_t *off, uint64_t *stage) {
-    const uint32_t key = stencil_bank<P, DUAL, PB>(pk, F);
-    __syncthreads();                           // stage and hist free
-    const uint32_t rank = wave_rank<6, true>(key, valid, hist);
-    __syncthreads();
-    small_exclusive_scan<64>(hist, off, 64);
-    __syncthreads();
-    if (valid) stage[off[key] + rank] = pk;
-    if (threadIdx.x < 64) hist[threadIdx.x] = 0;
-    __syncthreads();
-}
-
-// Paint of one work item (<= cap particles of one tile), in chunks of
-// PAINT_THREADS particles (coalesced loads).  Each chunk is painted into the
-// plain footprint, then the dual one, each time after a bank sort; the
-// stencils are accumulated as int32 fixed point, then the footprints are
-// flushed.
+// Paint of one work item (<= cap particles of one tile): each thread takes
+// particles PAINT_THREADS apart (coalesced loads) and adds its stencils to
+// the plain and the dual LDS footprint as int32 fixed point; the footprints
+// are then flushed.
 // Batched (jobs != nullptr): workgroup b paints item ji0[ja] + b of the
 // batch; the job owning it (ji0[job] <= item < ji0[job+1], ja <= job < jb)
 // supplies the geometry, grids and tile tables (tilebase, items + job * sl,
@@ -905,10 +856,9 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
               int cap, int interlace, unsigned long long *__restrict__ GA,
               unsigned long long *__restrict__ GB, const PaintJob *__restrict__ jobs,
               const long *__restrict__ ji0, int ja, int jb, long sl) {
-    constexpr int SA = T + 3, SB = T + 4, P = paint_stride(T);
-    constexpr int NA = SA * P * P, NB = SB * P * P;
+    constexpr int SA = T + 3, SB = T + 4;
+    constexpr int NA = brick_words(SA), NB = brick_words(SB);
     extern __shared__ int lds[];
-    __shared__ uint32_t hist[64], off[64];
     long it = blockIdx.x;
     if (jobs) {
         const long e = ji0[ja] + blockIdx.x;
@@ -928,35 +878,16 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
     if (!item_range(it, tilebase, items, nslots, cap, t, p0, p1)) return;
     const int nb = interlace ? NB : 0;
     int *fa = lds, *fb = lds + NA;
-    uint64_t *stage = (uint64_t *)(lds + NA + nb);             // [PAINT_THREADS]
     for (int i = threadIdx.x; i < NA + nb; i += blockDim.x) lds[i] = 0;
-    if (threadIdx.x < 64) hist[threadIdx.x] = 0;
     __syncthreads();
 
     const int F = g.F;
     const float fsc = 1.0f / (float)(1 << F);
     const int lane = __lane_id();
-    long p = p0 + threadIdx.x;
-    uint64_t nextpk = p < p1 ? sorted[p] : 0ull;
-    for (long c0 = p0; c0 < p1; c0 += PAINT_THREADS, p += PAINT_THREADS) {
-        const uint64_t pk = nextpk;
-        if (p + PAINT_THREADS < p1) nextpk = sorted[p + PAINT_THREADS];
-        const bool valid = p < p1;
-        if (!PAINT_BANKSORT) {
-            if (valid) {
-                paint_grid<P, PAINT_ROT, 0, PB>(fa, pk, F, fsc, wscale, lane);
-                if (interlace) paint_grid<P, PAINT_ROT, 1, PB>(fb, pk, F, fsc, wscale, lane);
-            }
-            continue;
-        }
-        const bool mine = threadIdx.x < p1 - c0;
-        bank_sort<P, 0, PB>(pk, valid, F, hist, off, stage);
-        if (mine) paint_grid<P, PAINT_ROT, 0, PB>(fa, stage[threadIdx.x], F, fsc, wscale, lane);
-        if (interlace) {
-            bank_sort<P, 1, PB>(pk, valid, F, hist, off, stage);
-            if (mine)
-                paint_grid<P, PAINT_ROT, 1, PB>(fb, stage[threadIdx.x], F, fsc, wscale, lane);
-        }
+    for (long p = p0 + threadIdx.x; p < p1; p += PAINT_THREADS) {
+        const uint64_t pk = sorted[p];
+        paint_grid<brick_n(SA), 0, PB>(fa, pk, F, fsc, wscale, lane);
+        if (interlace) paint_grid<brick_n(SB), 1, PB>(fb, pk, F, fsc, wscale, lane);
     }
     __syncthreads();
     const long b1 = t / nb2, f = t - b1 * nb2;
@@ -965,17 +896,15 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
     const int fx = (int)(f / (g.cg1 * g.cg2)), fy = (int)((f / g.cg2) % g.cg1),
               fz = (int)(f % g.cg2);
     const int ti = cx * g.cg0 + fx, tj = cy * g.cg1 + fy, tk = cz * g.cg2 + fz;
-    flush_footprint<SA, P>(fa, ti * T - 1, tj * T - 1, tk * T - 1, g.Ng, GA);
-    if (interlace) flush_footprint<SB, P>(fb, ti * T - 2, tj * T - 2, tk * T - 2, g.Ng, GB);
+    flush_footprint<SA>(fa, ti * T - 1, tj * T - 1, tk * T - 1, g.Ng, GA);
+    if (interlace) flush_footprint<SB>(fb, ti * T - 2, tj * T - 2, tk * T - 2, g.Ng, GB);
 }
 
 // ------------------------------------------------------------------ host
 
-// footprints and the staging of one chunk for the bank sort
+// the bricked LDS footprints of one work item
 static int paint_lds_bytes(int T, int interlace) {
-    const int P = paint_stride(T);
-    const int a = (T + 3) * P * P, b = interlace ? (T + 4) * P * P : 0;
-    return (a + b) * 4 + PAINT_THREADS * 8;
+    return (brick_words(T + 3) + (interlace ? brick_words(T + 4) : 0)) * 4;
 }
 
 static int ceil_log2(long v) {
