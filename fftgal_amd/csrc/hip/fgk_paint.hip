@@ -714,15 +714,19 @@ constexpr int brick_words(int side) { return brick_n(side) * brick_n(side) * bri
 
 // Flush a bricked LDS footprint of side^3 cells whose cell (0,0,0) sits at
 // grid cell (oi, oj, ok) into the int64 grid with one integer atomic per
-// non-zero cell.  Each wave takes groups of 64/side whole rows along k (so
-// a wave's atomics cover runs of side consecutive grid cells); a lane's k
-// offset is fixed, so the periodic wrap needs no division (footprints
-// extend at most 2 cells below 0 and side cells past the grid's last index).
+// non-zero cell.  (Plain stores of the cells no other tile reaches were
+// measured twice as slow: they share cache lines with the atomics.)  Each
+// wave takes groups of 64/side whole rows along k (so a wave covers runs of
+// side consecutive grid cells); a lane's k offset is fixed, so the periodic
+// wrap needs no division (footprints extend at most 2 cells below 0 and
+// side cells past the grid's last index).
 template <int SIDE>
 __device__ void flush_footprint(const int *__restrict__ fp, int oi, int oj, int ok, int Ng,
                                 unsigned long long *__restrict__ G) {
     constexpr int RPG = 64 / SIDE;             // rows per wave step
     constexpr int NBK = brick_n(SIDE);
+    typedef __attribute__((address_space(1))) unsigned long long gu64;
+    gu64 *g = (gu64 *)G;
     const int lane = __lane_id();
     const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const int ro = lane / SIDE, fk = lane - ro * SIDE;
@@ -741,7 +745,9 @@ __device__ void flush_footprint(const int *__restrict__ fp, int oi, int oj, int 
         const int v = fp[((fi >> 2) * NBK + (fj >> 2)) * NBK * 64 + (fi & 3) * 16 + (fj & 3) * 4 + wk];
         if (!v) continue;
         const int gi = wrap(oi + fi), gj = wrap(oj + fj);
-        atomicAdd(G + ((long)gi * Ng + gj) * Ng + gk, (unsigned long long)(long long)v);
+        __hip_atomic_fetch_add(g + ((long)gi * Ng + gj) * Ng + gk,
+                               (unsigned long long)(long long)v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 

@@ -22,15 +22,11 @@ def _rel_err(a, b, ref_scale):
     return np.abs(a - b) / ref_scale
 
 
-@pytest.mark.parametrize("Ng,tile", [(32, 16), (48, 16), (64, 8), (12, 16), (48, 1)])
-def test_fixed_point_paint_vs_oracle(torch_cuda, lib, orc, Ng, tile):
+def _paint_vs_oracle(torch, orc, x, y, z, L, Ng, tile):
+    """Fixed-point paint of float32 positions against the oracle's FP64
+    fft_p2g at offsets 0 and 1/2, within the stated per-cell bound; mass
+    deficit and run-to-run bit identity."""
     import fftgal_amd as fa
-    from fftgal_amd import mockgen
-    torch = torch_cuda
-    L = 400.0
-    (x, y, z), _ = mockgen.lognormal(40000, L, seed=Ng, Ngs=32)
-    # positions also outside [0, L): the paint wraps them
-    x = x - 0.3 * L
     tx, ty, tz = _dev(torch, x, y, z)
     p = fa.PowerPlan(Ng, L, interlace=True, Np_max=len(x), paint_tile=tile)
     A, B = p.paint_only(tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), len(x))
@@ -79,6 +75,33 @@ def test_fixed_point_paint_vs_oracle(torch_cuda, lib, orc, Ng, tile):
     A2, B2 = p.paint_only(tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), len(x))
     assert np.array_equal(A, A2) and np.array_equal(B, B2)
     p.close()
+
+
+@pytest.mark.parametrize("Ng,tile", [(32, 16), (48, 16), (64, 8), (12, 16), (48, 1), (40, 16)])
+def test_fixed_point_paint_vs_oracle(torch_cuda, lib, orc, Ng, tile):
+    """(40, 16): the tiles do not divide the grid, so the last tile per axis
+    wraps onto the first."""
+    from fftgal_amd import mockgen
+    L = 400.0
+    (x, y, z), _ = mockgen.lognormal(40000, L, seed=Ng, Ngs=32)
+    # positions also outside [0, L): the paint wraps them
+    x = x - 0.3 * L
+    _paint_vs_oracle(torch_cuda, orc, x, y, z, L, Ng, tile)
+
+
+def test_dense_tile_paint_vs_oracle(torch_cuda, lib, orc):
+    """One tile holds 30000 particles (more than one work item's 8192), the
+    rest of the box a thin uniform field: tiles painted by several work
+    items next to tiles painted by one."""
+    rng = np.random.default_rng(5)
+    L, Ng = 400.0, 64
+    H = L / Ng
+    clump = rng.uniform(20 * H, 23 * H, size=(30000, 3))
+    field = rng.uniform(0.0, L, size=(20000, 3))
+    pos = np.concatenate([clump, field]).astype(np.float32)
+    pos = pos[rng.permutation(len(pos))]
+    _paint_vs_oracle(torch_cuda, orc, pos[:, 0].copy(), pos[:, 1].copy(), pos[:, 2].copy(), L,
+                     Ng, 16)
 
 
 def _unpack_spectrum(c):
