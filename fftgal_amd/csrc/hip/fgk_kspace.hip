@@ -320,65 +320,163 @@ struct BinOut {
     unsigned long long *cnts;
 };
 
-// one mode of power P into the bins, given its count (1 on the k = 0 and
-// k = Ng/2 planes, else 2), |k| in units of KF (Kamp) and los . k (dot):
-// bin, mu wedge and Legendre weights as src/power.c:52-67 (Pl) and
-// 125-143 (Pmu) compute them, in FP64 without contraction.  SUMS adds the
-// power sums, a.geometry the K sums and mode counts.  win selects the LDS
-// window (0 or 1).
+// bin, wedge and sums of one mode of power P, given its count (1 on the
+// k = 0 and k = Ng/2 planes, else 2), |k| in units of KF (Kamp) and
+// los . k (dot), as src/power.c:52-67 (Pl) and 125-143 (Pmu) compute them,
+// in FP64 without contraction.  False when the mode lies past the last bin.
+struct ModeVals {
+    int iK, imu, count;
+    double pl[4];     // Legendre-weighted count * P (do_pl and SUMS)
+    double d2;        // count * P
+    double kc;        // count * |k|
+};
+
 template <bool SUMS>
-__device__ __forceinline__ void bin_mode(const BinArgs &a, const BinOut &o, int count,
-                                         double Kamp, double dot, double P, int win) {
+__device__ __forceinline__ bool mode_vals(const BinArgs &a, int count, double Kamp, double dot,
+                                          double P, ModeVals &m) {
 #pragma clang fp contract(off)
-    const int Wm = a.Nmu > 0 ? a.Nmu + 1 : 0;
-    const double d2 = P * count;
-    const int iK = (int)floor(Kamp * a.dKinv);
-    if (iK >= a.NK) return;   // cannot happen for NK of src/power.c:36; keeps LDS in bounds
+    m.count = count;
+    m.d2 = P * count;
+    m.iK = (int)floor(Kamp * a.dKinv);
+    if (m.iK >= a.NK) return false;   // cannot happen for NK of src/power.c:36; keeps LDS in bounds
     const double mu = dot / Kamp;
-    int imu = 0;
-    if (a.Nmu > 0) imu = (int)floor(fabs(mu * a.Nmu));
-    double pl[4];
+    m.imu = 0;
+    if (a.Nmu > 0) m.imu = (int)floor(fabs(mu * a.Nmu));
     if (a.do_pl && SUMS) {
         const double mu2 = mu * mu;
-        pl[0] = d2;
-        pl[1] = d2 * (1.5 * mu2 - 0.5);
-        pl[2] = d2 * ((4.375 * mu2 - 3.75) * mu2 + 0.375);
-        pl[3] = d2 * (((14.4375 * mu2 - 19.6875) * mu2 + 6.5625) * mu2 - 0.3125);
+        m.pl[0] = m.d2;
+        m.pl[1] = m.d2 * (1.5 * mu2 - 0.5);
+        m.pl[2] = m.d2 * ((4.375 * mu2 - 3.75) * mu2 + 0.375);
+        m.pl[3] = m.d2 * (((14.4375 * mu2 - 19.6875) * mu2 + 6.5625) * mu2 - 0.3125);
     }
+    m.kc = count * Kamp;
+    return true;
+}
+
+// LDS window row of bin iK (its counts in *crow), or nullptr when the bin
+// lies outside the window and goes straight to the global bins
+__device__ __forceinline__ double *window_row(const BinArgs &a, const BinOut &o, int iK, int win,
+                                              unsigned int **crow) {
     const int slot = win ? o.wn + iK - o.blo2 : iK - o.blo;
     if (o.windowed && slot >= (win ? o.wn : 0) && slot < (win ? o.wtot : o.wn)) {
-        double *v = o.sv + (size_t)slot * a.nvals;
-        int q = 0;
-        if (a.do_pl) {
-            if (SUMS)
-                for (int l = 0; l < 4; ++l) atomicAdd(v + l, pl[l]);
-            q = 4;
-        }
-        if (Wm && SUMS) atomicAdd(v + q + imu, d2);
+        *crow = o.sc + (size_t)slot * a.ncnt;
+        return o.sv + (size_t)slot * a.nvals;
+    }
+    return nullptr;
+}
+
+// the bin part of the sums of one mode or of a run of modes in bin iK:
+// Legendre power sums (SUMS), K sum and mode count (a.geometry).  LDS row
+// of a bin: [P_l x4 (do_pl)][wedge P x Wm][K][wedge K x Wm]; counts
+// [N][wedge N x Wm].  win selects the LDS window (0 or 1).
+template <bool SUMS>
+__device__ __forceinline__ void emit_bin(const BinArgs &a, const BinOut &o, int iK,
+                                         const double pl[4], double kc, int count, int win) {
+    const int Wm = a.Nmu > 0 ? a.Nmu + 1 : 0;
+    unsigned int *c;
+    double *v = window_row(a, o, iK, win, &c);
+    if (v) {
+        if (a.do_pl && SUMS)
+            for (int l = 0; l < 4; ++l) atomicAdd(v + l, pl[l]);
         if (a.geometry) {
-            const double kc = count * Kamp;
-            atomicAdd(v + q + Wm, kc);
-            if (Wm) atomicAdd(v + q + Wm + 1 + imu, kc);
-            unsigned int *c = o.sc + (size_t)slot * a.ncnt;
+            atomicAdd(v + (a.do_pl ? 4 : 0) + Wm, kc);
             atomicAdd(c, (unsigned)count);
-            if (Wm) atomicAdd(c + 1 + imu, (unsigned)count);
+        }
+        return;
+    }
+    const size_t offK = (size_t)a.NK * 4 + (size_t)a.NK * Wm;
+    if (a.do_pl && SUMS)
+        for (int l = 0; l < 4; ++l) atomicAdd(o.bins + (size_t)iK * 4 + l, pl[l]);
+    if (a.geometry) {
+        atomicAdd(o.bins + offK + iK, kc);
+        atomicAdd(o.cnts + iK, (unsigned long long)count);
+    }
+}
+
+// the wedge part: power sum (SUMS), K sum and count of wedge imu of bin iK
+template <bool SUMS>
+__device__ __forceinline__ void emit_wedge(const BinArgs &a, const BinOut &o, int iK, int imu,
+                                           double d2, double kc, int count, int win) {
+    const int Wm = a.Nmu > 0 ? a.Nmu + 1 : 0;
+    if (!Wm) return;
+    unsigned int *c;
+    double *v = window_row(a, o, iK, win, &c);
+    if (v) {
+        const int q = a.do_pl ? 4 : 0;
+        if (SUMS) atomicAdd(v + q + imu, d2);
+        if (a.geometry) {
+            atomicAdd(v + q + Wm + 1 + imu, kc);
+            atomicAdd(c + 1 + imu, (unsigned)count);
         }
         return;
     }
     const size_t offPmu = (size_t)a.NK * 4;
-    const size_t offK = offPmu + (size_t)a.NK * Wm;
-    const size_t offKmu = offK + (size_t)a.NK;
-    if (a.do_pl && SUMS)
-        for (int l = 0; l < 4; ++l) atomicAdd(o.bins + (size_t)iK * 4 + l, pl[l]);
-    if (Wm && SUMS) atomicAdd(o.bins + offPmu + (size_t)iK * Wm + imu, d2);
+    const size_t offKmu = offPmu + (size_t)a.NK * Wm + (size_t)a.NK;
+    if (SUMS) atomicAdd(o.bins + offPmu + (size_t)iK * Wm + imu, d2);
     if (a.geometry) {
-        const double kc = count * Kamp;
-        atomicAdd(o.bins + offK + iK, kc);
-        if (Wm) atomicAdd(o.bins + offKmu + (size_t)iK * Wm + imu, kc);
-        atomicAdd(o.cnts + iK, (unsigned long long)count);
-        if (Wm) atomicAdd(o.cnts + a.NK + (size_t)iK * Wm + imu, (unsigned long long)count);
+        atomicAdd(o.bins + offKmu + (size_t)iK * Wm + imu, kc);
+        atomicAdd(o.cnts + a.NK + (size_t)iK * Wm + imu, (unsigned long long)count);
     }
 }
+
+// one mode straight into the bins
+template <bool SUMS>
+__device__ __forceinline__ void bin_mode(const BinArgs &a, const BinOut &o, int count,
+                                         double Kamp, double dot, double P, int win) {
+    ModeVals m;
+    if (!mode_vals<SUMS>(a, count, Kamp, dot, P, m)) return;
+    emit_bin<SUMS>(a, o, m.iK, m.pl, m.kc, m.count, win);
+    emit_wedge<SUMS>(a, o, m.iK, m.imu, m.d2, m.kc, m.count, win);
+}
+
+// Sums of a thread's consecutive modes that fall in one bin (and one wedge)
+// are added in registers and go to the bins once per run: along a row of
+// the spectrum |k| grows with k, so a thread's consecutive modes mostly
+// share a bin, and a wave's lanes then mostly hold different bins (fewer
+// LDS atomics, and fewer of them on one address).
+template <bool SUMS>
+struct BinRun {
+    int iK = -1, imu = -1, cnt = 0, wcnt = 0;
+    double pl[4] = {0.0, 0.0, 0.0, 0.0}, kc = 0.0, wd2 = 0.0, wkc = 0.0;
+
+    __device__ __forceinline__ void flush(const BinArgs &a, const BinOut &o, int win) {
+        if (iK < 0) return;
+        emit_bin<SUMS>(a, o, iK, pl, kc, cnt, win);
+        emit_wedge<SUMS>(a, o, iK, imu, wd2, wkc, wcnt, win);
+    }
+
+    __device__ __forceinline__ void add(const BinArgs &a, const BinOut &o, const ModeVals &m,
+                                        int win) {
+#pragma clang fp contract(off)
+        if (m.iK != iK) {
+            flush(a, o, win);
+            iK = m.iK;
+            for (int l = 0; l < 4; ++l) pl[l] = m.pl[l];
+            kc = m.kc;
+            cnt = m.count;
+            imu = m.imu;
+            wd2 = m.d2;
+            wkc = m.kc;
+            wcnt = m.count;
+            return;
+        }
+        if (a.do_pl && SUMS)
+            for (int l = 0; l < 4; ++l) pl[l] += m.pl[l];
+        kc += m.kc;
+        cnt += m.count;
+        if (m.imu != imu) {
+            emit_wedge<SUMS>(a, o, iK, imu, wd2, wkc, wcnt, win);
+            imu = m.imu;
+            wd2 = m.d2;
+            wkc = m.kc;
+            wcnt = m.count;
+            return;
+        }
+        wd2 += m.d2;
+        wkc += m.kc;
+        wcnt += m.count;
+    }
+};
 
 // |k| of a mode in units of KF, sqrt(fi^2 + fj^2 + fk^2) as src/power.c:55
 __device__ __forceinline__ double kamp(int fi, int fj, int fk) {
@@ -560,39 +658,90 @@ k_bin(const double *__restrict__ fP, const float *__restrict__ A, const float *_
     const BinOut o = make_out(a, sh, blo, bhi, 0, -1, bins, cnts);
     window_clear(a, o);
 
-    const int k = k0 + threadIdx.x % BK;
-    const int fk = kval(k, Ng);
-    const int count = (k == 0 || 2 * k == Ng) ? 1 : 2;
-    KFac kf = {1.0, 0.0, 1.0};
+    // each thread takes KPT consecutive k of the rows it visits
+    constexpr int KPT = 4, TPR = BK / KPT;
+    const int kb = k0 + (threadIdx.x % TPR) * KPT;
+    int cnt[KPT], fk[KPT];
+    bool kl[KPT];
+    KFac kf[KPT];
     double na = norm, nb = norm;
     if (MODE == 1) grid_norms(A, B, norm, npart, na, nb);
-    if (MODE == 1 && k < k1) {
-        kf.pr = ph[2 * (2 * Ng + k)];
-        kf.pi = ph[2 * (2 * Ng + k) + 1];
-        kf.w = winv[k];
+#pragma unroll
+    for (int q = 0; q < KPT; ++q) {
+        const int k = kb + q;
+        fk[q] = kval(k, Ng);
+        cnt[q] = (k == 0 || 2 * k == Ng) ? 1 : 2;
+        kl[q] = k < k1 && !(MODE == 1 && k == 0);
+        kf[q] = {1.0, 0.0, 1.0};
+        if (MODE == 1 && k < k1) {
+            kf[q].pr = ph[2 * (2 * Ng + k)];
+            kf[q].pi = ph[2 * (2 * Ng + k) + 1];
+            kf[q].w = winv[k];
+        }
     }
-    const bool live = k < k1 && !(MODE == 1 && k == 0);
-    for (int r = threadIdx.x / BK; r < BI * BJ; r += BIN_THREADS / BK) {
+    // the packed spectra are read 4 modes (two float4) at a time when the
+    // rows keep that alignment
+    const bool vec = MODE == 1 && (Mh & 1) == 0 && kb + KPT <= k1;
+    BinRun<MODE != 2> run;
+    for (int r = threadIdx.x / TPR; r < BI * BJ; r += BIN_THREADS / TPR) {
         const int i = i0 + r / BJ, j = j0 + r % BJ;
-        if (!live || i >= i1 || j >= j1) continue;
-        if (i == 0 && j == 0 && k == 0) continue;
+        if (kb >= k1 || i >= i1 || j >= j1) continue;
         const int fi = kval(i, Ng), fj = kval(j, Ng);
-        double P = 0.0;
-        if (MODE == 0) {
-            P = fP[2 * (((long)i * Ng + j) * Nk + k)];
-        } else if (MODE == 1) {
-            const long m = ((long)i * Ng + j) * Mh + k;
-            const float2 av = ((const float2 *)A)[m];
-            float2 bv = make_float2(0.0f, 0.0f);
-            if (B) bv = ((const float2 *)B)[m];
+        float2 av[KPT], bv[KPT];
+        double qr = 1.0, qi = 0.0, wij = 1.0;
+        if (MODE == 1) {
+            const long m = ((long)i * Ng + j) * Mh + kb;
+            if (vec) {
+                const float4 *pa = (const float4 *)((const float2 *)A + m);
+                const float4 a0 = pa[0], a1 = pa[1];
+                av[0] = make_float2(a0.x, a0.y);
+                av[1] = make_float2(a0.z, a0.w);
+                av[2] = make_float2(a1.x, a1.y);
+                av[3] = make_float2(a1.z, a1.w);
+                if (B) {
+                    const float4 *pb = (const float4 *)((const float2 *)B + m);
+                    const float4 b0 = pb[0], b1 = pb[1];
+                    bv[0] = make_float2(b0.x, b0.y);
+                    bv[1] = make_float2(b0.z, b0.w);
+                    bv[2] = make_float2(b1.x, b1.y);
+                    bv[3] = make_float2(b1.z, b1.w);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < KPT; ++q) {
+                    av[q] = bv[q] = make_float2(0.0f, 0.0f);
+                    if (kb + q < k1) {
+                        av[q] = ((const float2 *)A)[m + q];
+                        if (B) bv[q] = ((const float2 *)B)[m + q];
+                    }
+                }
+            }
+            if (!B)
+                for (int q = 0; q < KPT; ++q) bv[q] = make_float2(0.0f, 0.0f);
             // (i, j) parts of the dual phase and the deconvolution
             const double *p0 = ph + 2 * i, *p1 = ph + 2 * (Ng + j);
-            const double qr = p0[0] * p1[0] - p0[1] * p1[1], qi = p0[0] * p1[1] + p0[1] * p1[0];
-            P = fused_power(av.x, av.y, bv.x, bv.y, B != nullptr, qr, qi, winv[i] * winv[j], kf,
-                            na, nb, Vinv, Pshot);
+            qr = p0[0] * p1[0] - p0[1] * p1[1];
+            qi = p0[0] * p1[1] + p0[1] * p1[0];
+            wij = winv[i] * winv[j];
         }
-        bin_mode<MODE != 2>(a, o, count, kamp(fi, fj, fk), los_dot(a, fi, fj, fk), P, 0);
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) {
+            if (!kl[q]) continue;
+            if (i == 0 && j == 0 && kb + q == 0) continue;
+            double P = 0.0;
+            if (MODE == 0) {
+                P = fP[2 * (((long)i * Ng + j) * Nk + kb + q)];
+            } else if (MODE == 1) {
+                P = fused_power(av[q].x, av[q].y, bv[q].x, bv[q].y, B != nullptr, qr, qi, wij,
+                                kf[q], na, nb, Vinv, Pshot);
+            }
+            ModeVals mv;
+            if (mode_vals<MODE != 2>(a, cnt[q], kamp(fi, fj, fk[q]), los_dot(a, fi, fj, fk[q]), P,
+                                     mv))
+                run.add(a, o, mv, 0);
+        }
     }
+    run.flush(a, o, 0);
     window_flush(a, o);
 }
 
