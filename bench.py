@@ -59,6 +59,10 @@ def parse():
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (default: tune each grid side on the workload "
                          "first, timed separately)")
+    ap.add_argument("--extras", action="store_true",
+                    help="time the off-path drop-in operations (x2k, k2x, randoms paint, "
+                         "super-sample filter, window deconvolution, QPM loaders) instead; "
+                         "one JSON line per operation")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed graph loop (for rocprofv3 runs)")
     a = ap.parse_args()
@@ -343,12 +347,120 @@ def main_batch(args):
     print(json.dumps(line))
 
 
+def main_extras(args):
+    """Times of the drop-in API's operations beside the power-spectrum path,
+    on an Ng = 512, L = 2000 grid kept on the device (fft_set_device_resident):
+    fft_x2k / fft_k2x (FP64 R2C / C2R with H^3, 1/L^3 and the offset phase),
+    fft_p2g of a catalog with randoms (FP64 atomics), fft_dl_filter (filter,
+    C2R, 4^3 samples) and fft_subvolume_deconv (C2R, window, R2C), and the
+    QPM loaders on a synthetic file of 2^21 lines.  Wall time per call with
+    the device synchronised; a grid operation's compulsory bytes are one
+    read and one write of the padded FP64 grid per pass over it."""
+    import ctypes as C
+    import tempfile
+    import numpy as np
+    import torch
+    import fftgal_amd as fa
+    lib = fa.load()
+    Ng, L = 512, 2000.0
+    Nc = Ng * Ng * 2 * (Ng // 2 + 1)
+    rw = 2 * 8 * Nc
+    reps = max(2, min(args.steps, 5))
+    dp = C.POINTER(C.c_double)
+
+    def emit(op, secs, alg_bytes, **kw):
+        s = sorted(secs)[len(secs) // 2]
+        line = {"op": op, "ms": round(s * 1e3, 3), "reps": len(secs), "alg_bytes": alg_bytes,
+                "GBps": round(alg_bytes / s / 1e9, 1) if alg_bytes else None,
+                "frac_of_hbm_peak": round(alg_bytes / s / 1e9 / HBM_PEAK_GBS, 4)
+                if alg_bytes else None}
+        line.update(kw)
+        print(json.dumps(line), flush=True)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    rng = np.random.default_rng(1)
+    g = fa.Grid(Ng, L, resident=True)
+    n, nr = 1 << 24, 1 << 25
+    cat = fa.Catalog(*(rng.random((3, n)) * L), L ** 3)
+    ran = fa.Catalog(*(rng.random((3, nr)) * L), L ** 3)
+    cat.ptr.contents.rand = ran.ptr
+    ran.ptr = None                       # freed with its parent
+    off = np.zeros(3)
+    secs = [timed(lambda: lib.fft_p2g(g.ptr, cat.ptr, off.ctypes.data_as(dp)))
+            for _ in range(reps + 1)][1:]
+    emit("fft_p2g with randoms", secs, 24 * (n + nr) + rw // 2, Np=n, Nrand=nr,
+         bound="FP64 global atomics (64 per particle)")
+    t_x, t_k = [], []
+    for r in range(reps + 1):
+        a = timed(lambda: g.x2k(1))
+        b = timed(lambda: g.k2x(1))
+        if r:
+            t_x.append(a)
+            t_k.append(b)
+    emit("fft_x2k (phase on)", t_x, 3 * rw, passes="rows, columns j, columns i")
+    emit("fft_k2x (phase off)", t_k, 3 * rw, passes="columns i, columns j, rows")
+    g.x2k(0)
+    los = np.array([0.0, 0.0, 1.0])
+    delta = np.zeros(64)
+    secs = [timed(lambda: lib.fft_dl_filter(g.ptr, 0, 4, los.ctypes.data_as(dp), 2, 1.0,
+                                            delta.ctypes.data_as(dp)))
+            for _ in range(reps + 1)][1:]
+    emit("fft_dl_filter (sub-box, Nsub 4, l 2)", secs, 4 * rw)
+    secs = [timed(lambda: lib.fft_subvolume_deconv(g.ptr, 0, 4)) for _ in range(reps + 1)][1:]
+    emit("fft_subvolume_deconv (sub-box, Nsub 4)", secs, 7 * rw)
+    g.free()
+    cat.free()
+    # QPM loaders: x y z vx vy vz halo_mass sat_flag halo_id, 2^21 lines
+    nq = 1 << 21
+    fd, path = tempfile.mkstemp(suffix=".qpm")
+    os.close(fd)
+    try:
+        cols = [rng.random(nq) * 1000.0 for _ in range(3)] + \
+            [rng.standard_normal(nq) * 300.0 for _ in range(3)]
+        with open(path, "w") as fp:
+            for i0 in range(0, nq, 1 << 18):
+                sl = slice(i0, i0 + (1 << 18))
+                block = np.column_stack([c[sl] for c in cols] + [np.full(len(cols[0][sl]), 1e13),
+                                                                 np.zeros(len(cols[0][sl])),
+                                                                 np.arange(len(cols[0][sl]))])
+                np.savetxt(fp, block, fmt="%.6f %.6f %.6f %.4f %.4f %.4f %.6e %d %d")
+        size = os.path.getsize(path)
+        secs = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            gp = lib.gal_loadqpm_cubic(path.encode(), 1000.0)
+            secs.append(time.perf_counter() - t0)
+            lib.gal_free(gp)
+        emit("gal_loadqpm_cubic (host)", secs, 0, lines=nq, file_bytes=size,
+             MBps=round(size / sorted(secs)[len(secs) // 2] / 1e6, 1))
+        xs = [torch.empty(nq, dtype=torch.float32, device="cuda:0") for _ in range(3)]
+        got = C.c_long()
+        secs = []
+        for _ in range(reps):
+            secs.append(timed(lambda: lib.fftgal_qpm_load_device(
+                path.encode(), nq, xs[0].data_ptr(), xs[1].data_ptr(), xs[2].data_ptr(),
+                C.byref(got))))
+        assert got.value == nq, got.value
+        emit("fftgal_qpm_load_device", secs, 0, lines=nq, file_bytes=size,
+             MBps=round(size / sorted(secs)[len(secs) // 2] / 1e6, 1))
+    finally:
+        os.unlink(path)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.extras:
+        return main_extras(args)
     if args.config == 4:
         return main_batch(args)
     import numpy as np
