@@ -13,10 +13,12 @@
 //    counting sort -- coarse bucket (<= 256 per grid) then tile within the
 //    bucket (<= 128); grids with more tiles take 2048 x 1024 digits -- whose
 //    scatters stage each 4096-particle chunk in LDS
-//    ordered by digit, so that global writes are contiguous runs.  Ranks
-//    come from a wave-level match of equal digits (one LDS atomic per
-//    distinct digit per wave), so heavily clustered catalogs, where most
-//    particles share a few buckets, do not serialise on one LDS counter.
+//    ordered by digit, so that global writes are contiguous runs.  The
+//    scatters' ranks come from a wave-level match of equal digits (one LDS
+//    atomic per distinct digit per wave), so heavily clustered catalogs,
+//    where most particles share a few buckets, do not serialise on one LDS
+//    counter; the count passes add one LDS atomic per particle (one per
+//    wave when the wave holds a single digit), which measured faster.
 //    Positions travel as 64-bit words: a 7-bit tile-within-bucket digit
 //    and a 19-bit tile-local fixed-point coordinate per axis (2^-15 cell
 //    for T = 16); the wide sort's 10-bit digit leaves 18 bits.  The paint
@@ -257,9 +259,10 @@ __device__ __forceinline__ void particle_key(float xv, float yv, float zv, const
 
 // Rank of this lane among the valid lanes of its wave holding the same
 // NB-bit digit, added to the workgroup histogram with one LDS atomic per
-// distinct digit per wave (RANK = false: the histogram only).  All lanes of
-// the wave must call it together.
-template <int NB, bool RANK>
+// distinct digit per wave, so that heavily clustered catalogs, where most
+// particles share a few digits, do not serialise on one LDS counter.  All
+// lanes of the wave must call it together.
+template <int NB>
 __device__ __forceinline__ uint32_t wave_rank(uint32_t d, bool valid, uint32_t *hist) {
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -270,15 +273,25 @@ __device__ __forceinline__ uint32_t wave_rank(uint32_t d, bool valid, uint32_t *
     }
     const int lane = __lane_id();
     const int leader = __ffsll((unsigned long long)peers) - 1;
-    if (!RANK) {
-        if (valid && lane == leader) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
-        return 0;
-    }
     const uint32_t below = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
     uint32_t base = 0;
     if (valid && lane == leader) base = atomicAdd(&hist[d], (uint32_t)__popcll(peers));
     base = __shfl(base, leader < 0 ? 0 : leader);
     return base + below;
+}
+
+// The digit histogram only (no ranks): one LDS atomic per valid lane, or
+// one per wave when all its valid lanes hold the same digit.  All lanes of
+// the wave must call it together.
+__device__ __forceinline__ void wave_count(uint32_t d, bool valid, uint32_t *hist) {
+    const uint64_t vm = __ballot(valid);
+    const int lead = __ffsll((unsigned long long)vm) - 1;
+    const uint32_t dl = __shfl(d, lead < 0 ? 0 : lead);
+    if (__ballot(valid && d == dl) == vm) {
+        if (__lane_id() == lead) atomicAdd(&hist[dl], (uint32_t)__popcll(vm));
+    } else if (valid) {
+        atomicAdd(&hist[d], 1u);
+    }
 }
 
 // exclusive scan of hist[0..n) (n <= NMAX <= 2048) into off[], by the first
@@ -426,7 +439,12 @@ k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__
                 d = (uint32_t)(pk >> g.fs);
             }
         }
-        rk[k] = wave_rank<NB, !COUNT>(d, valid, hist);
+        if (COUNT) {
+            wave_count(d, valid, hist);
+            rk[k] = 0;
+        } else {
+            rk[k] = wave_rank<NB>(d, valid, hist);
+        }
         dk[k] = d;
         vk[k] = pk;
     }
@@ -661,7 +679,7 @@ k_count_global(const float *__restrict__ x, const float *__restrict__ y,
     uint32_t t = 0;
     uint64_t pk;
     if (valid) particle_key(x[p], y[p], z[p], g, t, pk);
-    const uint32_t r = wave_rank<24, true>(t, valid, tot);
+    const uint32_t r = wave_rank<24>(t, valid, tot);
     if (valid) rank[p] = r;
 }
 
