@@ -89,6 +89,18 @@ def test_fixed_point_paint_vs_oracle(torch_cuda, lib, orc, Ng, tile):
     _paint_vs_oracle(torch_cuda, orc, x, y, z, L, Ng, tile)
 
 
+def test_single_cell_catalog_paint_vs_oracle(torch_cuda, lib, orc):
+    """Every particle in one grid cell: each wave of both sort levels holds
+    a single digit (the one-atomic path of the counts), and the one tile
+    takes several work items."""
+    rng = np.random.default_rng(8)
+    L, Ng = 400.0, 64
+    H = L / Ng
+    pos = rng.uniform(37 * H, 38 * H, size=(20000, 3)).astype(np.float32)
+    _paint_vs_oracle(torch_cuda, orc, pos[:, 0].copy(), pos[:, 1].copy(), pos[:, 2].copy(), L,
+                     Ng, 16)
+
+
 def test_dense_tile_paint_vs_oracle(torch_cuda, lib, orc):
     """One tile holds 30000 particles (more than one work item's 8192), the
     rest of the box a thin uniform field: tiles painted by several work
