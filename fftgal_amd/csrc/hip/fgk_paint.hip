@@ -950,7 +950,8 @@ static int paint_side(int Ng, int tile) {
     return 1;
 }
 
-// fixed-point fraction bits S of the weights for cap particles per item:
+// fixed-point fraction bits S of the weights for at most cap particles per
+// work item:
 // largest S with cap * (2/3)^3 * 2^S < 2^31 (int32 LDS sums cannot
 // overflow) and every rounded weight below 2^22 (single-FMA rounding)
 static int fixed_bits(int cap) {
@@ -1009,7 +1010,8 @@ extern "C" int fgk_paint_plan_create(fgk_paintplan **out, int Ng, int tile, long
     pp->T = T;
     pp->interlace = interlace ? 1 : 0;
     pp->cap = 8192;
-    pp->S = fixed_bits(pp->cap);
+    // no work item holds more than Np_max particles
+    pp->S = fixed_bits(Np_max < pp->cap ? (int)Np_max : pp->cap);
     pp->Np_max = Np_max;
     int mode = 0;
     pp->g = tile_geometry(Ng, T, 1, mode);
@@ -1406,7 +1408,11 @@ extern "C" int fgk_paint_batch_create(fgk_paintbatch **out, int njobs,
     if (!pb) { fgk_set_error("out of host memory"); return -1; }
     pb->njobs = njobs;
     pb->cap = 8192;
-    pb->S = fixed_bits(pb->cap);
+    {   // no work item holds more particles than the largest catalog
+        long npm = 1;
+        for (int j = 0; j < njobs; ++j) npm = jobs[j].Np_max > npm ? jobs[j].Np_max : npm;
+        pb->S = fixed_bits(npm < pb->cap ? (int)npm : pb->cap);
+    }
     pb->perm = (int *)calloc((size_t)njobs, sizeof(int));
     pb->hjobs = (PaintJob *)calloc((size_t)njobs, sizeof(PaintJob));
     pb->hNpmax = (long *)calloc((size_t)njobs, sizeof(long));

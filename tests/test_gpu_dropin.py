@@ -80,6 +80,21 @@ def test_p2g_vs_oracle_with_randoms(torch_cuda, lib, orc):
     assert np.abs(gg.f[:, :, :Ng] - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
+def test_p2g_empty_catalog_is_nan_like_the_reference(torch_cuda, lib, orc):
+    """Np = 0: nothing is painted and the normalisation 1/(nbar H^3) with
+    nbar = 0 (src/fft.c:125-129) turns the zero grid into NaN; the library
+    launches nothing for the empty catalog and gives the same grid."""
+    import fftgal_amd as fa
+    Ng, L = 16, 100.0
+    e = np.zeros(0)
+    co, cg = _catalogs(orc, fa, e, e, e, L ** 3)
+    go, gg = orc.Grid(Ng, L), fa.Grid(Ng, L)
+    go.p2g(co, (0, 0, 0))
+    gg.p2g(cg, (0, 0, 0))
+    assert np.isnan(go.f[:, :, :Ng]).all()
+    assert np.isnan(gg.f[:, :, :Ng]).all()
+
+
 def test_offset_phase_roundtrip_and_deconv(torch_cuda, lib, orc):
     import fftgal_amd as fa
     Ng, L = 12, 50.0
