@@ -80,6 +80,10 @@ int  fgk_fft_cols_f32(fgk_fftplan *p, float *f, int axis, int sign, int nbatch,
 int  fgk_fft_f32_specialised(int Ng);
 /* force the generic FP32 kernels (1) or the default choice (0); for tests */
 int  fgk_fft_force_generic(fgk_fftplan *p, int on);
+/* n1 > 0 when the plan's FP32 i-axis pass runs as a four-step Ng = n1 x
+ * (Ng / n1), which leaves frequency i on i-plane (i % n1) * (Ng / n1) +
+ * i / n1; 0 when the planes are in natural order */
+int  fgk_fft_fourstep(const fgk_fftplan *p);
 /* tuning knobs: columns per workgroup for the strided passes, rows per
  * workgroup for the row pass (0 = default); generic kernels only */
 int  fgk_fft_set_tuning(fgk_fftplan *p, int cols_per_wg, int rows_per_wg);
@@ -171,6 +175,10 @@ typedef struct fgk_binspec {
     int geometry;       /* also accumulate K sums and counts */
     int bk;             /* fused kernel: k extent of a workgroup's block of
                            1024 modes, 16, 32, 64 or 128 (0: 64) */
+    int fs_n1;          /* fused kernel: the spectra's i planes are in the
+                           order of a four-step i pass (fgk_fft_fourstep),
+                           frequency i on plane (i % fs_n1) * (Ng / fs_n1) +
+                           i / fs_n1; 0: natural order */
 } fgk_binspec;
 size_t fgk_bins_doubles(const fgk_binspec *b);
 size_t fgk_bins_counts(const fgk_binspec *b);

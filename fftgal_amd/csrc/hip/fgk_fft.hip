@@ -39,6 +39,9 @@ struct fgk_fftplan {
     int spec;            // FP32 passes use the specialised kernels
     void *stw_cols;      // stage twiddles, length Ng (specialised FP32 only)
     void *stw_rows;      // stage twiddles, length Ng/2
+    int fs_n1;           // i-axis pass as a four-step Ng = n1 x (Ng / n1), 0 = one pass
+    void *stw_fs1;       // stage twiddles, lengths fs_n1 and Ng / fs_n1
+    void *stw_fs2;
 };
 
 static int factor_stages(int n, Stages *s) {
@@ -472,11 +475,13 @@ __device__ __forceinline__ void spencil(int q, In in, Ld ld, St st, GOut gout,
 }
 
 // column pass: C consecutive columns of one row/plane per workgroup, the
-// pencil index t at stride stride_t; LDS row stride C+1 complex
+// pencil index t at stride stride_t; LDS row stride C+1 complex.  With tw4,
+// output t of the pencils of outer group o is multiplied by tw4[o t] (the
+// twiddle between the two passes of a four-step transform).
 template <int L, int C, int TPP>
 __global__ void __launch_bounds__(C * TPP)
 k_cols32(cpx<float> *g, int ncols, long stride_t, long stride_o, const cpx<float> *stw,
-         const cpx<float> *tw, int tws, int sign) {
+         const cpx<float> *tw, int tws, int sign, const cpx<float> *tw4) {
     constexpr int LS = C + 1;
     __shared__ cpx<float> buf[L * LS];
     const int c = threadIdx.x % C, q = threadIdx.x / C;
@@ -490,6 +495,7 @@ k_cols32(cpx<float> *g, int ncols, long stride_t, long stride_o, const cpx<float
         return live ? base[(long)t * stride_t] : cmk(0.0f, 0.0f);
     };
     auto gout = [&](int t, cpx<float> v) {
+        if (tw4) v = twmul(v, tw4[o * t], sign);
         if (live) base[(long)t * stride_t] = v;
     };
     auto ld = [&](int t) -> cpx<float> { return buf[t * LS + c]; };
@@ -585,14 +591,14 @@ extern "C" int fgk_fft_f32_specialised(int Ng) {
 // pencil element t at stride stride_t
 static int launch_cols32(int N, cpx<float> *g, int ncols, long stride_t, long stride_o,
                          long nouter, const cpx<float> *stw, const cpx<float> *tw, int sign,
-                         hipStream_t s) {
+                         hipStream_t s, const cpx<float> *tw4 = nullptr, int tws = 1) {
     switch (N) {
 #define CASE_NG(n)                                                                            \
     case n: {                                                                                 \
         constexpr int C = cols_C(n), TPP = cols_TPP(n);                                       \
         const long nb = nouter * ((ncols + C - 1) / C);                                       \
         hipLaunchKernelGGL((k_cols32<n, C, TPP>), dim3((unsigned)nb), dim3(C * TPP), 0, s, g, \
-                           ncols, stride_t, stride_o, stw, tw, 1, sign);                      \
+                           ncols, stride_t, stride_o, stw, tw, tws, sign, tw4);               \
         break;                                                                                \
     }
         FGK_SPECIALISED_NG(CASE_NG)
@@ -600,6 +606,31 @@ static int launch_cols32(int N, cpx<float> *g, int ncols, long stride_t, long st
     default:
         fgk_set_error("no specialised FFT kernel for Ng=%d", N);
         return -2;
+    }
+    FGK_LAUNCHED();
+    return 0;
+}
+
+// the passes of a four-step transform are short (n <= 64): 64 columns per
+// workgroup, so that each row piece is 512 bytes
+#define FGK_FOURSTEP_L(X) X(16) X(24) X(32) X(48) X(64)
+static int launch_cols32_wide(int N, cpx<float> *g, int ncols, long stride_t, long stride_o,
+                              long nouter, const cpx<float> *stw, const cpx<float> *tw,
+                              int sign, hipStream_t s, const cpx<float> *tw4, int tws) {
+    switch (N) {
+#define CASE_NG(n)                                                                            \
+    case n: {                                                                                 \
+        constexpr int C = 64, TPP = n / seq_rmax(n);                                          \
+        static_assert(C * TPP <= 1024 && n * (C + 1) * 8 <= 80 * 1024, "launch shape");       \
+        const long nb = nouter * ((ncols + C - 1) / C);                                       \
+        hipLaunchKernelGGL((k_cols32<n, C, TPP>), dim3((unsigned)nb), dim3(C * TPP), 0, s, g, \
+                           ncols, stride_t, stride_o, stw, tw, tws, sign, tw4);               \
+        break;                                                                                \
+    }
+        FGK_FOURSTEP_L(CASE_NG)
+#undef CASE_NG
+    default:
+        return launch_cols32(N, g, ncols, stride_t, stride_o, nouter, stw, tw, sign, s, tw4, tws);
     }
     FGK_LAUNCHED();
     return 0;
@@ -701,10 +732,28 @@ extern "C" int fgk_fft_plan_create(fgk_fftplan **out, int Ng, int precision) {
             fgk_fft_plan_destroy(p);
             return -1;
         }
+        // the i-axis column pass of large grids strides over Ng planes of
+        // Ng^2/2 complex (16 MB apart at Ng = 2048), one row per plane per
+        // workgroup: split it into passes of n1 and Ng / n1 rows
+        // (FFTGAL_FFT_FOURSTEP=n1 chooses the split for any grid, 0 turns
+        // it off)
+        int n1 = (Ng >= 2048 && Ng % 64 == 0) ? 64 : 0;
+        const char *fs = getenv("FFTGAL_FFT_FOURSTEP");
+        if (fs) n1 = atoi(fs);
+        if (n1 > 1 && Ng % n1 == 0 && fgk_fft_f32_specialised(n1) &&
+            fgk_fft_f32_specialised(Ng / n1)) {
+            p->fs_n1 = n1;
+            if (stage_twiddles(n1, &p->stw_fs1) || stage_twiddles(Ng / n1, &p->stw_fs2)) {
+                fgk_fft_plan_destroy(p);
+                return -1;
+            }
+        }
     }
     *out = p;
     return 0;
 }
+
+extern "C" int fgk_fft_fourstep(const fgk_fftplan *p) { return p->spec ? p->fs_n1 : 0; }
 
 extern "C" int fgk_fft_force_generic(fgk_fftplan *p, int on) {
     p->spec = (!on && p->stw_cols && p->stw_rows) ? 1 : 0;
@@ -716,6 +765,8 @@ extern "C" void fgk_fft_plan_destroy(fgk_fftplan *p) {
     fgk_free(p->tw);
     fgk_free(p->stw_cols);
     fgk_free(p->stw_rows);
+    fgk_free(p->stw_fs1);
+    fgk_free(p->stw_fs2);
     free(p);
 }
 
@@ -796,6 +847,26 @@ extern "C" int fgk_fft_cols_f32(fgk_fftplan *p, float *f, int axis, int sign, in
     if (nbatch < 1) { fgk_set_error("FFT: nbatch must be >= 1"); return -2; }
     const int N = p->Ng, M = N / 2;
     if (!p->spec) return launch_cols<float>(p, f, axis, sign, 1.0f, (hipStream_t)s, 1, nbatch);
+    if (axis == 0 && p->fs_n1) {
+        // four-step over i = n2 + N2 n1 (N1 = fs_n1, N2 = N / N1), per grid:
+        // pass A, length N1 over n1 (planes N2 apart) for each n2, output k1
+        // times W_N^(n2 k1); pass B, length N2 over n2 (consecutive planes)
+        // for each k1.  Frequency i = k1 + N1 k2 lands on plane k2 + N2 k1.
+        const int N1 = p->fs_n1, N2 = N / N1;
+        const long P = (long)N * M;                   // complex per i-plane
+        for (int g = 0; g < nbatch; ++g) {
+            cpx<float> *b = (cpx<float> *)f + (long)g * N * P;
+            if (launch_cols32_wide(N1, b, (int)P, N2 * P, P, N2, (const cpx<float> *)p->stw_fs1,
+                                   (const cpx<float> *)p->tw, sign, (hipStream_t)s,
+                                   (const cpx<float> *)p->tw, N / N1))
+                return -1;
+            if (launch_cols32_wide(N2, b, (int)P, P, N2 * P, N1, (const cpx<float> *)p->stw_fs2,
+                                   (const cpx<float> *)p->tw, sign, (hipStream_t)s, nullptr,
+                                   N / N2))
+                return -1;
+        }
+        return 0;
+    }
     const ColShape cs = col_shape(N, M, axis, nbatch);
     return launch_cols32(N, (cpx<float> *)f, cs.ncols, cs.stride_t, cs.stride_o, cs.nouter,
                          (const cpx<float> *)p->stw_cols, (const cpx<float> *)p->tw, sign,

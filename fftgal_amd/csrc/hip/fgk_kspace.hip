@@ -281,9 +281,15 @@ struct BinArgs {
     int nvals;        // doubles per bin in LDS
     int ncnt;         // counts per bin in LDS
     int wmax;         // LDS window capacity in bins
+    int fs_n1;        // spectra planes in four-step order (0: natural)
     double dKinv;
     double l0, l1, l2;
 };
+
+// i-plane holding frequency i of the packed spectra
+__device__ __forceinline__ int iplane(const BinArgs &a, int i) {
+    return a.fs_n1 ? (i % a.fs_n1) * (a.Ng / a.fs_n1) + i / a.fs_n1 : i;
+}
 
 // signed frequency of index i with the Nyquist index carrying -Ng/2
 // (src/power.c:46-51)
@@ -690,7 +696,7 @@ k_bin(const double *__restrict__ fP, const float *__restrict__ A, const float *_
         float2 av[KPT], bv[KPT];
         double qr = 1.0, qi = 0.0, wij = 1.0;
         if (MODE == 1) {
-            const long m = ((long)i * Ng + j) * Mh + kb;
+            const long m = ((long)iplane(a, i) * Ng + j) * Mh + kb;
             if (vec) {
                 const float4 *pa = (const float4 *)((const float2 *)A + m);
                 const float4 a0 = pa[0], a1 = pa[1];
@@ -777,7 +783,8 @@ k_bin_plane(const float *__restrict__ A, const float *__restrict__ B, BinArgs a,
     grid_norms(A, B, norm, npart, na, nb);
     if (i < i1 && j < j1) {
         const int im = i ? Ng - i : 0, jm = j ? Ng - j : 0;
-        const long m = ((long)i * Ng + j) * Mh, mm = ((long)im * Ng + jm) * Mh;
+        const long m = ((long)iplane(a, i) * Ng + j) * Mh,
+                   mm = ((long)iplane(a, im) * Ng + jm) * Mh;
         const float2 av = ((const float2 *)A)[m], am = ((const float2 *)A)[mm];
         float2 bv = make_float2(0.0f, 0.0f), bm = bv;
         if (B) {
@@ -824,6 +831,7 @@ static BinArgs make_args(const fgk_binspec *b) {
     // corner (about BK + 4 bins at dK = KF) even with many wedges
     a.wmax = per ? (int)(BIN_LDS_BYTES / per) : 1;
     if (a.wmax < BK + 32) a.wmax = BK + 32;
+    a.fs_n1 = b->fs_n1;
     a.dKinv = b->dKinv;
     a.l0 = b->los[0];
     a.l1 = b->los[1];

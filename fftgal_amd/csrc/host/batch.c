@@ -316,6 +316,7 @@ static int enqueue_work(fftgal_power_t *p, const float *x, const float *y, const
     double norm, Vinv, Pshot;
     catalog_norms(Ng, p->d.L, Np, V, &norm, &Vinv, &Pshot);
     const double in_scale = fgk_paint_fixed_scale(p->paint);
+    p->spec.fs_n1 = fgk_fft_fourstep(p->plan);   /* the tuner may switch FFT kernels */
     int st = 0;
 #define MARK() do { if (ev) TRY(fgk_event_record(ev[st], p->s)); ++st; } while (0)
     MARK();
@@ -931,6 +932,7 @@ static int batch_tables(fftgal_batch_t *b, const float *const *x, const float *c
  * fftgal_power_stage_name; ev (NSTAGES+1 events or NULL) brackets them */
 static int batch_work(fftgal_batch_t *b, fgk_event_t *ev) {
     int st = 0;
+    for (int q = 0; q < b->ngroups; ++q) b->grp[q].spec.fs_n1 = fgk_fft_fourstep(b->grp[q].plan);
 #define MARK() do { if (ev) TRY(fgk_event_record(ev[st], b->s)); ++st; } while (0)
     MARK();
     TRY(fgk_fill_zero(b->d_bins, b->nbins * sizeof(double), b->s));

@@ -60,6 +60,7 @@ static int run_bins(fft_t *grid, const double los[3], double dK, int Nmu, int do
     spec->do_pl = do_pl;
     spec->geometry = 1;
     spec->bk = 0;
+    spec->fs_n1 = 0;
     size_t nb = fgk_bins_doubles(spec), nc = fgk_bins_counts(spec);
     double *dbins = NULL;
     int64_t *dcnt = NULL;

@@ -74,7 +74,11 @@ int fftgal_power_fixed_bits(const fftgal_power_t *p);
  * k < Ng/2, unnormalised forward transform of the fixed-point grid in
  * particle units): column k = 0 holds Z = F(X0) + i F(XN), from which
  * delta(i,j,0) = (Z[i,j] + conj Z[-i,-j]) / 2 and
- * delta(i,j,Ng/2) = (Z[i,j] - conj Z[-i,-j]) / 2i */
+ * delta(i,j,Ng/2) = (Z[i,j] - conj Z[-i,-j]) / 2i.  Grids of Ng >= 2048
+ * (Ng a multiple of 64) run their i-axis pass as a four-step 64 x Ng/64,
+ * which leaves frequency i on i-plane (i % 64) * (Ng / 64) + i / 64
+ * (FFTGAL_FFT_FOURSTEP=n1 at plan creation sets the split n1, 0 keeps the
+ * natural order). */
 int fftgal_power_spectra(fftgal_power_t *p, void **A, void **B);
 /* tuning in use: paint tile side, generic FFT kernels' columns / rows per
  * workgroup (0: automatic), 1 if the generic FFT kernels run, k extent of

@@ -1,6 +1,8 @@
 """GPU parity of the mixed-precision pipeline (include/fftgal_batch.h)
 against the FP64 oracle: deterministic fixed-point paint, FP32 FFT, fused
 bin kernel.  Mode counts bit-exact; P_l within a stated tolerance."""
+import os
+
 import numpy as np
 import pytest
 
@@ -216,6 +218,47 @@ def test_tiny_catalogs_vs_oracle(torch_cuda, lib, orc, Np):
     scale = np.abs(ref["Pl"][ok, 0]) + V / Np
     for l in range(3):
         assert (np.abs(out["Pl"][ok, l] - ref["Pl"][ok, l]) / scale).max() <= 1e-5
+
+
+@pytest.mark.parametrize("Ng,n1", [(256, 16), (384, 16)])
+def test_fourstep_i_pass_matches_one_pass(torch_cuda, lib, Ng, n1):
+    """The i-axis column pass as a four-step n1 x Ng/n1 (what grids of
+    Ng >= 2048 run, forced here on small grids): frequency i lands on plane
+    (i % n1) * (Ng / n1) + i / n1 of the spectra, equal to the one-pass
+    transform to FP32 rounding, and the bins read it through that map."""
+    import fftgal_amd as fa
+    from fftgal_amd import mockgen
+    torch = torch_cuda
+    L = 800.0
+    (x, y, z), _ = mockgen.lognormal(300000, L, seed=Ng, Ngs=32)
+    t = _dev(torch, x, y, z)
+    ptr = [a.data_ptr() for a in t]
+    V = L ** 3
+    res, spec = {}, {}
+    for split in (0, n1):
+        os.environ["FFTGAL_FFT_FOURSTEP"] = str(split)
+        try:
+            p = fa.PowerPlan(Ng, L, interlace=True, Nmu=6, Np_max=len(x))
+        finally:
+            os.environ.pop("FFTGAL_FFT_FOURSTEP", None)
+        res[split] = p.run(*ptr, len(x), V)
+        if Ng <= 256:
+            spec[split] = p.spectra_host()[0]
+        p.close()
+    if spec:
+        i = np.arange(Ng)
+        plane = (i % n1) * (Ng // n1) + i // n1
+        ref = spec[0]
+        assert np.abs(spec[n1][plane] - ref).max() <= 1e-5 * np.abs(ref).max()
+    a, b = res[0], res[n1]
+    assert np.array_equal(a["N"], b["N"])
+    ok = a["N"] > 0
+    scale = np.abs(a["Pl"][ok, 0]) + V / len(x)
+    for l in range(4):
+        assert (np.abs(b["Pl"][ok, l] - a["Pl"][ok, l]) / scale).max() <= 1e-5
+    wm = a["Nmu"] > 0
+    assert np.array_equal(a["Nmu"], b["Nmu"])
+    assert (np.abs(b["Pmu"][wm] - a["Pmu"][wm]) / (np.abs(a["Pmu"][wm]) + V / len(x))).max() <= 1e-5
 
 
 def test_graph_replays_match_eager(torch_cuda, lib):
