@@ -763,7 +763,9 @@ __device__ void flush_footprint(const int *__restrict__ fp, int oi, int oj, int 
         const int v = fp[((fi >> 2) * NBK + (fj >> 2)) * NBK * 64 + (fi & 3) * 16 + (fj & 3) * 4 + wk];
         if (!v) continue;
         const int gi = wrap(oi + fi), gj = wrap(oj + fj);
-        __hip_atomic_fetch_add(g + ((long)gi * Ng + gj) * Ng + gk,
+        // Ng < 2^16: the row index fits 32 bits, one 32 x 32 -> 64 multiply-add
+        const uint32_t row = (uint32_t)gi * (uint32_t)Ng + (uint32_t)gj;
+        __hip_atomic_fetch_add(g + ((uint64_t)row * (uint32_t)Ng + (uint32_t)gk),
                                (unsigned long long)(long long)v, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -1000,7 +1002,7 @@ static KeyGeo tile_geometry(int Ng, int T, int allow_wide, int &mode) {
 extern "C" int fgk_paint_plan_create(fgk_paintplan **out, int Ng, int tile, long Np_max,
                                      int interlace) {
     *out = NULL;
-    if (Ng < 2 || Ng > (1 << 20)) { fgk_set_error("paint: Ng must be in [2, 2^20]"); return -2; }
+    if (Ng < 2 || Ng > 65535) { fgk_set_error("paint: Ng must be in [2, 65535]"); return -2; }
     if (Np_max < 1) Np_max = 1;
     if (Np_max >= (1l << 32) - 1) { fgk_set_error("paint: at most 2^32-2 particles"); return -2; }
     const int T = paint_side(Ng, tile);
@@ -1431,7 +1433,7 @@ extern "C" int fgk_paint_batch_create(fgk_paintbatch **out, int njobs,
         int first = n;
         for (int j = 0; j < njobs; ++j) {
             const fgk_paintjob_desc &d = jobs[j];
-            if (d.Ng < 2 || d.Ng > (1 << 20) || !(d.Hinv > 0) || d.Np_max < 1 || !d.gridA ||
+            if (d.Ng < 2 || d.Ng > 65535 || !(d.Hinv > 0) || d.Np_max < 1 || !d.gridA ||
                 (d.interlace && !d.gridB)) {
                 fgk_set_error("paint batch: invalid catalog %d", j);
                 fgk_paint_batch_destroy(pb);
