@@ -77,19 +77,23 @@ def parse():
     return a
 
 
-def stage_bytes(name, Np, Ng):
-    """Algorithmic (compulsory) bytes of one stage of one catalog."""
+def stage_bytes(name, Np, Ng, sets=1):
+    """Algorithmic (compulsory) bytes of one stage of one catalog.  The int64
+    grids are cleared for the next paint by the row FFT (one grid set) or by
+    the paint itself, which clears the other set (two sets)."""
     n3 = Ng ** 3
     spec = 4 * n3                                # FP32 packed half spectrum: Ng^3/2 complex
+    clear = 2 * 8 * n3                           # both int64 grids
     table = {
         "sort1_count": 12 * Np,                 # read float32 xyz
         "sort1_scatter": 12 * Np + 8 * Np,      # read xyz, write packed 8-byte word
         "sort2_count": 8 * Np,                  # read packed words
         "sort2_scatter": 8 * Np + 8 * Np,       # read, write packed words
-        "paint_tiles": 8 * Np + 2 * 16 * n3,    # read packed, RMW both int64 grids once
-        # both grids: read the int64 grid, clear it for the next paint,
-        # write the spectrum
-        "fft_rows": 2 * (8 * n3 + 8 * n3 + spec),
+        # read packed, RMW both int64 grids once (and clear the other set)
+        "paint_tiles": 8 * Np + 2 * 16 * n3 + (clear if sets == 2 else 0),
+        # both grids: read the int64 grid, write the spectrum (and clear the
+        # grid for the next paint)
+        "fft_rows": 2 * (8 * n3 + spec) + (clear if sets == 1 else 0),
         "fft_cols_j": 2 * 2 * spec, "fft_cols_i": 2 * 2 * spec,   # read + write FP32
         "fused_bin": 2 * spec,                  # read both spectra
     }
@@ -533,6 +537,7 @@ def main():
         return
 
     # per-stage times, HIP events on the pipeline's stream
+    sets = plan.grid_sets()
     nprof = max(3, min(args.steps, 5))
     acc = {}
     for _ in range(nprof):
@@ -542,12 +547,12 @@ def main():
     print("# stage profile done", file=sys.stderr, flush=True)
     stages = {}
     for k, ms in acc.items():
-        b = stage_bytes(k, Np, Ng)
+        b = stage_bytes(k, Np, Ng, sets)
         stages[k] = {"ms": round(ms, 4), "alg_bytes": b,
                      "GBps": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 and b else None}
-    dom = max((k for k in acc if stage_bytes(k, Np, Ng)), key=lambda k: acc[k])
+    dom = max((k for k in acc if stage_bytes(k, Np, Ng, sets)), key=lambda k: acc[k])
     dom_ms = acc[dom]
-    ach = stage_bytes(dom, Np, Ng) / (dom_ms * 1e-3) / 1e9
+    ach = stage_bytes(dom, Np, Ng, sets) / (dom_ms * 1e-3) / 1e9
     traffic = None
     tr = load_traffic()
     # the counter profile counts only for the workload it was taken on
@@ -629,7 +634,7 @@ def main():
                                 f", PCS paint + interlacing, l=0,2,4(,6) and "
                                 f"{nmu} |mu|-wedges, dK=kF"),
                    "Np": Np, "Ng": Ng, "L": L, "fold": args.fold, "interlace": True,
-                   "Nmu": nmu,
+                   "Nmu": nmu, "grid_sets": sets,
                    "precision": "int64 fixed-point paint, FP32 FFT (FP64 twiddles), "
                                 "FP64 k-space math and bin sums",
                    "parallelism": "1 GPU"},

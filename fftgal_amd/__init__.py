@@ -94,6 +94,7 @@ API = {
                                           C.c_long, C.POINTER(C.c_void_p),
                                           C.POINTER(C.c_void_p)]),
     "fftgal_power_fixed_bits": (C.c_int, [C.c_void_p]),
+    "fftgal_power_grid_sets": (C.c_int, [C.c_void_p]),
     "fftgal_power_spectra": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p),
                                        C.POINTER(C.c_void_p)]),
     "fftgal_power_tuning": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -387,6 +388,11 @@ class PowerPlan:
 
     def fixed_bits(self):
         return self.lib.fftgal_power_fixed_bits(self.h)
+
+    def grid_sets(self):
+        """Number of int64 grid sets (2: catalogs alternate and each paint
+        clears the other set)."""
+        return self.lib.fftgal_power_grid_sets(self.h)
 
     def spectra(self):
         a, b = C.c_void_p(), C.c_void_p()

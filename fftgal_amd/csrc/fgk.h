@@ -25,6 +25,7 @@ int  fgk_init(void);                       /* idempotent; selects device 0 */
 const char *fgk_last_error(void);
 void fgk_set_error(const char *fmt, ...);
 int  fgk_malloc(void **ptr, size_t bytes);
+size_t fgk_mem_free(void);                   /* free device memory, bytes (0 if unknown) */
 int  fgk_free(void *ptr);
 int  fgk_host_alloc(void **ptr, size_t bytes);   /* pinned host memory */
 int  fgk_host_free(void *ptr);
@@ -108,6 +109,11 @@ int  fgk_paint_fixed(fgk_paintplan *pp, const float *x, const float *y,
                      const float *z, long Np, double Hinv,
                      int64_t *gridA, int64_t *gridB, fgk_stream_t s);
 double fgk_paint_fixed_scale(const fgk_paintplan *pp);  /* 2^-S */
+/* Memory the paint kernel clears while it paints (bytes a multiple of 16,
+ * 16-byte aligned; NULL: none): spread over the paint's workgroups, for a
+ * caller that alternates between two grid sets and so needs the other set
+ * zero for its next paint.  Kept until changed. */
+void fgk_paint_set_zero(fgk_paintplan *pp, void *ptr, size_t bytes);
 int  fgk_paint_plan_tile(const fgk_paintplan *pp);
 /* the paint stages separately (profiling / tests): fgk_paint_nstages()
  * stages, named by fgk_paint_stage_name() */

@@ -198,6 +198,8 @@ struct fgk_paintplan {
     uint32_t *tilebase;  // [nslots+1]
     uint32_t *items;     // [nslots+1] paint work items, exclusive prefix
     uint32_t *rank32;    // [Np_max] single-level path
+    uint64_t *zero_ptr;  // cleared by the paint kernel (fgk_paint_set_zero), 8-byte words
+    long zero_words;
     uint64_t *sorted1, *sorted2;
 };
 
@@ -871,6 +873,8 @@ __device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, in
 // particles PAINT_THREADS apart (coalesced loads) and adds its stencils to
 // the plain and the dual LDS footprint as int32 fixed point; the footprints
 // are then flushed.
+// zbuf (one plan): every launched workgroup first clears its share of
+// zwords words there.
 // Batched (jobs != nullptr): workgroup b paints item ji0[ja] + b of the
 // batch; the job owning it (ji0[job] <= item < ji0[job+1], ja <= job < jb)
 // supplies the geometry, grids and tile tables (tilebase, items + job * sl,
@@ -881,8 +885,15 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
               const uint32_t *__restrict__ items, long nslots, KeyGeo g, int nb2, float wscale,
               int cap, int interlace, unsigned long long *__restrict__ GA,
               unsigned long long *__restrict__ GB, const PaintJob *__restrict__ jobs,
-              const long *__restrict__ ji0, int ja, int jb, long sl) {
+              const long *__restrict__ ji0, int ja, int jb, long sl, uint64_t *__restrict__ zbuf,
+              long zwords) {
     constexpr int SA = T + 3, SB = T + 4;
+    if (zbuf) {   // this workgroup's slice of the memory to clear (16-byte stores)
+        const long per = ((zwords + gridDim.x - 1) / gridDim.x + 1) & ~1l;
+        const long z0 = blockIdx.x * per, z1 = z0 + per < zwords ? z0 + per : zwords;
+        for (long q = z0 + 2 * threadIdx.x; q + 1 < z1; q += 2 * blockDim.x)
+            *(ulonglong2 *)(zbuf + q) = make_ulonglong2(0ull, 0ull);
+    }
     constexpr int NA = brick_words(SA), NB = brick_words(SB);
     extern __shared__ int lds[];
     long it = blockIdx.x;
@@ -1268,12 +1279,14 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
             hipLaunchKernelGGL((k_paint_tiles<TT, POS_BITS_WIDE>), grid, block, lds, s,       \
                                pp->sorted2, pp->tilebase, pp->items, pp->nslots, g, pp->nb2,  \
                                ws, pp->cap, pp->interlace, A, B, (const PaintJob *)nullptr,   \
-                               (const long *)nullptr, 0, 0, 0l);                              \
+                               (const long *)nullptr, 0, 0, 0l, pp->zero_ptr,                 \
+                               pp->zero_words);                                               \
         else                                                                                  \
             hipLaunchKernelGGL((k_paint_tiles<TT, POS_BITS>), grid, block, lds, s,            \
                                pp->sorted2, pp->tilebase, pp->items, pp->nslots, g, pp->nb2,  \
                                ws, pp->cap, pp->interlace, A, B, (const PaintJob *)nullptr,   \
-                               (const long *)nullptr, 0, 0, 0l);                              \
+                               (const long *)nullptr, 0, 0, 0l, pp->zero_ptr,                 \
+                               pp->zero_words);                                               \
     } while (0)
         switch (pp->T) {
         case 16: PAINT_T(16); break;
@@ -1292,6 +1305,11 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
     }
     fgk_set_error("paint: unknown stage %d", stage);
     return -2;
+}
+
+extern "C" void fgk_paint_set_zero(fgk_paintplan *pp, void *ptr, size_t bytes) {
+    pp->zero_ptr = (uint64_t *)ptr;
+    pp->zero_words = ptr ? (long)(bytes / 16) * 2 : 0;
 }
 
 extern "C" int fgk_paint_fixed(fgk_paintplan *pp, const float *x, const float *y, const float *z,
@@ -1670,7 +1688,8 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
                        pb->tilebase,                                                          \
                        pb->items, 0l, g0, MAX_D2, ws, pb->cap, 0,                             \
                        (unsigned long long *)nullptr, (unsigned long long *)nullptr,          \
-                       (const PaintJob *)pb->djobs, (const long *)pb->di0, ja, jb, BATCH_SL)
+                       (const PaintJob *)pb->djobs, (const long *)pb->di0, ja, jb, BATCH_SL,  \
+                       (uint64_t *)nullptr, 0l)
             switch (pb->side[i]) {
             case 16: PAINT_TB(16); break;
             case 12: PAINT_TB(12); break;

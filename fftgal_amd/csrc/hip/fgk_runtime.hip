@@ -158,6 +158,12 @@ extern "C" int fgk_event_record(fgk_event_t e, fgk_stream_t s) {
     return 0;
 }
 
+extern "C" size_t fgk_mem_free(void) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+    return fr;
+}
+
 extern "C" int fgk_event_sync(fgk_event_t e) {
     FGK_HIP(hipEventSynchronize((hipEvent_t)e));
     return 0;

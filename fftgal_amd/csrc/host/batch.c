@@ -2,6 +2,9 @@
  *
  * One fftgal_power_t holds everything a catalog needs on the device:
  *   int64 fixed-point grids  gA (plain) and gB (dual)     Ng^3 each, gB = gA + Ng^3
+ *   (and, when device memory allows, a second such set: catalogs alternate
+ *   between the two, and each paint clears the other set while it runs, so
+ *   that the row FFT need not clear the grid it reads)
  *   FP32 spectra             fA, fB  (packed half spectrum) Ng^3 floats each, fB after fA
  *   (one allocation each, so that the FFT passes run over both grids in one launch)
  *   paint plan (sort buffers), FFT plan (FP32 twiddles from FP64),
@@ -39,6 +42,8 @@ struct fftgal_power {
     fgk_fftplan *plan;
     fgk_paintplan *paint;
     int64_t *gA, *gB;
+    int64_t *gA2;              /* second grid set (its dual grid at gA2 + Ng^3), or NULL */
+    int parity;                /* grid set of the next enqueue (0: gA, 1: gA2) */
     float *fA, *fB;
     double *d_phase, *d_winv;
     fgk_binspec spec;          /* per-run sums */
@@ -52,12 +57,12 @@ struct fftgal_power {
     long Np;
     double V;
     int grids_dirty;
-    /* graph */
+    /* graphs, one per grid set */
     int use_graph;
-    fgk_graph_t graph;
-    const float *kx, *ky, *kz;
-    long kNp;
-    double kV;
+    fgk_graph_t graph[2];
+    const float *kx[2], *ky[2], *kz[2];
+    long kNp[2];
+    double kV[2];
     /* tuning */
     fftgal_tuning tune;
 };
@@ -159,10 +164,12 @@ static void normalise_bins(int NK, int W, int Nmu, double KF, const double *hb, 
 int fftgal_power_destroy(fftgal_power_t *p) {
     if (!p) return 0;
     if (p->s) fgk_stream_sync(p->s);
-    fgk_graph_destroy(p->graph);
+    fgk_graph_destroy(p->graph[0]);
+    fgk_graph_destroy(p->graph[1]);
     fgk_fft_plan_destroy(p->plan);
     fgk_paint_plan_destroy(p->paint);
     fgk_free(p->gA);   /* gB, fB share these allocations */
+    fgk_free(p->gA2);
     fgk_free(p->fA);
     fgk_free(p->d_phase);
     fgk_free(p->d_winv);
@@ -274,6 +281,13 @@ int fftgal_power_create(fftgal_power_t **out, const fftgal_power_desc *desc) {
     }
     STEP(fgk_fill_zero(p->gA, n64, p->s));
     if (desc->interlace) STEP(fgk_fill_zero(p->gB, n64, p->s));
+    /* the second grid set, when it leaves at least 4 GiB of device memory */
+    if (!rc && fgk_mem_free() > (size_t)ngrids * n64 + ((size_t)4 << 30)) {
+        if (fgk_malloc((void **)&p->gA2, ngrids * n64) == 0)
+            STEP(fgk_fill_zero(p->gA2, ngrids * n64, p->s));
+        else
+            p->gA2 = NULL;
+    }
     STEP(compute_geometry(p));
     STEP(fgk_stream_sync(p->s));
 #undef STEP
@@ -317,17 +331,28 @@ static int enqueue_work(fftgal_power_t *p, const float *x, const float *y, const
     catalog_norms(Ng, p->d.L, Np, V, &norm, &Vinv, &Pshot);
     const double in_scale = fgk_paint_fixed_scale(p->paint);
     p->spec.fs_n1 = fgk_fft_fourstep(p->plan);   /* the tuner may switch FFT kernels */
+    const size_t Ng3 = (size_t)Ng * Ng * Ng;
+    const int ngrids = p->d.interlace ? 2 : 1;
+    /* grid set of this catalog; with two sets the paint clears the other */
+    int64_t *gA = p->gA;
+    if (p->gA2) {
+        int64_t *other = p->parity ? p->gA : p->gA2;
+        if (p->parity) gA = p->gA2;
+        fgk_paint_set_zero(p->paint, other, (size_t)ngrids * Ng3 * sizeof(int64_t));
+    } else {
+        fgk_paint_set_zero(p->paint, NULL, 0);
+    }
+    int64_t *gB = p->d.interlace ? gA + Ng3 : NULL;
     int st = 0;
 #define MARK() do { if (ev) TRY(fgk_event_record(ev[st], p->s)); ++st; } while (0)
     MARK();
     TRY(fgk_fill_zero(p->d_bins, p->nbins * sizeof(double), p->s));
     for (int stage = 0; stage < NPAINT; ++stage) {
         MARK();
-        TRY(fgk_paint_stage(p->paint, stage, x, y, z, Np, Hinv, p->gA, p->gB, p->s));
+        TRY(fgk_paint_stage(p->paint, stage, x, y, z, Np, Hinv, gA, gB, p->s));
     }
-    const int ngrids = p->d.interlace ? 2 : 1;
     MARK();
-    TRY(fgk_fft_rows_f32_from_fixed(p->plan, p->fA, p->gA, in_scale, 1, ngrids, p->s));
+    TRY(fgk_fft_rows_f32_from_fixed(p->plan, p->fA, gA, in_scale, p->gA2 ? 0 : 1, ngrids, p->s));
     MARK();
     TRY(fgk_fft_cols_f32(p->plan, p->fA, 1, -1, ngrids, p->s));
     MARK();
@@ -351,6 +376,15 @@ const char *fftgal_power_stage_name(int i) {
     return NULL;
 }
 
+/* zero every int64 grid of the plan (after a paint_only) */
+static int zero_grids(fftgal_power_t *p) {
+    const size_t n = (size_t)p->d.Ng * p->d.Ng * p->d.Ng * sizeof(int64_t) * (p->d.interlace ? 2 : 1);
+    TRY(fgk_fill_zero(p->gA, n, p->s));
+    if (p->gA2) TRY(fgk_fill_zero(p->gA2, n, p->s));
+    p->grids_dirty = 0;
+    return 0;
+}
+
 int fftgal_power_profile(fftgal_power_t *p, const float *x, const float *y, const float *z,
                          long Np, double V, float *stage_ms) {
     if (Np < 1 || Np > p->d.Np_max || !(V > 0)) {
@@ -360,13 +394,9 @@ int fftgal_power_profile(fftgal_power_t *p, const float *x, const float *y, cons
     fgk_event_t ev[NSTAGES + 1] = {0};
     int rc = 0;
     for (int i = 0; i <= NSTAGES && !rc; ++i) rc = fgk_event_create(&ev[i]);
-    if (!rc && p->grids_dirty) {
-        const size_t n64 = (size_t)p->d.Ng * p->d.Ng * p->d.Ng * sizeof(int64_t);
-        rc = fgk_fill_zero(p->gA, n64, p->s);
-        if (!rc && p->d.interlace) rc = fgk_fill_zero(p->gB, n64, p->s);
-        p->grids_dirty = 0;
-    }
+    if (!rc && p->grids_dirty) rc = zero_grids(p);
     if (!rc) rc = enqueue_work(p, x, y, z, Np, V, ev);
+    if (!rc && p->gA2) p->parity ^= 1;
     if (!rc) rc = fgk_stream_sync(p->s);
     for (int i = 0; i < NSTAGES && !rc; ++i) rc = fgk_event_elapsed_ms(&stage_ms[i], ev[i], ev[i + 1]);
     for (int i = 0; i <= NSTAGES; ++i) fgk_event_destroy(ev[i]);
@@ -387,20 +417,23 @@ int fftgal_power_enqueue(fftgal_power_t *p, const float *x, const float *y, cons
         fgk_set_error("enqueue: V must be > 0");
         return -2;
     }
-    if (p->grids_dirty) {
-        const size_t n64 = (size_t)p->d.Ng * p->d.Ng * p->d.Ng * sizeof(int64_t);
-        TRY(fgk_fill_zero(p->gA, n64, p->s));
-        if (p->d.interlace) TRY(fgk_fill_zero(p->gB, n64, p->s));
-        p->grids_dirty = 0;
-    }
+    if (p->grids_dirty) TRY(zero_grids(p));
     p->Np = Np;
     p->V = V;
-    if (!p->use_graph) return enqueue_work(p, x, y, z, Np, V, NULL);
-    if (!p->graph || p->kx != x || p->ky != y || p->kz != z || p->kNp != Np || p->kV != V) {
-        if (p->graph) {
+    /* grid set q for this catalog (enqueue_work reads p->parity); the next
+     * catalog takes the other set */
+    const int q = p->parity;
+    if (!p->use_graph) {
+        TRY(enqueue_work(p, x, y, z, Np, V, NULL));
+        if (p->gA2) p->parity = q ^ 1;
+        return 0;
+    }
+    if (!p->graph[q] || p->kx[q] != x || p->ky[q] != y || p->kz[q] != z || p->kNp[q] != Np ||
+        p->kV[q] != V) {
+        if (p->graph[q]) {
             TRY(fgk_stream_sync(p->s));
-            fgk_graph_destroy(p->graph);
-            p->graph = NULL;
+            fgk_graph_destroy(p->graph[q]);
+            p->graph[q] = NULL;
         }
         TRY(fgk_graph_begin(p->s));
         int rc = enqueue_work(p, x, y, z, Np, V, NULL);
@@ -411,14 +444,16 @@ int fftgal_power_enqueue(fftgal_power_t *p, const float *x, const float *y, cons
             return rc;
         }
         TRY(rc2);
-        p->graph = g;
-        p->kx = x;
-        p->ky = y;
-        p->kz = z;
-        p->kNp = Np;
-        p->kV = V;
+        p->graph[q] = g;
+        p->kx[q] = x;
+        p->ky[q] = y;
+        p->kz[q] = z;
+        p->kNp[q] = Np;
+        p->kV[q] = V;
     }
-    return fgk_graph_launch(p->graph, p->s);
+    TRY(fgk_graph_launch(p->graph[q], p->s));
+    if (p->gA2) p->parity = q ^ 1;
+    return 0;
 }
 
 int fftgal_power_sync(fftgal_power_t *p) { return fgk_stream_sync(p->s); }
@@ -435,13 +470,16 @@ int fftgal_power_paint_only(fftgal_power_t *p, const float *x, const float *y, c
     const size_t n64 = (size_t)p->d.Ng * p->d.Ng * p->d.Ng * sizeof(int64_t);
     TRY(fgk_fill_zero(p->gA, n64, p->s));
     if (p->d.interlace) TRY(fgk_fill_zero(p->gB, n64, p->s));
-    p->grids_dirty = 1;
+    p->grids_dirty = 1;          /* every grid is zeroed before the next enqueue */
+    fgk_paint_set_zero(p->paint, NULL, 0);
     TRY(fgk_paint_fixed(p->paint, x, y, z, Np, p->d.Ng / p->d.L, p->gA, p->gB, p->s));
     TRY(fgk_stream_sync(p->s));
     if (gridA) *gridA = p->gA;
     if (gridB) *gridB = p->gB;
     return 0;
 }
+
+int fftgal_power_grid_sets(const fftgal_power_t *p) { return p->gA2 ? 2 : 1; }
 
 int fftgal_power_fixed_bits(const fftgal_power_t *p) {
     return (int)lround(-log2(fgk_paint_fixed_scale(p->paint)));
@@ -496,12 +534,16 @@ static int tune_measure(fftgal_power_t *p, const float *x, const float *y, const
 
 /* switch a plan's FFT kernels: family (generic or specialised) and the
  * generic kernels' columns / rows per workgroup */
+static void drop_graphs(fftgal_power_t *p) {
+    for (int q = 0; q < 2; ++q) {
+        fgk_graph_destroy(p->graph[q]);
+        p->graph[q] = NULL;
+    }
+}
+
 static int tune_fft(fftgal_power_t *p, int generic, int cols, int rows) {
     TRY(fgk_stream_sync(p->s));
-    if (p->graph) {
-        fgk_graph_destroy(p->graph);
-        p->graph = NULL;
-    }
+    drop_graphs(p);
     TRY(fgk_fft_set_tuning(p->plan, cols, rows));
     TRY(fgk_fft_force_generic(p->plan, generic));
     p->tune.fft_generic = generic;
@@ -612,10 +654,7 @@ int fftgal_power_use_graph(fftgal_power_t *p, int on) {
 
 int fftgal_power_fft_generic(fftgal_power_t *p, int on) {
     TRY(fgk_stream_sync(p->s));
-    if (p->graph) {   /* the captured graph holds the other kernels */
-        fgk_graph_destroy(p->graph);
-        p->graph = NULL;
-    }
+    drop_graphs(p);   /* the captured graphs hold the other kernels */
     TRY(fgk_fft_force_generic(p->plan, on));
     return fgk_fft_f32_specialised(p->d.Ng) && !on ? 0 : 1;
 }

@@ -69,6 +69,10 @@ int fftgal_power_results(fftgal_power_t *p, double *K, long *N, double *Pl, doub
 int fftgal_power_paint_only(fftgal_power_t *p, const float *x, const float *y, const float *z,
                             long Np, void **gridA, void **gridB);
 int fftgal_power_fixed_bits(const fftgal_power_t *p);
+/* int64 grid sets of the plan: 2 when device memory allowed a second set at
+ * creation (catalogs alternate between the sets and each paint clears the
+ * other one, so the row FFT does not clear the grid it reads), else 1 */
+int fftgal_power_grid_sets(const fftgal_power_t *p);
 /* device address of the FP32 spectra (plain, dual) after an enqueue, in
  * the packed half-spectrum layout c[(i*Ng+j)*(Ng/2) + k] (complex float,
  * k < Ng/2, unnormalised forward transform of the fixed-point grid in

@@ -289,6 +289,50 @@ def test_graph_replays_match_eager(torch_cuda, lib):
     p.close()
 
 
+def test_consecutive_catalogs_match_fresh_plans(torch_cuda, lib):
+    """Catalogs run back to back through one plan (which may alternate
+    between two int64 grid sets, each paint clearing the other): graph and
+    eager runs, after paint_only and after a profile, each give the bins of
+    a fresh plan for that catalog."""
+    import fftgal_amd as fa
+    from fftgal_amd import mockgen
+    torch = torch_cuda
+    Ng, L = 64, 500.0
+    cats = {}
+    for name, (Np, seed) in {"A": (150000, 5), "B": (90000, 6)}.items():
+        (x, y, z), _ = mockgen.lognormal(Np, L, seed=seed, Ngs=64)
+        cats[name] = (_dev(torch, x, y, z), Np)
+    V = L ** 3
+    ref = {}
+    for name, (t, Np) in cats.items():
+        q = fa.PowerPlan(Ng, L, interlace=True, Nmu=5, Np_max=Np)
+        ref[name] = q.run(*[a.data_ptr() for a in t], Np, V)
+        q.close()
+
+    def check(out, name):
+        r = ref[name]
+        tol = 1e-12 * np.nanmax(np.abs(r["Pl"][:, 0]))
+        assert np.array_equal(out["N"], r["N"]) and np.array_equal(out["Nmu"], r["Nmu"])
+        assert np.allclose(out["Pl"], r["Pl"], rtol=1e-12, atol=tol, equal_nan=True), name
+        assert np.allclose(out["Pmu"], r["Pmu"], rtol=1e-12, atol=tol, equal_nan=True), name
+
+    p = fa.PowerPlan(Ng, L, interlace=True, Nmu=5, Np_max=150000)
+    for step in ("A", "B", "B", "A", "paint_only A", "B", "profile A", "A", "eager B",
+                 "eager A", "A", "B"):
+        name = step.split()[-1]
+        t, Np = cats[name]
+        ptr = [a.data_ptr() for a in t]
+        if step.startswith("paint_only"):
+            p.paint_only(*ptr, Np)
+            continue
+        if step.startswith("profile"):
+            p.profile(*ptr, Np, V)
+            continue
+        p.use_graph(not step.startswith("eager"))
+        check(p.run(*ptr, Np, V), name)
+    p.close()
+
+
 def test_config2_clustered_vs_oracle(torch_cuda, lib, orc):
     """2,000,000 float32 particles of the lognormal recipe (L = 1000 Mpc/h,
     256^3 seed grid), Ng = 256, interlaced, l = 0,2,4: the clustered
