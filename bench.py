@@ -273,6 +273,7 @@ def main_batch(args):
           file=sys.stderr, flush=True)
     fast = [b.results(j) for j in range(len(jobs))]
     prof = b.profile(cats)
+    sets = b.grid_sets()
     b.close()
     torch.cuda.empty_cache()
 
@@ -335,7 +336,7 @@ def main_batch(args):
                                f"l=0,2,4(,6) and {nmu} |mu|-wedges, dK=kF of each grid box",
                    "grids": {str(n): sum(1 for j in jobs if j["Ng"] == n)
                              for n in (128, 192, 256, 384)},
-                   "parallelism": "1 GPU"},
+                   "grid_sets": sets, "parallelism": "1 GPU"},
         "stages_ms": {k: round(v, 3) for k, v in prof.items()},
         "plain_fp64": {"value": round(plain_value, 2), "unit": "Mparticles/s",
                        "ms_total": round(t_plain * 1e3, 1),

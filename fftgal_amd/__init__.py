@@ -135,6 +135,7 @@ API = {
                                        C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     "fftgal_batch_use_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "fftgal_batch_tile": (C.c_int, [C.c_void_p, C.c_int]),
+    "fftgal_batch_grid_sets": (C.c_int, [C.c_void_p]),
     "fftgal_batch_stream": (C.c_void_p, [C.c_void_p]),
 }
 
@@ -563,6 +564,9 @@ class BatchPlan:
 
     def tile(self, job):
         return self.lib.fftgal_batch_tile(self.h, int(job))
+
+    def grid_sets(self):
+        return self.lib.fftgal_batch_grid_sets(self.h)
 
     def stream(self):
         return self.lib.fftgal_batch_stream(self.h)

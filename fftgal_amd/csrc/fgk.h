@@ -134,6 +134,7 @@ typedef struct fgk_paintjob_desc {
     long Np_max;
     int interlace;
     int64_t *gridA, *gridB;   /* zeroed int64 grids (gridB: interlace only) */
+    int64_t *gridA2, *gridB2; /* optional second set (NULL: one set) */
 } fgk_paintjob_desc;
 typedef struct fgk_paintbatch fgk_paintbatch;
 int  fgk_paint_batch_create(fgk_paintbatch **pb, int njobs, const fgk_paintjob_desc *jobs);
@@ -143,6 +144,9 @@ void fgk_paint_batch_destroy(fgk_paintbatch *pb);
 int  fgk_paint_batch_set(fgk_paintbatch *pb, const float *const *x, const float *const *y,
                          const float *const *z, const long *Np, fgk_stream_t s);
 int  fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t s);
+/* grid set the paint stage writes (1: gridA2/gridB2) and the memory its
+ * launches clear meanwhile (NULL: none; bytes a multiple of 16) */
+int  fgk_paint_batch_select(fgk_paintbatch *pb, int set, void *zero, size_t bytes);
 double fgk_paint_batch_fixed_scale(const fgk_paintbatch *pb);
 int  fgk_paint_batch_tile(const fgk_paintbatch *pb, int job);
 

@@ -210,6 +210,7 @@ struct PaintJob {
     long Np;
     KeyGeo g;
     unsigned long long *GA, *GB;
+    unsigned long long *GA2, *GB2;   // second grid set (set 1 of the paint launch)
     int nb1;             // coarse buckets (slots: nb1 * MAX_D2, MAX_D2 per bucket)
     int interlace;
 };
@@ -875,18 +876,19 @@ __device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, in
 // are then flushed.
 // zbuf (one plan): every launched workgroup first clears its share of
 // zwords words there.
+// zbuf (batch): the launches of one paint stage clear consecutive pieces.
 // Batched (jobs != nullptr): workgroup b paints item ji0[ja] + b of the
 // batch; the job owning it (ji0[job] <= item < ji0[job+1], ja <= job < jb)
-// supplies the geometry, grids and tile tables (tilebase, items + job * sl,
-// MAX_D2 slots per coarse bucket).
+// supplies the geometry, grids (set 0: GA/GB, set 1: GA2/GB2) and tile
+// tables (tilebase, items + job * sl, MAX_D2 slots per coarse bucket).
 template <int T, int PB>
 __global__ void __launch_bounds__(PAINT_THREADS, 8)
 k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ tilebase,
               const uint32_t *__restrict__ items, long nslots, KeyGeo g, int nb2, float wscale,
               int cap, int interlace, unsigned long long *__restrict__ GA,
               unsigned long long *__restrict__ GB, const PaintJob *__restrict__ jobs,
-              const long *__restrict__ ji0, int ja, int jb, long sl, uint64_t *__restrict__ zbuf,
-              long zwords) {
+              const long *__restrict__ ji0, int ja, int jb, long sl, int set,
+              uint64_t *__restrict__ zbuf, long zwords) {
     constexpr int SA = T + 3, SB = T + 4;
     if (zbuf) {   // this workgroup's slice of the memory to clear (16-byte stores)
         const long per = ((zwords + gridDim.x - 1) / gridDim.x + 1) & ~1l;
@@ -908,8 +910,8 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
         g = J.g;
         nb2 = MAX_D2;
         interlace = J.interlace;
-        GA = J.GA;
-        GB = J.GB;
+        GA = set ? J.GA2 : J.GA;
+        GB = set ? J.GB2 : J.GB;
     }
     long t, p0, p1;
     if (!item_range(it, tilebase, items, nslots, cap, t, p0, p1)) return;
@@ -1279,13 +1281,13 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
             hipLaunchKernelGGL((k_paint_tiles<TT, POS_BITS_WIDE>), grid, block, lds, s,       \
                                pp->sorted2, pp->tilebase, pp->items, pp->nslots, g, pp->nb2,  \
                                ws, pp->cap, pp->interlace, A, B, (const PaintJob *)nullptr,   \
-                               (const long *)nullptr, 0, 0, 0l, pp->zero_ptr,                 \
+                               (const long *)nullptr, 0, 0, 0l, 0, pp->zero_ptr,              \
                                pp->zero_words);                                               \
         else                                                                                  \
             hipLaunchKernelGGL((k_paint_tiles<TT, POS_BITS>), grid, block, lds, s,            \
                                pp->sorted2, pp->tilebase, pp->items, pp->nslots, g, pp->nb2,  \
                                ws, pp->cap, pp->interlace, A, B, (const PaintJob *)nullptr,   \
-                               (const long *)nullptr, 0, 0, 0l, pp->zero_ptr,                 \
+                               (const long *)nullptr, 0, 0, 0l, 0, pp->zero_ptr,              \
                                pp->zero_words);                                               \
     } while (0)
         switch (pp->T) {
@@ -1396,6 +1398,10 @@ struct fgk_paintbatch {
     uint32_t *tilebase, *items;
     uint64_t *sorted1, *sorted2;
     int table_ok;                  // djobs holds hjobs
+    int sets;                      // grid sets of every job (1 or 2)
+    int set;                       // set the paint stage writes
+    uint64_t *zero;                // cleared by the paint stage's launches
+    long zero_words;
 };
 
 extern "C" void fgk_paint_batch_destroy(fgk_paintbatch *pb) {
@@ -1485,6 +1491,15 @@ extern "C" int fgk_paint_batch_create(fgk_paintbatch **out, int njobs,
         J.g = g;
         J.GA = (unsigned long long *)d.gridA;
         J.GB = (unsigned long long *)(d.interlace ? d.gridB : nullptr);
+        J.GA2 = (unsigned long long *)d.gridA2;
+        J.GB2 = (unsigned long long *)(d.interlace ? d.gridB2 : nullptr);
+        if (q == 0) pb->sets = d.gridA2 ? 2 : 1;
+        if ((d.gridA2 != nullptr) != (pb->sets == 2) || (d.gridA2 && d.interlace && !d.gridB2)) {
+            fgk_set_error("paint batch: catalog %d: every catalog needs the same number of "
+                          "grid sets", pb->perm[q]);
+            fgk_paint_batch_destroy(pb);
+            return -2;
+        }
         J.nb1 = g.nc0 * g.nc1 * g.nc2;
         J.interlace = d.interlace ? 1 : 0;
         pb->interlace_any |= J.interlace;
@@ -1550,6 +1565,18 @@ extern "C" int fgk_paint_batch_create(fgk_paintbatch **out, int njobs,
         return -1;
     }
     *out = pb;
+    return 0;
+}
+
+extern "C" int fgk_paint_batch_select(fgk_paintbatch *pb, int set, void *zero, size_t bytes) {
+    if (set < 0 || set >= pb->sets || (zero && (bytes % 16 || ((uintptr_t)zero & 15)))) {
+        fgk_set_error("paint batch: grid set %d of %d, or clear region not 16-byte aligned", set,
+                      pb->sets);
+        return -2;
+    }
+    pb->set = set;
+    pb->zero = (uint64_t *)zero;
+    pb->zero_words = zero ? (long)(bytes / 8) : 0;
     return 0;
 }
 
@@ -1677,9 +1704,16 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
         return 0;
     case 6: {
         const float ws = (float)ldexp(1.0, pb->S);
+        // the region to clear, in even-word pieces proportional to each
+        // launch's work items
+        const long pairs = pb->zero_words / 2, all = pb->hi0[pb->sja[pb->nsides]];
         for (int i = 0; i < pb->nsides; ++i) {
             const int ja = pb->sja[i], jb = pb->sja[i + 1];
             const long nitems = pb->hi0[jb] - pb->hi0[ja];
+            const long w0 = 2 * (long)((double)pairs * pb->hi0[ja] / all),
+                       w1 = i + 1 == pb->nsides ? 2 * pairs
+                                                : 2 * (long)((double)pairs * pb->hi0[jb] / all);
+            uint64_t *zb = pb->zero && w1 > w0 ? pb->zero + w0 : nullptr;
             if (nitems <= 0) continue;
             const size_t lds = (size_t)paint_lds_bytes(pb->side[i], pb->interlace_any);
             const dim3 grid((unsigned)nitems), block(PAINT_THREADS);
@@ -1689,7 +1723,7 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
                        pb->items, 0l, g0, MAX_D2, ws, pb->cap, 0,                             \
                        (unsigned long long *)nullptr, (unsigned long long *)nullptr,          \
                        (const PaintJob *)pb->djobs, (const long *)pb->di0, ja, jb, BATCH_SL,  \
-                       (uint64_t *)nullptr, 0l)
+                       pb->set, zb, zb ? w1 - w0 : 0l)
             switch (pb->side[i]) {
             case 16: PAINT_TB(16); break;
             case 12: PAINT_TB(12); break;

@@ -184,6 +184,15 @@ int fftgal_power_destroy(fftgal_power_t *p) {
     return 0;
 }
 
+/* a second int64 grid set (catalogs alternate between the sets, each paint
+ * clears the other set): unless FFTGAL_GRID_SETS=1, when it leaves at least
+ * 4 GiB of device memory */
+static int want_second_set(size_t bytes) {
+    const char *e = getenv("FFTGAL_GRID_SETS");
+    if (e && atoi(e) == 1) return 0;
+    return fgk_mem_free() > bytes + ((size_t)4 << 30);
+}
+
 int fftgal_power_create(fftgal_power_t **out, const fftgal_power_desc *desc) {
     *out = NULL;
     if (!desc) { fgk_set_error("NULL descriptor"); return -2; }
@@ -281,8 +290,7 @@ int fftgal_power_create(fftgal_power_t **out, const fftgal_power_desc *desc) {
     }
     STEP(fgk_fill_zero(p->gA, n64, p->s));
     if (desc->interlace) STEP(fgk_fill_zero(p->gB, n64, p->s));
-    /* the second grid set, when it leaves at least 4 GiB of device memory */
-    if (!rc && fgk_mem_free() > (size_t)ngrids * n64 + ((size_t)4 << 30)) {
+    if (!rc && want_second_set((size_t)ngrids * n64)) {
         if (fgk_malloc((void **)&p->gA2, ngrids * n64) == 0)
             STEP(fgk_fill_zero(p->gA2, ngrids * n64, p->s));
         else
@@ -687,7 +695,8 @@ typedef struct batch_group {
     int Ng, interlace, n;        /* grids of side Ng; n catalogs */
     int *jobs;                   /* caller's catalog indices, in grid order */
     fgk_fftplan *plan;
-    int64_t *g;                  /* n * G int64 grids */
+    int64_t *g, *g2;             /* n * G int64 grids of set 0 and set 1 (NULL: one set) */
+    size_t goff;                 /* offset of g in the batch's grid sets, int64 words */
     float *f;                    /* n * G packed spectra */
     double *d_phase, *d_winv;
     fgk_binspec spec;            /* Ng, Nmu, los, do_pl (NK per catalog) */
@@ -717,19 +726,22 @@ struct fftgal_batch {
     size_t nbins;
     int tables_ok;
     int use_graph;
-    fgk_graph_t graph;
+    fgk_graph_t graph[2];        /* one per grid set */
+    int64_t *gset[2];            /* the int64 grids of every group, set 0 and set 1 */
+    size_t gwords;               /* int64 words of one set */
+    int parity;                  /* set of the next enqueue */
     int ran;                     /* an enqueue completed setup */
 };
 
 int fftgal_batch_destroy(fftgal_batch_t *b) {
     if (!b) return 0;
     if (b->s) fgk_stream_sync(b->s);
-    fgk_graph_destroy(b->graph);
+    fgk_graph_destroy(b->graph[0]);
+    fgk_graph_destroy(b->graph[1]);
     fgk_paint_batch_destroy(b->paint);
     for (int q = 0; q < b->ngroups; ++q) {
         batch_group *G = &b->grp[q];
         fgk_fft_plan_destroy(G->plan);
-        fgk_free(G->g);
         fgk_free(G->f);
         fgk_free(G->d_phase);
         fgk_free(G->d_winv);
@@ -745,6 +757,8 @@ int fftgal_batch_destroy(fftgal_batch_t *b) {
     }
     free(b->grp);
     free(b->job);
+    fgk_free(b->gset[0]);
+    fgk_free(b->gset[1]);
     fgk_free(b->d_bins);
     fgk_host_free(b->h_bins);
     if (b->s) fgk_stream_destroy(b->s);
@@ -770,9 +784,8 @@ static int group_setup(fftgal_batch_t *b, batch_group *G, const char *wisdom) {
     TRY(fgk_fft_plan_create(&G->plan, Ng, 4));
     TRY(fgk_fft_set_tuning(G->plan, t.fft_cols, t.fft_rows));
     if (t.fft_generic) TRY(fgk_fft_force_generic(G->plan, 1));
-    TRY(fgk_malloc((void **)&G->g, (size_t)G->n * NG * cells * sizeof(int64_t)));
+    G->g = b->gset[0] + G->goff;
     TRY(fgk_malloc((void **)&G->f, (size_t)G->n * NG * cells * sizeof(float)));
-    TRY(fgk_fill_zero(G->g, (size_t)G->n * NG * cells * sizeof(int64_t), b->s));
     TRY(fgk_malloc((void **)&G->d_phase, (size_t)3 * Ng * 2 * sizeof(double)));
     TRY(fgk_malloc((void **)&G->d_winv, (size_t)Ng * sizeof(double)));
     TRY(fgk_malloc((void **)&G->djob, (size_t)G->n * sizeof(fgk_binjob)));
@@ -886,7 +899,25 @@ int fftgal_batch_create(fftgal_batch_t **out, int njobs, const fftgal_job_desc *
     for (int j = 0; j < njobs && !rc; ++j) b->grp[b->job[j].group].jobs[b->job[j].slot] = j;
     STEP(fgk_malloc((void **)&b->d_bins, b->nbins * sizeof(double)));
     STEP(fgk_host_alloc((void **)&b->h_bins, b->nbins * sizeof(double)));
+    /* one block per grid set for the int64 grids of all groups (each group's
+     * part 256-byte aligned), so that one region clears a whole set */
+    for (int q = 0; q < b->ngroups; ++q) {
+        batch_group *G = &b->grp[q];
+        const size_t w = (size_t)G->n * (G->interlace ? 2 : 1) * G->Ng * G->Ng * G->Ng;
+        G->goff = b->gwords;
+        b->gwords += (w + 31) & ~(size_t)31;
+    }
+    STEP(fgk_malloc((void **)&b->gset[0], b->gwords * sizeof(int64_t)));
+    STEP(fgk_fill_zero(b->gset[0], b->gwords * sizeof(int64_t), b->s));
     for (int q = 0; q < b->ngroups && !rc; ++q) rc = group_setup(b, &b->grp[q], wisdom);
+    if (!rc && want_second_set(b->gwords * sizeof(int64_t))) {
+        if (fgk_malloc((void **)&b->gset[1], b->gwords * sizeof(int64_t)) == 0)
+            STEP(fgk_fill_zero(b->gset[1], b->gwords * sizeof(int64_t), b->s));
+        else
+            b->gset[1] = NULL;
+    }
+    for (int q = 0; q < b->ngroups && b->gset[1]; ++q)
+        b->grp[q].g2 = b->gset[1] + b->grp[q].goff;
     /* the paint batch writes each catalog's grids in its group */
     if (!rc) {
         fgk_paintjob_desc *pd = (fgk_paintjob_desc *)calloc((size_t)njobs, sizeof *pd);
@@ -904,6 +935,10 @@ int fftgal_batch_create(fftgal_batch_t **out, int njobs, const fftgal_job_desc *
             pd[j].interlace = J->d.interlace;
             pd[j].gridA = G->g + (size_t)J->slot * (G->interlace ? 2 : 1) * cells;
             pd[j].gridB = G->interlace ? pd[j].gridA + cells : NULL;
+            if (G->g2) {
+                pd[j].gridA2 = G->g2 + (size_t)J->slot * (G->interlace ? 2 : 1) * cells;
+                pd[j].gridB2 = G->interlace ? pd[j].gridA2 + cells : NULL;
+            }
         }
         if (!rc) rc = fgk_paint_batch_create(&b->paint, njobs, pd);
         free(pd);
@@ -971,7 +1006,12 @@ static int batch_tables(fftgal_batch_t *b, const float *const *x, const float *c
  * fftgal_power_stage_name; ev (NSTAGES+1 events or NULL) brackets them */
 static int batch_work(fftgal_batch_t *b, fgk_event_t *ev) {
     int st = 0;
+    const int set = b->parity;
     for (int q = 0; q < b->ngroups; ++q) b->grp[q].spec.fs_n1 = fgk_fft_fourstep(b->grp[q].plan);
+    /* with two grid sets the paint writes set `set` and clears the other */
+    if (b->gset[1])
+        TRY(fgk_paint_batch_select(b->paint, set, b->gset[set ^ 1],
+                                   b->gwords * sizeof(int64_t)));
 #define MARK() do { if (ev) TRY(fgk_event_record(ev[st], b->s)); ++st; } while (0)
     MARK();
     TRY(fgk_fill_zero(b->d_bins, b->nbins * sizeof(double), b->s));
@@ -983,8 +1023,8 @@ static int batch_work(fftgal_batch_t *b, fgk_event_t *ev) {
     MARK();
     for (int q = 0; q < b->ngroups; ++q) {
         batch_group *G = &b->grp[q];
-        TRY(fgk_fft_rows_f32_from_fixed(G->plan, G->f, G->g, in_scale, 1,
-                                        G->n * (G->interlace ? 2 : 1), b->s));
+        TRY(fgk_fft_rows_f32_from_fixed(G->plan, G->f, set ? G->g2 : G->g, in_scale,
+                                        G->g2 ? 0 : 1, G->n * (G->interlace ? 2 : 1), b->s));
     }
     for (int axis = 1; axis >= 0; --axis) {
         MARK();
@@ -1009,8 +1049,13 @@ static int batch_work(fftgal_batch_t *b, fgk_event_t *ev) {
 int fftgal_batch_enqueue(fftgal_batch_t *b, const float *const *x, const float *const *y,
                          const float *const *z, const long *Np, const double *V) {
     TRY(batch_tables(b, x, y, z, Np, V));
-    if (!b->use_graph) return batch_work(b, NULL);
-    if (!b->graph) {
+    const int q = b->parity;
+    if (!b->use_graph) {
+        TRY(batch_work(b, NULL));
+        if (b->gset[1]) b->parity = q ^ 1;
+        return 0;
+    }
+    if (!b->graph[q]) {
         TRY(fgk_graph_begin(b->s));
         int rc = batch_work(b, NULL);
         fgk_graph_t g = NULL;
@@ -1020,10 +1065,14 @@ int fftgal_batch_enqueue(fftgal_batch_t *b, const float *const *x, const float *
             return rc;
         }
         TRY(rc2);
-        b->graph = g;
+        b->graph[q] = g;
     }
-    return fgk_graph_launch(b->graph, b->s);
+    TRY(fgk_graph_launch(b->graph[q], b->s));
+    if (b->gset[1]) b->parity = q ^ 1;
+    return 0;
 }
+
+int fftgal_batch_grid_sets(const fftgal_batch_t *b) { return b->gset[1] ? 2 : 1; }
 
 int fftgal_batch_sync(fftgal_batch_t *b) { return fgk_stream_sync(b->s); }
 
@@ -1035,6 +1084,7 @@ int fftgal_batch_profile(fftgal_batch_t *b, const float *const *x, const float *
     int rc = 0;
     for (int i = 0; i <= NSTAGES && !rc; ++i) rc = fgk_event_create(&ev[i]);
     if (!rc) rc = batch_work(b, ev);
+    if (!rc && b->gset[1]) b->parity ^= 1;
     if (!rc) rc = fgk_stream_sync(b->s);
     for (int i = 0; i < NSTAGES && !rc; ++i) rc = fgk_event_elapsed_ms(&stage_ms[i], ev[i], ev[i + 1]);
     for (int i = 0; i <= NSTAGES; ++i) fgk_event_destroy(ev[i]);

@@ -71,7 +71,8 @@ int fftgal_power_paint_only(fftgal_power_t *p, const float *x, const float *y, c
 int fftgal_power_fixed_bits(const fftgal_power_t *p);
 /* int64 grid sets of the plan: 2 when device memory allowed a second set at
  * creation (catalogs alternate between the sets and each paint clears the
- * other one, so the row FFT does not clear the grid it reads), else 1 */
+ * other one, so the row FFT does not clear the grid it reads), else 1.
+ * FFTGAL_GRID_SETS=1 in the environment at creation keeps one set. */
 int fftgal_power_grid_sets(const fftgal_power_t *p);
 /* device address of the FP32 spectra (plain, dual) after an enqueue, in
  * the packed half-spectrum layout c[(i*Ng+j)*(Ng/2) + k] (complex float,
@@ -166,6 +167,8 @@ int fftgal_batch_profile(fftgal_batch_t *b, const float *const *x, const float *
 int fftgal_batch_use_graph(fftgal_batch_t *b, int on);
 /* paint tile side of catalog `job` */
 int fftgal_batch_tile(const fftgal_batch_t *b, int job);
+/* int64 grid sets of the batch, as fftgal_power_grid_sets */
+int fftgal_batch_grid_sets(const fftgal_batch_t *b);
 void *fftgal_batch_stream(fftgal_batch_t *b);
 
 /* ---- input ----

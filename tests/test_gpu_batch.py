@@ -54,7 +54,21 @@ def _catalogs(torch, specs):
     return out
 
 
-def test_batch_matches_own_plans(torch_cuda, lib):
+def _batch(fa, monkeypatch, sets, jobs, **kw):
+    """A batch with `sets` int64 grid sets (2: the paint clears the other
+    set; 1: the row FFT clears the grids)."""
+    if sets == 1:
+        monkeypatch.setenv("FFTGAL_GRID_SETS", "1")
+    else:
+        monkeypatch.delenv("FFTGAL_GRID_SETS", raising=False)
+    b = fa.BatchPlan(jobs, **kw)
+    monkeypatch.delenv("FFTGAL_GRID_SETS", raising=False)
+    assert b.grid_sets() == sets
+    return b
+
+
+@pytest.mark.parametrize("sets", [2, 1])
+def test_batch_matches_own_plans(torch_cuda, lib, monkeypatch, sets):
     """Uneven batch: grid sides 32/48/64/40 (specialised and generic FFT
     kernels), folds 1, 2 and 4 of one catalog, tiles 16 and 8, one catalog
     without interlacing, in one batch; each equals its own plan."""
@@ -77,7 +91,7 @@ def test_batch_matches_own_plans(torch_cuda, lib):
         j = dict(j)
         j["Np_max"] = cats[c][1]
         jobs.append(j)
-    b = fa.BatchPlan(jobs, los=los, Nmu=Nmu)
+    b = _batch(fa, monkeypatch, sets, jobs, los=los, Nmu=Nmu)
     args = [(cats[c][0][0].data_ptr(), cats[c][0][1].data_ptr(), cats[c][0][2].data_ptr(),
              cats[c][1], cats[c][2]) for c, _ in plan]
     res = b.run(args)
@@ -114,23 +128,32 @@ def test_batch_folded_vs_oracle(torch_cuda, lib, orc):
         assert err.max() <= 1e-5, (l, err.max())
 
 
-def test_batch_replays_and_new_catalogs(torch_cuda, lib):
-    """The captured graph replays; a new set of catalogs (other arrays,
-    other Np and V) updates the tables and replays the same graph.  Each
-    replay must equal the plans of its own catalogs."""
+@pytest.mark.parametrize("sets", [2, 1])
+def test_batch_replays_and_new_catalogs(torch_cuda, lib, monkeypatch, sets):
+    """The captured graphs replay; a new set of catalogs (other arrays,
+    other Np and V) updates the tables and replays the same graphs.  Graph
+    and eager runs and profiles interleave (with two grid sets each run
+    paints one set and clears the other).  Each run must equal the plans of
+    its own catalogs."""
     import fftgal_amd as fa
     torch = torch_cuda
     cats = _catalogs(torch, [(40000, 400.0, 11), (30000, 400.0, 12), (50000, 700.0, 13)])
     los, Nmu = (0.0, 1.0, 1.0), 3
     jobs = [{"Ng": 32, "L": 400.0, "Np_max": 50000}, {"Ng": 48, "L": 200.0, "Np_max": 50000}]
-    b = fa.BatchPlan(jobs, los=los, Nmu=Nmu)
+    b = _batch(fa, monkeypatch, sets, jobs, los=los, Nmu=Nmu)
 
     def args_of(c):
         t, Np, V, _ = cats[c]
         return (t[0].data_ptr(), t[1].data_ptr(), t[2].data_ptr(), Np, V)
 
-    for sel in ((0, 1), (0, 1), (2, 1), (1, 0)):
+    for mode, sel in (("graph", (0, 1)), ("graph", (0, 1)), ("graph", (2, 1)),
+                      ("profile", (0, 2)), ("graph", (1, 0)), ("eager", (2, 0)),
+                      ("graph", (0, 1)), ("eager", (1, 2)), ("graph", (2, 1))):
         args = [args_of(c) for c in sel]
+        if mode == "profile":
+            b.profile(args)
+            continue
+        b.use_graph(mode == "graph")
         res = b.run(args)
         torch.cuda.synchronize()
         for q, c in enumerate(sel):

@@ -289,11 +289,12 @@ def test_graph_replays_match_eager(torch_cuda, lib):
     p.close()
 
 
-def test_consecutive_catalogs_match_fresh_plans(torch_cuda, lib):
-    """Catalogs run back to back through one plan (which may alternate
-    between two int64 grid sets, each paint clearing the other): graph and
-    eager runs, after paint_only and after a profile, each give the bins of
-    a fresh plan for that catalog."""
+@pytest.mark.parametrize("sets", [2, 1])
+def test_consecutive_catalogs_match_fresh_plans(torch_cuda, lib, monkeypatch, sets):
+    """Catalogs run back to back through one plan, with two int64 grid sets
+    (catalogs alternate, each paint clears the other set) or one (the row
+    FFT clears it): graph and eager runs, after paint_only and after a
+    profile, each give the bins of a fresh plan for that catalog."""
     import fftgal_amd as fa
     from fftgal_amd import mockgen
     torch = torch_cuda
@@ -316,7 +317,11 @@ def test_consecutive_catalogs_match_fresh_plans(torch_cuda, lib):
         assert np.allclose(out["Pl"], r["Pl"], rtol=1e-12, atol=tol, equal_nan=True), name
         assert np.allclose(out["Pmu"], r["Pmu"], rtol=1e-12, atol=tol, equal_nan=True), name
 
+    if sets == 1:
+        monkeypatch.setenv("FFTGAL_GRID_SETS", "1")
     p = fa.PowerPlan(Ng, L, interlace=True, Nmu=5, Np_max=150000)
+    monkeypatch.delenv("FFTGAL_GRID_SETS", raising=False)
+    assert p.grid_sets() == sets
     for step in ("A", "B", "B", "A", "paint_only A", "B", "profile A", "A", "eager B",
                  "eager A", "A", "B"):
         name = step.split()[-1]
