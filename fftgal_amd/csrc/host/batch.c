@@ -505,7 +505,7 @@ int fftgal_power_tuning(const fftgal_power_t *p, int *paint_tile, int *fft_cols,
     if (fft_cols) *fft_cols = p->tune.fft_cols;
     if (fft_rows) *fft_rows = p->tune.fft_rows;
     if (fft_generic) *fft_generic = p->tune.fft_generic || !fgk_fft_f32_specialised(p->d.Ng);
-    if (bin_bk) *bin_bk = p->spec.bk ? p->spec.bk : 64;
+    if (bin_bk) *bin_bk = p->spec.bk ? p->spec.bk : 16;
     return 0;
 }
 
@@ -632,7 +632,7 @@ int fftgal_power_autotune(const fftgal_power_desc *desc, const float *x, const f
     }
     if (!rc) rc = tune_fft(p, best.fft_generic, best.fft_cols, best.fft_rows);
     float bbest = 1e30f;
-    static const int kBk[] = {64, 32, 16, 128};
+    static const int kBk[] = {16, 32, 64, 128};
     for (int i = 0; i < 4 && !rc; ++i) {
         if (kBk[i] != 64 && kBk[i] > Ng / 2) continue;
         p->spec.bk = kBk[i];

@@ -94,7 +94,7 @@ int fftgal_power_tuning(const fftgal_power_t *p, int *paint_tile, int *fft_cols,
  * enqueue takes them), the paint tile sides 16, 12, 8, 4, the FFT kernels
  * (compile-time specialised where the grid side has them, generic LDS
  * Stockham with its default, halved-column and halved-row launch shapes)
- * and the fused bin block (k extent 64, 32, 16, 128), one knob at a time in
+ * and the fused bin block (k extent 16, 32, 64, 128), one knob at a time in
  * pipeline order, the fastest of `reps` eager runs per candidate.  Stores
  * the winner for (desc->Ng, desc->interlace) in the wisdom file
  * desc->wisdom, the role FFTW's wisdom has for fft_init (src/fft.c:26-45),
