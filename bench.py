@@ -563,6 +563,12 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "kernel_ms": round(dom_ms, 4)}
+    # paint stencil updates: 64 cells per particle and grid (PCS), both grids
+    upd = 64 * 2 * Np
+    paint = {"stencil_updates": upd,
+             "updates_per_s": round(upd / (acc["paint_tiles"] * 1e-3), -6),
+             "kernel_ms": round(acc["paint_tiles"], 4),
+             "note": "int32 LDS atomics; the footprints then flush with int64 global atomics"}
     B = b_alg(Np, Ng)
     pipe = {"B_alg_bytes": B, "GBps": round(B / (ms_per_step * 1e-3) / 1e9, 1),
             "frac_of_peak": round(B / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -640,6 +646,7 @@ def main():
                                 "FP64 k-space math and bin sums",
                    "parallelism": "1 GPU"},
         "roofline": roofline,
+        "paint_stencil": paint,
         "cpu_baseline": cpu,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain_fp64": round(value / plain["value"], 2) if plain else None,
