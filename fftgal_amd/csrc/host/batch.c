@@ -540,8 +540,8 @@ static int tune_measure(fftgal_power_t *p, const float *x, const float *y, const
     return 0;
 }
 
-/* switch a plan's FFT kernels: family (generic or specialised) and the
- * generic kernels' columns / rows per workgroup */
+/* destroy the captured graphs (both grid sets): they hold the kernels and
+ * launch shapes of the tuning in use */
 static void drop_graphs(fftgal_power_t *p) {
     for (int q = 0; q < 2; ++q) {
         fgk_graph_destroy(p->graph[q]);
@@ -549,6 +549,8 @@ static void drop_graphs(fftgal_power_t *p) {
     }
 }
 
+/* switch a plan's FFT kernels: family (generic or specialised) and the
+ * generic kernels' columns / rows per workgroup */
 static int tune_fft(fftgal_power_t *p, int generic, int cols, int rows) {
     TRY(fgk_stream_sync(p->s));
     drop_graphs(p);
