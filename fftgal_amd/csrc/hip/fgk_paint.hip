@@ -544,12 +544,30 @@ k_digitscan(const uint32_t *__restrict__ v, long n, int mode, int cap,
     const long per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
     const long i0 = threadIdx.x * per;
     const long i1 = i0 + per < n ? i0 + per : n;
+    auto aux_of = [mode, cap](uint32_t c) -> uint32_t {
+        return mode == 1 ? ((c + SORT_CHUNK - 1) / SORT_CHUNK + RB - 1) / RB * RB
+                         : (c + cap - 1) / cap;
+    };
+    // up to RP entries per thread: loaded into registers together (one
+    // memory latency rather than one per entry)
+    constexpr int RP = 32;
+    uint32_t vals[RP];
+    const bool inreg = per <= RP;
     uint32_t ca = 0, cb = 0;
-    for (long i = i0; i < i1; ++i) {
-        const uint32_t c = v[i];
-        ca += c;
-        if (mode == 1) cb += ((c + SORT_CHUNK - 1) / SORT_CHUNK + RB - 1) / RB * RB;
-        else cb += (c + cap - 1) / cap;
+    if (inreg) {
+#pragma unroll
+        for (int q = 0; q < RP; ++q) vals[q] = i0 + q < i1 ? v[i0 + q] : 0u;
+#pragma unroll
+        for (int q = 0; q < RP; ++q) {
+            ca += vals[q];
+            cb += i0 + q < i1 ? aux_of(vals[q]) : 0u;
+        }
+    } else {
+        for (long i = i0; i < i1; ++i) {
+            const uint32_t c = v[i];
+            ca += c;
+            cb += aux_of(c);
+        }
     }
     sa[threadIdx.x] = ca;
     sb[threadIdx.x] = cb;
@@ -563,14 +581,25 @@ k_digitscan(const uint32_t *__restrict__ v, long n, int mode, int cap,
         __syncthreads();
     }
     uint32_t ra = sa[threadIdx.x] - ca, rb = sb[threadIdx.x] - cb;
-    for (long i = i0; i < i1; ++i) {
-        base[i] = ra;
-        aux[i] = rb;
-        if (aux_blk) aux_blk[i] = rb / RB;
-        const uint32_t c = v[i];
-        ra += c;
-        if (mode == 1) rb += ((c + SORT_CHUNK - 1) / SORT_CHUNK + RB - 1) / RB * RB;
-        else rb += (c + cap - 1) / cap;
+    if (inreg) {
+#pragma unroll
+        for (int q = 0; q < RP; ++q) {
+            if (i0 + q >= i1) break;
+            base[i0 + q] = ra;
+            aux[i0 + q] = rb;
+            if (aux_blk) aux_blk[i0 + q] = rb / RB;
+            ra += vals[q];
+            rb += aux_of(vals[q]);
+        }
+    } else {
+        for (long i = i0; i < i1; ++i) {
+            base[i] = ra;
+            aux[i] = rb;
+            if (aux_blk) aux_blk[i] = rb / RB;
+            const uint32_t c = v[i];
+            ra += c;
+            rb += aux_of(c);
+        }
     }
     if (threadIdx.x == SCAN_THREADS - 1) {
         base[n] = sa[SCAN_THREADS - 1];
