@@ -184,21 +184,6 @@ def _cpu_model():
     return "unknown"
 
 
-def batch_specs(nmocks):
-    """Mock parameters of the batched workload: per seed s (numpy PCG64
-    stream [s, 3]) L ~ U[250, 2000], Ng from {128, 192, 256, 384}, Np =
-    round(exp U[ln 5e5, ln 2e7])."""
-    import numpy as np
-    out = []
-    for s in range(nmocks):
-        r = np.random.default_rng([s, 3])
-        L = float(r.uniform(250.0, 2000.0))
-        Ng = int((128, 192, 256, 384)[r.integers(4)])
-        Np = int(round(np.exp(r.uniform(np.log(5e5), np.log(2e7)))))
-        out.append((s, L, Ng, Np))
-    return out
-
-
 def _batch_mock(spec):
     from fftgal_amd import mockgen
     s, L, Ng, Np = spec
@@ -212,6 +197,7 @@ def main_batch(args):
     against the plain FP64 form run catalog by catalog."""
     import multiprocessing as mp
     import numpy as np
+    from fftgal_amd.mockgen import batch_specs
     specs = batch_specs(args.mocks)
     t0 = time.perf_counter()
     # the mocks are drawn before anything touches the GPU (forked workers)

@@ -93,7 +93,8 @@ API = {
     "fftgal_power_paint_only": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_long, C.POINTER(C.c_void_p),
                                           C.POINTER(C.c_void_p)]),
-    "fftgal_power_fixed_bits": (C.c_int, [C.c_void_p]),
+    "fftgal_power_fixed_bits": (C.c_int, [C.c_long]),
+    "fftgal_power_fixed_scale": (C.c_double, []),
     "fftgal_power_grid_sets": (C.c_int, [C.c_void_p]),
     "fftgal_power_spectra": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p),
                                        C.POINTER(C.c_void_p)]),
@@ -372,7 +373,8 @@ class PowerPlan:
 
     def paint_only(self, x, y, z, Np):
         """Run only the paint stage; returns the int64 grids (plain, dual or
-        None) copied to host as [Ng, Ng, Ng] arrays in units of 2^-fixed_bits."""
+        None) copied to host as [Ng, Ng, Ng] arrays in units of fixed_scale()
+        particles."""
         a, b = C.c_void_p(), C.c_void_p()
         self._check(self.lib.fftgal_power_paint_only(self.h, x, y, z, int(Np), C.byref(a),
                                                      C.byref(b)), "fftgal_power_paint_only")
@@ -387,8 +389,13 @@ class PowerPlan:
             out.append(h)
         return tuple(out)
 
-    def fixed_bits(self):
-        return self.lib.fftgal_power_fixed_bits(self.h)
+    def fixed_bits(self, n):
+        """fraction bits of a paint tile whose work items hold n particles"""
+        return self.lib.fftgal_power_fixed_bits(int(n))
+
+    def fixed_scale(self):
+        """particle mass of one int64 grid unit (about 2^-25)"""
+        return self.lib.fftgal_power_fixed_scale()
 
     def grid_sets(self):
         """Number of int64 grid sets (2: catalogs alternate and each paint

@@ -108,7 +108,12 @@ void fgk_paint_plan_destroy(fgk_paintplan *pp);
 int  fgk_paint_fixed(fgk_paintplan *pp, const float *x, const float *y,
                      const float *z, long Np, double Hinv,
                      int64_t *gridA, int64_t *gridB, fgk_stream_t s);
-double fgk_paint_fixed_scale(const fgk_paintplan *pp);  /* 2^-S */
+/* Particle mass of one unit of the int64 grids: 1 / (216 C_25), within
+ * 1.1e-4 of 2^-25.  Each tile paints with the fixed point its particle
+ * count allows (fgk_paint_tile_bits) and is shifted to this unit. */
+double fgk_paint_fixed_scale(void);
+/* fraction bits S (19..25) of a tile whose work items hold n particles */
+int  fgk_paint_tile_bits(long n);
 /* Memory the paint kernel clears while it paints (bytes a multiple of 16,
  * 16-byte aligned; NULL: none): spread over the paint's workgroups, for a
  * caller that alternates between two grid sets and so needs the other set
@@ -147,7 +152,6 @@ int  fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t s);
 /* grid set the paint stage writes (1: gridA2/gridB2) and the memory its
  * launches clear meanwhile (NULL: none; bytes a multiple of 16) */
 int  fgk_paint_batch_select(fgk_paintbatch *pb, int set, void *zero, size_t bytes);
-double fgk_paint_batch_fixed_scale(const fgk_paintbatch *pb);
 int  fgk_paint_batch_tile(const fgk_paintbatch *pb, int job);
 
 /* ---- k space (fgk_kspace.hip) ---- */

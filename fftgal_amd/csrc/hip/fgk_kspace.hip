@@ -525,22 +525,6 @@ __device__ __forceinline__ double fused_power(double ar, double ai, double br, d
     return (dr * dr + di * di) * Vinv - Pshot;
 }
 
-// normalisations of the plain (na) and dual (nb) spectra: norm = V / Np
-// times Np over the grid's painted mass, Re of its packed (0, 0, 0) entry
-// (the DC mode).  The fixed-point paint rounds each of a particle's 64
-// weights to nearest, which paints slightly less than unit mass on average;
-// dividing by the painted mass removes that bias from delta.
-__device__ __forceinline__ void grid_norms(const float *A, const float *B, double norm,
-                                           double npart, double &na, double &nb) {
-    na = nb = norm;
-    const double ma = A[0];
-    if (ma > 0.0) na = norm * (npart / ma);
-    if (B) {
-        const double mb = B[0];
-        if (mb > 0.0) nb = norm * (npart / mb);
-    }
-}
-
 // k-bin range [lo, hi] of modes with |frequency| per axis in [lo_d, hi_d]
 __device__ __forceinline__ void bin_window(const BinArgs &a, int lo0, int hi0, int lo1, int hi1,
                                            int lo2, int hi2, int &lo, int &hi) {
@@ -673,8 +657,11 @@ k_bin(const double *__restrict__ fP, const float *__restrict__ A, const float *_
     int cnt[KPT], fk[KPT];
     bool kl[KPT];
     KFac kf[KPT];
-    double na = norm, nb = norm;
-    if (MODE == 1) grid_norms(A, B, norm, npart, na, nb);
+    // norm = V / Np (src/fft.c:125-127 with the H^3 of x2k), times 2^-S
+    // where the FFT read the fixed-point grids unscaled.  The fixed-point
+    // paint rounds without bias (dithered), so its expected mass is Np, as
+    // the reference's.
+    const double na = norm, nb = norm;
 #pragma unroll
     for (int q = 0; q < KPT; ++q) {
         const int k = kb + q;
@@ -782,8 +769,7 @@ k_bin_plane(const float *__restrict__ A, const float *__restrict__ B, BinArgs a,
     const BinOut o = make_out(a, sh, blo, bhi, blo2, bhi2, bins, nullptr);
     window_clear(a, o);
     const int i = i0 + threadIdx.x / PJ, j = j0 + threadIdx.x % PJ;
-    double na, nb;
-    grid_norms(A, B, norm, npart, na, nb);
+    const double na = norm, nb = norm;
     if (i < i1 && j < j1) {
         const int im = i ? Ng - i : 0, jm = j ? Ng - j : 0;
         const long m = ((long)iplane(a, i) * Ng + j) * Mh,

@@ -179,7 +179,7 @@ struct KeyGeo {
 };
 
 struct fgk_paintplan {
-    int Ng, T, nt, interlace, F, S, cap;
+    int Ng, T, nt, interlace, F, cap;
     int two_level;
     int wide;            // two levels of WIDE_D1 x WIDE_D2 digits
     uint32_t *bitem, *ibase, *bscratch;   // [nb1+1] wide mode: items per bucket, their scan
@@ -727,13 +727,17 @@ __global__ void k_scatter_global(const float *__restrict__ x, const float *__res
     out[base[t] + rank[p]] = pk;
 }
 
-__device__ __forceinline__ void pcs_w_f32(float d, float w[4]) {
+// Six times the PCS weights of src/fft.c:83-88 in FP32 (they sum to 6).
+// A term of the fixed-point paint is (6wx)(6wy)(6wz) times the tile's
+// integer unit C_S (below), so a particle paints 216 C_S units: the three
+// factors 1/6 never round, and the unit is exactly known.
+__device__ __forceinline__ void pcs_w6_f32(float d, float w[4]) {
     const float e = 1.0f - d;
     const float d2 = d * d;
-    w[0] = e * e * e * (1.0f / 6.0f);
-    w[1] = ((3.0f * d - 6.0f) * d2 + 4.0f) * (1.0f / 6.0f);
-    w[2] = (((-3.0f * d + 3.0f) * d + 3.0f) * d + 1.0f) * (1.0f / 6.0f);
-    w[3] = d2 * d * (1.0f / 6.0f);
+    w[0] = e * e * e;
+    w[1] = (3.0f * d - 6.0f) * d2 + 4.0f;
+    w[2] = ((-3.0f * d + 3.0f) * d + 3.0f) * d + 1.0f;
+    w[3] = d2 * d;
 }
 
 // o[q] = w[(q + r) & 3] as a two-level barrel shift with static indices
@@ -748,10 +752,63 @@ __device__ __forceinline__ void rot4(const float w[4], int r, float o[4]) {
     o[3] = r1 ? t1 : t3;
 }
 
-// round-to-nearest-even of a*b (|a*b| < 2^22) with one FMA: adding
-// 1.5*2^23 puts the rounded integer in the low mantissa bits
-__device__ __forceinline__ int fixed_rne(float a, float b) {
-    return __float_as_int(__builtin_fmaf(a, b, 12582912.0f)) - 0x4B400000;
+// Fixed point of the tile paint.  A tile whose work items hold at most n
+// particles (n = min(cap, particles in the tile)) paints in units of
+// 1 / (216 C_S) particle, C_S = 2427 * 2^(S - 19) (2427 = floor(2^19 / 216)),
+// with the largest S <= 25 for which n * (64 C_S + 1) < 2^31: a footprint
+// cell gets at most one term of at most 64 C_S + 1 units per particle, so
+// the int32 LDS sums cannot overflow.  Dense tiles (8192 particles) get
+// S = 19, tiles of at most 216 particles S = 25.  The int64 grids hold units
+// of 1 / (216 C_25) particle; a tile's footprint is shifted left by 25 - S
+// when it is flushed, exactly.  S depends only on the tile's particle count,
+// so the grids stay bit-identical run to run.
+constexpr int FIX_S0 = 19, FIX_SMAX = 25;
+constexpr float FIX_C0 = 2427.0f;
+__host__ __device__ constexpr long fix_limit(int S) {
+    return (2147483647l - 1) / (64l * (2427l << (S - FIX_S0)) + 1);
+}
+
+__device__ __forceinline__ int tile_bits(long n) {
+    int S = FIX_SMAX;
+    while (S > FIX_S0 && n > fix_limit(S)) --S;
+    return S;
+}
+
+// floor(a*b + u) for a*b >= 0, u in [0, 1), a*b + 1 < 2^24: one FMA and
+// one truncating convert (truncation is floor for non-negative values).
+// With u uniform and independent of a*b this rounds a*b up with probability
+// frac(a*b) and down otherwise, so the expected rounded value is a*b
+// exactly: no term, however small, is lost on average.
+__device__ __forceinline__ int fixed_dither(float a, float b, float u) {
+    return (int)__builtin_fmaf(a, b, u);
+}
+
+// Dither of one stencil row (the 4 cells along k of one (i, j) node pair):
+// frac(u0 + 97 wab) with u0 the particle's dither base and wab the row's
+// x-y weight product (at most 16), so the rows of a particle get different,
+// effectively independent offsets from a function of the particle and the
+// row alone (not of the lane or the visiting order).  Sharing one offset
+// among all 64 terms instead makes their rounding errors add coherently:
+// about 17 units of noise per particle at low k against 4.9 per row.
+__device__ __forceinline__ float row_dither(float u0, float wab) {
+    return __builtin_amdgcn_fractf(__builtin_fmaf(wab, 97.0f, u0));
+}
+
+// Dither bases of one particle for its plain and dual stencils: two 24-bit
+// uniforms in [0, 1) from a 32-bit finaliser hash (MurmurHash3's fmix32,
+// Appleby 2011) of its packed sort word and its tile slot (the word holds
+// only tile-local coordinates).  They depend only on the particle's
+// quantised position, so the grids stay bit-identical run to run.
+__device__ __forceinline__ void dither_pair(uint64_t pk, uint32_t tile, float &ua, float &ub) {
+    uint32_t h = (uint32_t)pk ^ ((uint32_t)(pk >> 32) * 0x9E3779B9u) ^ (tile * 0x7FEB352Du);
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    const uint32_t h2 = (h ^ (h >> 15)) * 0x2C1B3C6Du;
+    ua = (float)(h >> 8) * 0x1p-24f;
+    ub = (float)(h2 >> 8) * 0x1p-24f;
 }
 
 // LDS footprints are stored in 4x4x4 bricks: cell (i,j,k) of a footprint
@@ -772,7 +829,7 @@ constexpr int brick_words(int side) { return brick_n(side) * brick_n(side) * bri
 // side cells past the grid's last index).
 template <int SIDE>
 __device__ void flush_footprint(const int *__restrict__ fp, int oi, int oj, int ok, int Ng,
-                                unsigned long long *__restrict__ G) {
+                                int shift, unsigned long long *__restrict__ G) {
     constexpr int RPG = 64 / SIDE;             // rows per wave step
     constexpr int NBK = brick_n(SIDE);
     typedef __attribute__((address_space(1))) unsigned long long gu64;
@@ -798,7 +855,7 @@ __device__ void flush_footprint(const int *__restrict__ fp, int oi, int oj, int 
         // Ng < 2^16: the row index fits 32 bits, one 32 x 32 -> 64 multiply-add
         const uint32_t row = (uint32_t)gi * (uint32_t)Ng + (uint32_t)gj;
         __hip_atomic_fetch_add(g + ((uint64_t)row * (uint32_t)Ng + (uint32_t)gk),
-                               (unsigned long long)(long long)v, __ATOMIC_RELAXED,
+                               (unsigned long long)((long long)v << shift), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -845,7 +902,7 @@ __device__ __forceinline__ void brick_axis(int base, int rl, int stride, int uni
 template <int NBK>
 __device__ __forceinline__ void stencil_lds(int *__restrict__ fp, int bi, int bj, int bk,
                                             const float wx[4], const float wy[4],
-                                            const float wzs[4], int lane) {
+                                            const float wzs[4], float u, int lane) {
     float xr[4], yr[4], zr[4];
     int ox[4], oy[4], oz[4];
     brick_axis(bi, (lane >> 4) & 3, NBK * NBK * 64, 16, wx, ox, xr);
@@ -856,9 +913,10 @@ __device__ __forceinline__ void stencil_lds(int *__restrict__ fp, int bi, int bj
 #pragma unroll
         for (int bb = 0; bb < 4; ++bb) {
             const float wab = xr[aa] * yr[bb];
+            const float ur = row_dither(u, wab);
             int *row = fp + ox[aa] + oy[bb];
 #pragma unroll
-            for (int cc = 0; cc < 4; ++cc) atomicAdd(row + oz[cc], fixed_rne(wab, zr[cc]));
+            for (int cc = 0; cc < 4; ++cc) atomicAdd(row + oz[cc], fixed_dither(wab, zr[cc], ur));
         }
     }
 }
@@ -879,7 +937,7 @@ __device__ __forceinline__ uint32_t pk_axis(uint64_t pk, int a) {
 // DUAL = 1: dual grid (offset 1/2), base li or li+1, origin tile origin - 2
 template <int NBK, int DUAL, int PB>
 __device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, int F, float fsc,
-                                           float wscale, int lane) {
+                                           float unit, float u, int lane) {
     int li[3];
     float d[3];
 #pragma unroll
@@ -891,12 +949,12 @@ __device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, in
         }
     }
     float wx[4], wy[4], wz[4], wzs[4];
-    pcs_w_f32(d[0], wx);
-    pcs_w_f32(d[1], wy);
-    pcs_w_f32(d[2], wz);
+    pcs_w6_f32(d[0], wx);
+    pcs_w6_f32(d[1], wy);
+    pcs_w6_f32(d[2], wz);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * wscale;
-    stencil_lds<NBK>(fp, li[0], li[1], li[2], wx, wy, wzs, lane);
+    for (int c = 0; c < 4; ++c) wzs[c] = wz[c] * unit;
+    stencil_lds<NBK>(fp, li[0], li[1], li[2], wx, wy, wzs, u, lane);
 }
 
 // Paint of one work item (<= cap particles of one tile): each thread takes
@@ -913,7 +971,7 @@ __device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, in
 template <int T, int PB>
 __global__ void __launch_bounds__(PAINT_THREADS, 8)
 k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ tilebase,
-              const uint32_t *__restrict__ items, long nslots, KeyGeo g, int nb2, float wscale,
+              const uint32_t *__restrict__ items, long nslots, KeyGeo g, int nb2,
               int cap, int interlace, unsigned long long *__restrict__ GA,
               unsigned long long *__restrict__ GB, const PaintJob *__restrict__ jobs,
               const long *__restrict__ ji0, int ja, int jb, long sl, int set,
@@ -952,10 +1010,16 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
     const int F = g.F;
     const float fsc = 1.0f / (float)(1 << F);
     const int lane = __lane_id();
+    // the tile's fixed point, from its particle count (all its items agree)
+    const long ntile = (long)tilebase[t + 1] - (long)tilebase[t];
+    const int S = tile_bits(ntile < cap ? ntile : cap);
+    const float unit = FIX_C0 * (float)(1 << (S - FIX_S0));
     for (long p = p0 + threadIdx.x; p < p1; p += PAINT_THREADS) {
         const uint64_t pk = sorted[p];
-        paint_grid<brick_n(SA), 0, PB>(fa, pk, F, fsc, wscale, lane);
-        if (interlace) paint_grid<brick_n(SB), 1, PB>(fb, pk, F, fsc, wscale, lane);
+        float ua, ub;
+        dither_pair(pk, (uint32_t)t, ua, ub);
+        paint_grid<brick_n(SA), 0, PB>(fa, pk, F, fsc, unit, ua, lane);
+        if (interlace) paint_grid<brick_n(SB), 1, PB>(fb, pk, F, fsc, unit, ub, lane);
     }
     __syncthreads();
     const long b1 = t / nb2, f = t - b1 * nb2;
@@ -964,8 +1028,9 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
     const int fx = (int)(f / (g.cg1 * g.cg2)), fy = (int)((f / g.cg2) % g.cg1),
               fz = (int)(f % g.cg2);
     const int ti = cx * g.cg0 + fx, tj = cy * g.cg1 + fy, tk = cz * g.cg2 + fz;
-    flush_footprint<SA>(fa, ti * T - 1, tj * T - 1, tk * T - 1, g.Ng, GA);
-    if (interlace) flush_footprint<SB>(fb, ti * T - 2, tj * T - 2, tk * T - 2, g.Ng, GB);
+    flush_footprint<SA>(fa, ti * T - 1, tj * T - 1, tk * T - 1, g.Ng, FIX_SMAX - S, GA);
+    if (interlace)
+        flush_footprint<SB>(fb, ti * T - 2, tj * T - 2, tk * T - 2, g.Ng, FIX_SMAX - S, GB);
 }
 
 // ------------------------------------------------------------------ host
@@ -994,17 +1059,15 @@ static int paint_side(int Ng, int tile) {
     return 1;
 }
 
-// fixed-point fraction bits S of the weights for at most cap particles per
-// work item:
-// largest S with cap * (2/3)^3 * 2^S < 2^31 (int32 LDS sums cannot
-// overflow) and every rounded weight below 2^22 (single-FMA rounding)
-static int fixed_bits(int cap) {
-    int S = 0;
-    while ((double)cap * (8.0 / 27.0) * ldexp(1.0, S + 1) < 2147483647.0 &&
-           (8.0 / 27.0) * ldexp(1.0, S + 1) < 4194304.0)
-        ++S;
-    return S;
-}
+// fixed-point fraction bits S of the weights for a catalog whose work
+// items hold at most n particles (n = min(cap, Np)): the largest S <= 25
+// with n * ((2/3)^3 2^S + 1) < 2^31 (a footprint cell gets at most one
+// dithered term of at most (2/3)^3 2^S + 1 per particle, so the int32 LDS
+// sums cannot overflow) and (2/3)^3 2^S + 1 < 2^24 (FP32 holds every
+// dithered sum exactly).  S = 19 for full 8192-particle items, 25 for a
+// single particle.
+// particle mass of one int64 grid unit: 1 / (216 C_25), exact in FP64
+static double fixed_scale(void) { return 1.0 / (216.0 * (double)FIX_C0 * (double)(1 << (FIX_SMAX - FIX_S0))); }
 
 // Tile geometry of a grid of side Ng painted in tiles of side T: coarse
 // buckets group up to d2 tiles, doubling the group along the axis with the
@@ -1054,8 +1117,6 @@ extern "C" int fgk_paint_plan_create(fgk_paintplan **out, int Ng, int tile, long
     pp->T = T;
     pp->interlace = interlace ? 1 : 0;
     pp->cap = 8192;
-    // no work item holds more than Np_max particles
-    pp->S = fixed_bits(Np_max < pp->cap ? (int)Np_max : pp->cap);
     pp->Np_max = Np_max;
     int mode = 0;
     pp->g = tile_geometry(Ng, T, 1, mode);
@@ -1132,7 +1193,13 @@ extern "C" void fgk_paint_plan_destroy(fgk_paintplan *pp) {
     free(pp);
 }
 
-extern "C" double fgk_paint_fixed_scale(const fgk_paintplan *pp) { return ldexp(1.0, -pp->S); }
+extern "C" double fgk_paint_fixed_scale(void) { return fixed_scale(); }
+
+extern "C" int fgk_paint_tile_bits(long n) {
+    int S = FIX_SMAX;
+    while (S > FIX_S0 && n > fix_limit(S)) --S;
+    return S;
+}
 extern "C" int fgk_paint_plan_tile(const fgk_paintplan *pp) { return pp->T; }
 
 static const char *const kPaintStages[] = {"sort1_count", "sort1_scan", "sort1_scatter",
@@ -1302,20 +1369,19 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
         const long max_items = pp->nslots + (Np + pp->cap - 1) / pp->cap;
         const size_t lds = (size_t)paint_lds_bytes(pp->T, pp->interlace);
         const dim3 grid((unsigned)max_items), block(PAINT_THREADS);
-        const float ws = (float)ldexp(1.0, pp->S);
         unsigned long long *A = (unsigned long long *)gridA, *B = (unsigned long long *)gridB;
 #define PAINT_T(TT)                                                                           \
     do {                                                                                      \
         if (pp->wide)                                                                         \
             hipLaunchKernelGGL((k_paint_tiles<TT, POS_BITS_WIDE>), grid, block, lds, s,       \
                                pp->sorted2, pp->tilebase, pp->items, pp->nslots, g, pp->nb2,  \
-                               ws, pp->cap, pp->interlace, A, B, (const PaintJob *)nullptr,   \
+                               pp->cap, pp->interlace, A, B, (const PaintJob *)nullptr,       \
                                (const long *)nullptr, 0, 0, 0l, 0, pp->zero_ptr,              \
                                pp->zero_words);                                               \
         else                                                                                  \
             hipLaunchKernelGGL((k_paint_tiles<TT, POS_BITS>), grid, block, lds, s,            \
                                pp->sorted2, pp->tilebase, pp->items, pp->nslots, g, pp->nb2,  \
-                               ws, pp->cap, pp->interlace, A, B, (const PaintJob *)nullptr,   \
+                               pp->cap, pp->interlace, A, B, (const PaintJob *)nullptr,       \
                                (const long *)nullptr, 0, 0, 0l, 0, pp->zero_ptr,              \
                                pp->zero_words);                                               \
     } while (0)
@@ -1412,7 +1478,7 @@ k_digitscan_jobs(const uint32_t *__restrict__ T2, const uint32_t *__restrict__ b
 }
 
 struct fgk_paintbatch {
-    int njobs, S, cap;
+    int njobs, cap;
     long NW1, NW2, Np_tot;
     int nsides, side[6], sja[7];   // jobs [sja[i], sja[i+1]) paint with tile side side[i]
     int *perm;                     // internal job -> caller's job
@@ -1463,11 +1529,6 @@ extern "C" int fgk_paint_batch_create(fgk_paintbatch **out, int njobs,
     if (!pb) { fgk_set_error("out of host memory"); return -1; }
     pb->njobs = njobs;
     pb->cap = 8192;
-    {   // no work item holds more particles than the largest catalog
-        long npm = 1;
-        for (int j = 0; j < njobs; ++j) npm = jobs[j].Np_max > npm ? jobs[j].Np_max : npm;
-        pb->S = fixed_bits(npm < pb->cap ? (int)npm : pb->cap);
-    }
     pb->perm = (int *)calloc((size_t)njobs, sizeof(int));
     pb->hjobs = (PaintJob *)calloc((size_t)njobs, sizeof(PaintJob));
     pb->hNpmax = (long *)calloc((size_t)njobs, sizeof(long));
@@ -1609,9 +1670,6 @@ extern "C" int fgk_paint_batch_select(fgk_paintbatch *pb, int set, void *zero, s
     return 0;
 }
 
-extern "C" double fgk_paint_batch_fixed_scale(const fgk_paintbatch *pb) {
-    return ldexp(1.0, -pb->S);
-}
 
 extern "C" int fgk_paint_batch_tile(const fgk_paintbatch *pb, int job) {
     for (int q = 0; q < pb->njobs; ++q)
@@ -1732,7 +1790,6 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
         FGK_LAUNCHED();
         return 0;
     case 6: {
-        const float ws = (float)ldexp(1.0, pb->S);
         // the region to clear, in even-word pieces proportional to each
         // launch's work items
         const long pairs = pb->zero_words / 2, all = pb->hi0[pb->sja[pb->nsides]];
@@ -1749,7 +1806,7 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
 #define PAINT_TB(TT)                                                                          \
     hipLaunchKernelGGL((k_paint_tiles<TT, POS_BITS>), grid, block, lds, s, pb->sorted2,      \
                        pb->tilebase,                                                          \
-                       pb->items, 0l, g0, MAX_D2, ws, pb->cap, 0,                             \
+                       pb->items, 0l, g0, MAX_D2, pb->cap, 0,                                 \
                        (unsigned long long *)nullptr, (unsigned long long *)nullptr,          \
                        (const PaintJob *)pb->djobs, (const long *)pb->di0, ja, jb, BATCH_SL,  \
                        pb->set, zb, zb ? w1 - w0 : 0l)

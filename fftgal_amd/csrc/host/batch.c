@@ -337,7 +337,9 @@ static int enqueue_work(fftgal_power_t *p, const float *x, const float *y, const
     const double Hinv = Ng / p->d.L;
     double norm, Vinv, Pshot;
     catalog_norms(Ng, p->d.L, Np, V, &norm, &Vinv, &Pshot);
-    const double in_scale = fgk_paint_fixed_scale(p->paint);
+    /* the FFT reads the int64 grids unscaled; the particle mass of one unit
+     * goes into norm */
+    norm *= fgk_paint_fixed_scale();
     p->spec.fs_n1 = fgk_fft_fourstep(p->plan);   /* the tuner may switch FFT kernels */
     const size_t Ng3 = (size_t)Ng * Ng * Ng;
     const int ngrids = p->d.interlace ? 2 : 1;
@@ -360,7 +362,7 @@ static int enqueue_work(fftgal_power_t *p, const float *x, const float *y, const
         TRY(fgk_paint_stage(p->paint, stage, x, y, z, Np, Hinv, gA, gB, p->s));
     }
     MARK();
-    TRY(fgk_fft_rows_f32_from_fixed(p->plan, p->fA, gA, in_scale, p->gA2 ? 0 : 1, ngrids, p->s));
+    TRY(fgk_fft_rows_f32_from_fixed(p->plan, p->fA, gA, 1.0, p->gA2 ? 0 : 1, ngrids, p->s));
     MARK();
     TRY(fgk_fft_cols_f32(p->plan, p->fA, 1, -1, ngrids, p->s));
     MARK();
@@ -489,9 +491,9 @@ int fftgal_power_paint_only(fftgal_power_t *p, const float *x, const float *y, c
 
 int fftgal_power_grid_sets(const fftgal_power_t *p) { return p->gA2 ? 2 : 1; }
 
-int fftgal_power_fixed_bits(const fftgal_power_t *p) {
-    return (int)lround(-log2(fgk_paint_fixed_scale(p->paint)));
-}
+int fftgal_power_fixed_bits(long n) { return fgk_paint_tile_bits(n); }
+
+double fftgal_power_fixed_scale(void) { return fgk_paint_fixed_scale(); }
 
 int fftgal_power_spectra(fftgal_power_t *p, void **A, void **B) {
     if (A) *A = p->fA;
@@ -993,6 +995,8 @@ static int batch_tables(fftgal_batch_t *b, const float *const *x, const float *c
             const long n = Np[G->jobs[k]];
             const double v = V[G->jobs[k]];
             catalog_norms(G->Ng, J->d.L, n, v, &h->norm, &h->Vinv, &h->Pshot);
+            /* the FFT reads the int64 grids unscaled */
+            h->norm *= fgk_paint_fixed_scale();
             h->npart = (double)n;
             J->Np = n;
             J->V = v;
@@ -1021,11 +1025,10 @@ static int batch_work(fftgal_batch_t *b, fgk_event_t *ev) {
         MARK();
         TRY(fgk_paint_batch_stage(b->paint, stage, b->s));
     }
-    const double in_scale = fgk_paint_batch_fixed_scale(b->paint);
     MARK();
     for (int q = 0; q < b->ngroups; ++q) {
         batch_group *G = &b->grp[q];
-        TRY(fgk_fft_rows_f32_from_fixed(G->plan, G->f, set ? G->g2 : G->g, in_scale,
+        TRY(fgk_fft_rows_f32_from_fixed(G->plan, G->f, set ? G->g2 : G->g, 1.0,
                                         G->g2 ? 0 : 1, G->n * (G->interlace ? 2 : 1), b->s));
     }
     for (int axis = 1; axis >= 0; --axis) {

@@ -17,6 +17,10 @@ lognormal(Np, L, seed, Ngs=256, ...)
          displacement psi_z(k) = i k_z / k^2 delta_G(k):  z += f psi_z[cell];
       5. wrapped into [0, L), returned in a random order, float32 by default.
     Deterministic for a given seed (numpy PCG64 and pocketfft).
+
+batch_specs(nmocks)
+    Parameters of the batched uneven workload: per seed s, L ~ U[250, 2000],
+    Ng from {128, 192, 256, 384}, Np log-uniform in [5e5, 2e7].
 """
 import numpy as np
 
@@ -86,3 +90,17 @@ def lognormal(Np, L, seed, Ngs=256, A=2e4, k0=0.1, n=-1.5, f=0.76, rsd=True,
     for a in out:
         a[a >= L] -= L
     return out, sigma2
+
+
+def batch_specs(nmocks):
+    """(seed, L, Ng, Np) of mocks 0..nmocks-1 of the batched workload: per
+    seed s (numpy PCG64 stream [s, 3]) L ~ U[250, 2000], Ng uniform from
+    {128, 192, 256, 384}, Np = round(exp U[ln 5e5, ln 2e7])."""
+    out = []
+    for s in range(nmocks):
+        r = np.random.default_rng([s, 3])
+        L = float(r.uniform(250.0, 2000.0))
+        Ng = int((128, 192, 256, 384)[r.integers(4)])
+        Np = int(round(np.exp(r.uniform(np.log(5e5), np.log(2e7)))))
+        out.append((s, L, Ng, Np))
+    return out

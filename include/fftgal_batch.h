@@ -65,10 +65,19 @@ int fftgal_power_results(fftgal_power_t *p, double *K, long *N, double *Pl, doub
 
 /* ---- stage access for profiling and tests ---- */
 /* device address of the int64 grids (plain, dual) after the paint stage,
- * layout [Ng][Ng][Ng], units 2^-fftgal_power_fixed_bits() particles */
+ * layout [Ng][Ng][Ng], in units of fftgal_power_fixed_scale() particles */
 int fftgal_power_paint_only(fftgal_power_t *p, const float *x, const float *y, const float *z,
                             long Np, void **gridA, void **gridB);
-int fftgal_power_fixed_bits(const fftgal_power_t *p);
+/* Fixed point of the paint.  Each paint tile rounds its particles'
+ * weights to 1 / (216 C_S) particle, C_S = 2427 * 2^(S-19), with S (19 to
+ * 25) the most fraction bits its particle count allows without int32
+ * overflow (fftgal_power_fixed_bits(n) for work items of n particles; 19
+ * for 8192 or more, 25 for at most 216), with a deterministic dither that
+ * makes every rounding unbiased.  The int64 grids hold units of
+ * fftgal_power_fixed_scale() = 1 / (216 C_25) particle (within 1.1e-4 of
+ * 2^-25); coarser tiles are shifted to it exactly. */
+int fftgal_power_fixed_bits(long n);
+double fftgal_power_fixed_scale(void);
 /* int64 grid sets of the plan: 2 when device memory allowed a second set at
  * creation (catalogs alternate between the sets and each paint clears the
  * other one, so the row FFT does not clear the grid it reads), else 1.
@@ -76,8 +85,9 @@ int fftgal_power_fixed_bits(const fftgal_power_t *p);
 int fftgal_power_grid_sets(const fftgal_power_t *p);
 /* device address of the FP32 spectra (plain, dual) after an enqueue, in
  * the packed half-spectrum layout c[(i*Ng+j)*(Ng/2) + k] (complex float,
- * k < Ng/2, unnormalised forward transform of the fixed-point grid in
- * particle units): column k = 0 holds Z = F(X0) + i F(XN), from which
+ * k < Ng/2, unnormalised forward transform of the int64 fixed-point grid
+ * in its own units, fftgal_power_fixed_scale() particles each): column
+ * k = 0 holds Z = F(X0) + i F(XN), from which
  * delta(i,j,0) = (Z[i,j] + conj Z[-i,-j]) / 2 and
  * delta(i,j,Ng/2) = (Z[i,j] - conj Z[-i,-j]) / 2i.  Grids of Ng >= 2048
  * (Ng a multiple of 64) run their i-axis pass as a four-step 64 x Ng/64,

@@ -25,7 +25,47 @@
 #include <string.h>
 #include <math.h>
 #include <assert.h>
+#include <pthread.h>
 #include "oracle.h"
+
+static int orc_nthreads = 1;
+
+void oracle_set_threads(int n) { orc_nthreads = n < 1 ? 1 : (n > 256 ? 256 : n); }
+int oracle_threads(void) { return orc_nthreads; }
+
+typedef struct orc_range {
+    void (*fn)(void *, long, long);
+    void *arg;
+    long lo, hi;
+} orc_range;
+
+static void *orc_range_run(void *p) {
+    orc_range *r = (orc_range *)p;
+    r->fn(r->arg, r->lo, r->hi);
+    return NULL;
+}
+
+void orc_parallel(long n, void (*fn)(void *arg, long lo, long hi), void *arg) {
+    int T = orc_nthreads;
+    if (T > n) T = n > 0 ? (int)n : 1;
+    if (T <= 1) {
+        fn(arg, 0, n);
+        return;
+    }
+    orc_range r[256];
+    pthread_t th[256];
+    for (int t = 0; t < T; ++t) {
+        r[t].fn = fn;
+        r[t].arg = arg;
+        r[t].lo = n * t / T;
+        r[t].hi = n * (t + 1) / T;
+    }
+    for (int t = 1; t < T; ++t)
+        assert(pthread_create(&th[t], NULL, orc_range_run, &r[t]) == 0);
+    orc_range_run(&r[0]);
+    for (int t = 1; t < T; ++t)
+        pthread_join(th[t], NULL);
+}
 
 int offt64_plan_init(offt64_plan *p, int n) {
     assert(n >= 1);
@@ -193,20 +233,29 @@ static void row_c2r(const offt_t *g, double *row, double *buf, double *work) {
 
 #define OBLK 8
 
-/* complex transforms along j (axis=1) or i (axis=0) for all k <= Ng/2 */
-static void axis_c2c(offt_t *g, int axis, int sign) {
+typedef struct axis_args {
+    offt_t *g;
+    int axis, sign;
+} axis_args;
+
+/* complex transforms along j (axis=1) or i (axis=0) for all k <= Ng/2, for
+ * the other spatial axis o in [o0, o1) */
+static void axis_c2c_range(void *arg, long o0, long o1) {
+    const axis_args *a = (const axis_args *)arg;
+    offt_t *g = a->g;
+    const int axis = a->axis, sign = a->sign;
     const int N = g->Ng, Nk = N / 2 + 1;
     const long rowstride = g->Ng_pad;            /* doubles between j rows */
     const long planestride = (long)N * g->Ng_pad; /* doubles between i planes */
     double *buf = (double *)malloc((size_t)OBLK * 2 * N * sizeof(double));
     double *work = (double *)malloc((size_t)2 * N * sizeof(double));
     assert(buf && work);
-    for (int o = 0; o < N; ++o) {                 /* the other spatial axis */
+    for (long o = o0; o < o1; ++o) {
         for (int k0 = 0; k0 < Nk; k0 += OBLK) {
             int nb = Nk - k0 < OBLK ? Nk - k0 : OBLK;
             for (int t = 0; t < N; ++t) {
-                long off = axis == 1 ? (long)o * planestride + (long)t * rowstride
-                                     : (long)t * planestride + (long)o * rowstride;
+                long off = axis == 1 ? o * planestride + (long)t * rowstride
+                                     : (long)t * planestride + o * rowstride;
                 const double *src = g->f + off + 2 * k0;
                 for (int b = 0; b < nb; ++b) {
                     buf[(size_t)b * 2 * N + 2 * t] = src[2 * b];
@@ -216,8 +265,8 @@ static void axis_c2c(offt_t *g, int axis, int sign) {
             for (int b = 0; b < nb; ++b)
                 offt64_exec(g->pN, buf + (size_t)b * 2 * N, work, sign);
             for (int t = 0; t < N; ++t) {
-                long off = axis == 1 ? (long)o * planestride + (long)t * rowstride
-                                     : (long)t * planestride + (long)o * rowstride;
+                long off = axis == 1 ? o * planestride + (long)t * rowstride
+                                     : (long)t * planestride + o * rowstride;
                 double *dst = g->f + off + 2 * k0;
                 for (int b = 0; b < nb; ++b) {
                     dst[2 * b] = buf[(size_t)b * 2 * N + 2 * t];
@@ -230,28 +279,42 @@ static void axis_c2c(offt_t *g, int axis, int sign) {
     free(work);
 }
 
-void offt64_r2c_3d(offt_t *g) {
-    const int N = g->Ng;
-    double *buf = (double *)malloc((size_t)2 * N * sizeof(double));
-    double *work = (double *)malloc((size_t)2 * N * sizeof(double));
+static void axis_c2c(offt_t *g, int axis, int sign) {
+    axis_args a = {g, axis, sign};
+    orc_parallel(g->Ng, axis_c2c_range, &a);
+}
+
+/* real rows [r0, r1) of the grid, forward (r2c) or inverse (c2r) */
+static void rows_r2c(void *arg, long r0, long r1) {
+    offt_t *g = (offt_t *)arg;
+    double *buf = (double *)malloc((size_t)2 * g->Ng * sizeof(double));
+    double *work = (double *)malloc((size_t)2 * g->Ng * sizeof(double));
     assert(buf && work);
-    for (long r = 0; r < (long)N * N; ++r)
+    for (long r = r0; r < r1; ++r)
         row_r2c(g, g->f + r * g->Ng_pad, buf, work);
     free(buf);
     free(work);
+}
+
+static void rows_c2r(void *arg, long r0, long r1) {
+    offt_t *g = (offt_t *)arg;
+    double *buf = (double *)malloc((size_t)2 * g->Ng * sizeof(double));
+    double *work = (double *)malloc((size_t)2 * g->Ng * sizeof(double));
+    assert(buf && work);
+    for (long r = r0; r < r1; ++r)
+        row_c2r(g, g->f + r * g->Ng_pad, buf, work);
+    free(buf);
+    free(work);
+}
+
+void offt64_r2c_3d(offt_t *g) {
+    orc_parallel((long)g->Ng * g->Ng, rows_r2c, g);
     axis_c2c(g, 1, -1);
     axis_c2c(g, 0, -1);
 }
 
 void offt64_c2r_3d(offt_t *g) {
-    const int N = g->Ng;
     axis_c2c(g, 0, +1);
     axis_c2c(g, 1, +1);
-    double *buf = (double *)malloc((size_t)2 * N * sizeof(double));
-    double *work = (double *)malloc((size_t)2 * N * sizeof(double));
-    assert(buf && work);
-    for (long r = 0; r < (long)N * N; ++r)
-        row_c2r(g, g->f + r * g->Ng_pad, buf, work);
-    free(buf);
-    free(work);
+    orc_parallel((long)g->Ng * g->Ng, rows_c2r, g);
 }

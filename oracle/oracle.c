@@ -287,24 +287,74 @@ static inline void pcs_axis(double u, int Ng, int idx[4], double w[4]) {
     w[3] = cube(d) / 6;
 }
 
-static void paint_catalog(offt_t *g, const ogal_t *c, const double off[3],
-                          double Hinv, double scale, int negate) {
+typedef struct paint_args {
+    offt_t *g;
+    const ogal_t *c;
+    const double *off;
+    double Hinv, scale;
+    int negate;
+} paint_args;
+
+/* The particle loop of src/fft.c:67-93 (and its randoms twin 94-124),
+ * restricted to the cells of i-planes [i0, i1).  Every cell is owned by one
+ * range and receives its terms in the serial particle order, so any split
+ * over threads gives the serial result bit for bit. */
+static void paint_slab(void *arg, long i0, long i1) {
+    const paint_args *a = (const paint_args *)arg;
+    offt_t *g = a->g;
+    const ogal_t *c = a->c;
     const int Ng = g->Ng;
+    const int all = i0 == 0 && i1 == Ng;
     for (long p = 0; p < c->Np; ++p) {
         int gi[4], gj[4], gk[4];
         double wx[4], wy[4], wz[4];
-        pcs_axis(c->x[p] * Hinv - off[0], Ng, gi, wx);
-        pcs_axis(c->y[p] * Hinv - off[1], Ng, gj, wy);
-        pcs_axis(c->z[p] * Hinv - off[2], Ng, gk, wz);
-        for (int i = 0; i < 4; ++i)
+        pcs_axis(c->x[p] * a->Hinv - a->off[0], Ng, gi, wx);
+        if (!all) {
+            int hit = 0;
+            for (int i = 0; i < 4; ++i) hit |= gi[i] >= i0 && gi[i] < i1;
+            if (!hit) continue;
+        }
+        pcs_axis(c->y[p] * a->Hinv - a->off[1], Ng, gj, wy);
+        pcs_axis(c->z[p] * a->Hinv - a->off[2], Ng, gk, wz);
+        for (int i = 0; i < 4; ++i) {
+            if (gi[i] < i0 || gi[i] >= i1) continue;
             for (int j = 0; j < 4; ++j)
                 for (int k = 0; k < 4; ++k) {
-                    if (negate)
-                        g->f[FIDX(g, gi[i], gj[j], gk[k])] -= scale * wx[i] * wy[j] * wz[k];
+                    if (a->negate)
+                        g->f[FIDX(g, gi[i], gj[j], gk[k])] -= a->scale * wx[i] * wy[j] * wz[k];
                     else
                         g->f[FIDX(g, gi[i], gj[j], gk[k])] += wx[i] * wy[j] * wz[k];
                 }
+        }
     }
+}
+
+static void paint_catalog(offt_t *g, const ogal_t *c, const double off[3],
+                          double Hinv, double scale, int negate) {
+    paint_args a = {g, c, off, Hinv, scale, negate};
+    orc_parallel(g->Ng, paint_slab, &a);
+}
+
+typedef struct sweep_args {
+    offt_t *g;
+    double s;
+} sweep_args;
+
+static void scale_range(void *arg, long q0, long q1) {
+    const sweep_args *a = (const sweep_args *)arg;
+    for (long q = q0; q < q1; ++q)
+        a->g->f[q] *= a->s;
+}
+
+static void zero_range(void *arg, long q0, long q1) {
+    const sweep_args *a = (const sweep_args *)arg;
+    memset(a->g->f + q0, 0, (size_t)(q1 - q0) * sizeof(double));
+}
+
+/* f[q] *= s over the whole padded grid (elementwise, any split) */
+static void grid_scale(offt_t *g, double s) {
+    sweep_args a = {g, s};
+    orc_parallel(g->Ng3_pad, scale_range, &a);
 }
 
 /* src/fft.c:57-131 */
@@ -314,7 +364,8 @@ void oracle_fft_p2g(offt_t *g, const ogal_t *part, double offset[3]) {
         offset[i] = remainder(offset[i], Ng);
         g->offset[i] = offset[i];
     }
-    memset(g->f, 0, (size_t)g->Ng3_pad * sizeof(double));
+    sweep_args z = {g, 0};
+    orc_parallel(g->Ng3_pad, zero_range, &z);
     double Hinv = Ng / g->L;
     paint_catalog(g, part, offset, Hinv, 1.0, 0);
     if (part->rand != NULL) {
@@ -324,8 +375,38 @@ void oracle_fft_p2g(offt_t *g, const ogal_t *part, double offset[3]) {
     double H3 = cube(g->L / Ng);
     double nbar = part->Np / part->V;
     double Npcinv = 1 / nbar / H3;
-    for (long q = 0; q < g->Ng3_pad; ++q)
-        g->f[q] *= Npcinv;
+    grid_scale(g, Npcinv);
+}
+
+typedef struct phase_args {
+    offt_t *g;
+    const double *ph;   /* [3][Ng] complex factors */
+} phase_args;
+
+/* the mode loop of src/fft.c:217-232 over i-planes [i0, i1) */
+static void phase_planes(void *arg, long i0, long i1) {
+    const phase_args *a = (const phase_args *)arg;
+    offt_t *g = a->g;
+    const int Ng = g->Ng;
+    const double *ph = a->ph;
+#define PH(d, j, c) ph[((d) * Ng + (j)) * 2 + (c)]
+    for (long i = i0; i < i1; ++i)
+        for (int j = 0; j < Ng; ++j)
+            for (int k = 0; k <= Ng / 2; ++k) {
+                double pr = PH(0, i, 0) * PH(1, j, 0) * PH(2, k, 0)
+                          - PH(0, i, 0) * PH(1, j, 1) * PH(2, k, 1)
+                          - PH(0, i, 1) * PH(1, j, 0) * PH(2, k, 1)
+                          - PH(0, i, 1) * PH(1, j, 1) * PH(2, k, 0);
+                double pi = PH(0, i, 1) * PH(1, j, 0) * PH(2, k, 0)
+                          + PH(0, i, 0) * PH(1, j, 1) * PH(2, k, 0)
+                          + PH(0, i, 0) * PH(1, j, 0) * PH(2, k, 1)
+                          - PH(0, i, 1) * PH(1, j, 1) * PH(2, k, 1);
+                double *c = g->f + FIDX(g, i, j, 2 * k);
+                double fr = c[0], fi = c[1];
+                c[0] = fr * pr - fi * pi;
+                c[1] = fr * pi + fi * pr;
+            }
+#undef PH
 }
 
 /* src/fft.c:196-233 */
@@ -345,22 +426,8 @@ void oracle_fft_offset_phase(offt_t *g, int onoff) {
             PH(d, Ng - j, 0) = PH(d, j, 0);      /* overwrites j = Ng/2: Nyquist = -Ng/2 */
             PH(d, Ng - j, 1) = -PH(d, j, 1);
         }
-    for (int i = 0; i < Ng; ++i)
-        for (int j = 0; j < Ng; ++j)
-            for (int k = 0; k <= Ng / 2; ++k) {
-                double pr = PH(0, i, 0) * PH(1, j, 0) * PH(2, k, 0)
-                          - PH(0, i, 0) * PH(1, j, 1) * PH(2, k, 1)
-                          - PH(0, i, 1) * PH(1, j, 0) * PH(2, k, 1)
-                          - PH(0, i, 1) * PH(1, j, 1) * PH(2, k, 0);
-                double pi = PH(0, i, 1) * PH(1, j, 0) * PH(2, k, 0)
-                          + PH(0, i, 0) * PH(1, j, 1) * PH(2, k, 0)
-                          + PH(0, i, 0) * PH(1, j, 0) * PH(2, k, 1)
-                          - PH(0, i, 1) * PH(1, j, 1) * PH(2, k, 1);
-                double *c = g->f + FIDX(g, i, j, 2 * k);
-                double fr = c[0], fi = c[1];
-                c[0] = fr * pr - fi * pi;
-                c[1] = fr * pi + fi * pr;
-            }
+    phase_args a = {g, ph};
+    orc_parallel(Ng, phase_planes, &a);
 #undef PH
     free(ph);
 }
@@ -369,8 +436,7 @@ void oracle_fft_offset_phase(offt_t *g, int onoff) {
 void oracle_fft_x2k(offt_t *g, int offset_phase_on) {
     offt64_r2c_3d(g);
     double H3 = cube(g->L / g->Ng);
-    for (long q = 0; q < g->Ng3_pad; ++q)
-        g->f[q] *= H3;
+    grid_scale(g, H3);
     if (offset_phase_on)
         oracle_fft_offset_phase(g, 1);
 }
@@ -379,10 +445,29 @@ void oracle_fft_x2k(offt_t *g, int offset_phase_on) {
 void oracle_fft_k2x(offt_t *g, int offset_phase_off) {
     offt64_c2r_3d(g);
     double L3inv = 1 / cube(g->L);
-    for (long q = 0; q < g->Ng3_pad; ++q)
-        g->f[q] *= L3inv;
+    grid_scale(g, L3inv);
     if (offset_phase_off)
         oracle_fft_offset_phase(g, -1);
+}
+
+typedef struct deconv_args {
+    offt_t *g;
+    const double *Winv;
+} deconv_args;
+
+/* the mode loop of src/fft.c:246-252 over i-planes [i0, i1) */
+static void deconv_planes(void *arg, long i0, long i1) {
+    const deconv_args *a = (const deconv_args *)arg;
+    offt_t *g = a->g;
+    const int Ng = g->Ng;
+    const double *Winv = a->Winv;
+    for (long i = i0; i < i1; ++i)
+        for (int j = 0; j < Ng; ++j)
+            for (int k = 0; k <= Ng / 2; ++k) {
+                double W3 = Winv[i] * Winv[j] * Winv[k];
+                g->f[FIDX(g, i, j, 2 * k)] *= W3;
+                g->f[FIDX(g, i, j, 2 * k + 1)] *= W3;
+            }
 }
 
 /* src/fft.c:236-254 */
@@ -396,13 +481,8 @@ void oracle_fft_deconv(offt_t *g) {
         Winv[i] = quart(arg / sin(arg));
         Winv[Ng - i] = Winv[i];
     }
-    for (int i = 0; i < Ng; ++i)
-        for (int j = 0; j < Ng; ++j)
-            for (int k = 0; k <= Ng / 2; ++k) {
-                double W3 = Winv[i] * Winv[j] * Winv[k];
-                g->f[FIDX(g, i, j, 2 * k)] *= W3;
-                g->f[FIDX(g, i, j, 2 * k + 1)] *= W3;
-            }
+    deconv_args a = {g, Winv};
+    orc_parallel(Ng, deconv_planes, &a);
     free(Winv);
 }
 
@@ -420,6 +500,17 @@ void oracle_fft_importf(offt_t *g, const double *src) {
     memcpy(g->f, src, (size_t)g->Ng3_pad * sizeof(double));
 }
 
+typedef struct avg_args {
+    offt_t *g;
+    const double *fd;
+} avg_args;
+
+static void avg_range(void *arg, long q0, long q1) {
+    const avg_args *a = (const avg_args *)arg;
+    for (long q = q0; q < q1; ++q)
+        a->g->f[q] = 0.5 * (a->fd[q] + a->g->f[q]);
+}
+
 /* src/fft.c:257-276: interlacing = dual (offset 1/2, phase on) + plain, average, deconvolve */
 void oracle_fft_p2k(offt_t *g, const ogal_t *part) {
     double od[3] = {0.5, 0.5, 0.5};
@@ -429,13 +520,33 @@ void oracle_fft_p2k(offt_t *g, const ogal_t *part) {
     double on[3] = {0, 0, 0};
     oracle_fft_p2g(g, part, on);
     oracle_fft_x2k(g, 0);
-    for (long q = 0; q < g->Ng3_pad; ++q)
-        g->f[q] = 0.5 * (fd[q] + g->f[q]);
+    avg_args a = {g, fd};
+    orc_parallel(g->Ng3_pad, avg_range, &a);
     oracle_fft_deconv(g);
     free(fd);
 }
 
 /* ------------------------------------------------------------- estimators */
+
+typedef struct P_args {
+    offt_t *g;
+    double Vinv, Pshot;
+} P_args;
+
+/* the mode loop of src/power.c:19-25 over i-planes [i0, i1) */
+static void P_planes(void *arg, long i0, long i1) {
+    const P_args *a = (const P_args *)arg;
+    offt_t *g = a->g;
+    const int Ng = g->Ng;
+    for (long i = i0; i < i1; ++i)
+        for (int j = 0; j < Ng; ++j)
+            for (int k = (i == 0 && j == 0); k <= Ng / 2; ++k) {
+                double *c = g->f + FIDX(g, i, j, 2 * k);
+                double d2 = sq(c[0]) + sq(c[1]);
+                c[0] = d2 * a->Vinv - a->Pshot;
+                c[1] = 0;
+            }
+}
 
 /* src/power.c:12-27 */
 void oracle_P(offt_t *g, const ogal_t *part) {
@@ -444,14 +555,8 @@ void oracle_P(offt_t *g, const ogal_t *part) {
     double alpha = part->rand == NULL ? 0 : (double)part->Np / part->rand->Np;
     double Pshot = (1 + alpha) * part->V / part->Np;
     g->f[FIDX(g, 0, 0, 0)] = g->f[FIDX(g, 0, 0, 1)] = 0;
-    for (int i = 0; i < Ng; ++i)
-        for (int j = 0; j < Ng; ++j)
-            for (int k = (i == 0 && j == 0); k <= Ng / 2; ++k) {
-                double *c = g->f + FIDX(g, i, j, 2 * k);
-                double d2 = sq(c[0]) + sq(c[1]);
-                c[0] = d2 * Vinv - Pshot;
-                c[1] = 0;
-            }
+    P_args a = {g, Vinv, Pshot};
+    orc_parallel(Ng, P_planes, &a);
 }
 
 /* power.c:36,110 */

@@ -73,6 +73,17 @@ typedef struct ogal {
     struct ogal *rand;
 } ogal_t;
 
+/* ---- threads (fft64.c) ----
+ * The reference is single-threaded.  For large test workloads the oracle can
+ * split work over threads without changing a single bit of its results:
+ * threads own disjoint output elements (paint: slabs of i-planes, each
+ * filled in the serial particle order; FFT: whole 1D transforms; k-space
+ * sweeps: i-planes), and the bin sums stay serial.  Default 1. */
+void oracle_set_threads(int n);
+int  oracle_threads(void);
+/* fn(arg, lo, hi) over contiguous ranges covering [0, n), one per thread */
+void orc_parallel(long n, void (*fn)(void *arg, long lo, long hi), void *arg);
+
 /* ---- FP64 FFT (fft64.c) ---- */
 int  offt64_plan_init(offt64_plan *p, int n);
 void offt64_plan_free(offt64_plan *p);

@@ -78,6 +78,8 @@ def load():
         "oracle_gal_subbox": (P, [P, dptr, C.c_double]),
         "oracle_gal_subsphere": (P, [P, dptr, C.c_double]),
         "oracle_subvolume_deconv": (None, [G, C.c_int, C.c_int]),
+        "oracle_set_threads": (None, [C.c_int]),
+        "oracle_threads": (C.c_int, []),
         "offt64_r2c_3d": (None, [G]),
         "offt64_c2r_3d": (None, [G]),
     }
@@ -88,6 +90,23 @@ def load():
     del d3
     _LIB = lib
     return lib
+
+
+def set_threads(n):
+    """Threads for the large sweeps (paint by i-slabs in the serial particle
+    order, whole 1D FFTs, k-space planes); results are bit-identical to one
+    thread.  The bin sums stay serial."""
+    load().oracle_set_threads(int(n))
+
+
+def default_threads():
+    """CPU threads this process may use, at most 16."""
+    import os
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
 def _dp(a):

@@ -32,9 +32,13 @@ def _paint_vs_oracle(torch, orc, x, y, z, L, Ng, tile):
     tx, ty, tz = _dev(torch, x, y, z)
     p = fa.PowerPlan(Ng, L, interlace=True, Np_max=len(x), paint_tile=tile)
     A, B = p.paint_only(tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), len(x))
-    S = p.fixed_bits()
-    a = A.astype(np.float64) * 2.0 ** -S
-    b = B.astype(np.float64) * 2.0 ** -S
+    # each tile rounds to 2^-S particle or finer, S >= 19 (8192-particle items)
+    S = p.fixed_bits(8192)
+    assert S == 19 and p.fixed_bits(1) == 25 and p.fixed_bits(216) == 25
+    unit_fx = p.fixed_scale()
+    assert abs(unit_fx * 2.0 ** 25 - 1) <= 1.1e-4
+    a = A.astype(np.float64) * unit_fx
+    b = B.astype(np.float64) * unit_fx
     # oracle grids in particle units: undo 1/(nbar H^3)
     V = L ** 3
     H3 = (L / Ng) ** 3
@@ -45,7 +49,7 @@ def _paint_vs_oracle(torch, orc, x, y, z, L, Ng, tile):
     ra = g.f[:, :, :Ng] / unit
     g.p2g(c, (0.5, 0.5, 0.5))
     rb = g.f[:, :, :Ng] / unit
-    # Stated tolerance per stencil term: weight rounding 2^-(S+1), the packed
+    # Stated tolerance per stencil term: dithered weight rounding (< 2^-S), the packed
     # tile-local position (F = PB - log2(T) fraction bits, PB = 19, or 18 for
     # grids of more than 32^3 tiles, which take the wide sort; so at most
     # 2^-(F+1) cell away, moving a weight product by <= 0.67 * 2^-(F+1)),
@@ -55,7 +59,7 @@ def _paint_vs_oracle(torch, orc, x, y, z, L, Ng, tile):
     T = min(tile, Ng)
     nt = -(-Ng // T)
     F = (19 if nt ** 3 <= 32768 else 18) - int(np.ceil(np.log2(T)))
-    term = 2.0 ** -(S + 1) + 0.34 * 2.0 ** -F + 1e-7
+    term = 2.0 ** -S + 0.34 * 2.0 ** -F + 1e-7
     for grid, ref, off in ((a, ra, 0.0), (b, rb, 0.5)):
         cnt = np.zeros((Ng, Ng, Ng))
         base = [np.floor(v.astype(np.float64) * Ng / L - off).astype(np.int64)
@@ -66,13 +70,15 @@ def _paint_vs_oracle(torch, orc, x, y, z, L, Ng, tile):
                     np.add.at(cnt, ((base[0] + di) % Ng, (base[1] + dj) % Ng,
                                     (base[2] + dk) % Ng), 1.0)
         assert np.all(np.abs(grid - ref) <= cnt * term + 1e-12)
-    # mass: every particle contributes 1 up to 64 rounding steps; the
-    # rounding of the many small PCS products leaves a deficit of order
-    # 2^-(S+1) per particle, which the k-space normalisation removes
+    # mass: every particle contributes 1 up to 64 rounding steps of less
+    # than 2^-S each.  The rounding is dithered (unbiased), so the total
+    # error is a zero-mean sum over particles: its standard deviation is at
+    # most sqrt(N) * 64/sqrt(12) * 2^-S even if a particle's 64 errors were
+    # fully correlated; 6 sigma of that (a deterministic check for this seed)
     for grid in (a, b):
         deficit = len(x) - grid.sum()
-        assert abs(deficit) <= len(x) * 64 * 2.0 ** -(S + 1)
-        assert 0 <= deficit <= len(x) * 2.0 ** -S
+        assert abs(deficit) <= len(x) * 64 * 2.0 ** -S
+        assert abs(deficit) <= 6 * np.sqrt(len(x)) * 64 / np.sqrt(12) * 2.0 ** -S
     # determinism: a second paint gives bit-identical integer grids
     A2, B2 = p.paint_only(tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), len(x))
     assert np.array_equal(A, A2) and np.array_equal(B, B2)
@@ -148,11 +154,10 @@ def test_fp32_fft_vs_numpy(torch_cuda, lib, Ng, generic):
     used = p.fft_generic(generic)
     assert used == (generic or Ng in (40, 80))
     A, B = p.paint_only(tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), Np)
-    S = p.fixed_bits()
     p.run(tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), Np, L ** 3)
     sA, sB = p.spectra_host()
     for grid, spec in ((A, sA), (B, sB)):
-        ref = np.fft.rfftn(grid.astype(np.float64) * 2.0 ** -S, axes=(0, 1, 2))
+        ref = np.fft.rfftn(grid.astype(np.float64), axes=(0, 1, 2))
         got = _unpack_spectrum(spec)
         # FP32 transform: relative L2 error a few FP32 ulps times log2 of the size
         rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
@@ -335,32 +340,6 @@ def test_consecutive_catalogs_match_fresh_plans(torch_cuda, lib, monkeypatch, se
             continue
         p.use_graph(not step.startswith("eager"))
         check(p.run(*ptr, Np, V), name)
-    p.close()
-
-
-def test_config2_clustered_vs_oracle(torch_cuda, lib, orc):
-    """2,000,000 float32 particles of the lognormal recipe (L = 1000 Mpc/h,
-    256^3 seed grid), Ng = 256, interlaced, l = 0,2,4: the clustered
-    catalog at full size against the FP64 oracle."""
-    import fftgal_amd as fa
-    from fftgal_amd import mockgen
-    torch = torch_cuda
-    Np, L, Ng = 2_000_000, 1000.0, 256
-    (x, y, z), _ = mockgen.lognormal(Np, L, seed=0, Ngs=256)
-    V = L ** 3
-    dK = 2 * np.pi / L
-    ref = _oracle_bins(orc, x, y, z, V, Ng, L, True, (0, 0, 1), dK, 10)
-    tx, ty, tz = _dev(torch, x, y, z)
-    p = fa.PowerPlan(Ng, L, interlace=True, dK=dK, los=(0, 0, 1), Nmu=10, Np_max=Np)
-    out = p.run(tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), Np, V)
-    assert np.array_equal(out["N"], ref["N"])
-    assert np.array_equal(out["Nmu"], ref["Nmu"])
-    sel = (ref["N"] > 0) & (np.arange(len(ref["N"])) < Ng // 2)
-    for l in range(3):
-        rel = np.abs(out["Pl"][sel, l] - ref["Pl"][sel, l]) / np.abs(ref["Pl"][sel, l])
-        print(f"l={2*l}: max |dP/P| = {rel.max():.3e}")
-    rel0 = np.abs(out["Pl"][sel, 0] - ref["Pl"][sel, 0]) / np.abs(ref["Pl"][sel, 0])
-    assert rel0.max() <= 1e-5
     p.close()
 
 

@@ -472,3 +472,38 @@ def test_subvolume_deconv_window(orc, kind, Ng, Nsub):
                         exp[i % Ng, j % Ng, k % Ng] /= frac
     assert np.abs(got - exp).max() <= 1e-11 * np.abs(exp).max()
     g.free()
+
+
+@pytest.mark.parametrize("Ng,interlace", [(48, True), (30, False), (2, True)])
+def test_threaded_oracle_is_bit_identical(orc, Ng, interlace):
+    """The oracle's threaded sweeps (paint by i-slabs, each filled in the
+    serial particle order; whole 1D FFTs; k-space planes) give the serial
+    result bit for bit, randoms included."""
+    rng = np.random.default_rng(Ng)
+    L = 300.0
+    x, y, z = rng.random((3, 5000)) * L * 1.3 - 0.1 * L
+    rx, ry, rz = rng.random((3, 9000)) * L
+    out = []
+    for threads in (1, 7):
+        orc.set_threads(threads)
+        try:
+            r = orc.Catalog(rx, ry, rz, L ** 3)
+            c = orc.Catalog(x, y, z, L ** 3, rand=r)
+            g = orc.Grid(Ng, L)
+            if interlace:
+                g.p2k(c)
+            else:
+                g.p2g(c, (0.25, 0.0, 0.5))
+                g.x2k(1)
+                g.deconv()
+            g.P(c)
+            K, N, P = g.Pl_bins((0, 1, 1), 2 * np.pi / L)
+            f = g.f.copy()
+            g.k2x(0)
+            out.append((f, g.f.copy(), K, N, P))
+            g.free()
+            c.free()
+        finally:
+            orc.set_threads(1)
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b, equal_nan=a.dtype.kind == "f")
