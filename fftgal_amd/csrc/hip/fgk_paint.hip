@@ -19,9 +19,11 @@
 //    where most particles share a few buckets, do not serialise on one LDS
 //    counter; the count passes add one LDS atomic per particle (one per
 //    wave when the wave holds a single digit), which measured faster.
-//    Positions travel as 64-bit words: a 7-bit tile-within-bucket digit
-//    and a 19-bit tile-local fixed-point coordinate per axis (2^-15 cell
-//    for T = 16); the wide sort's 10-bit digit leaves 18 bits.  The paint
+//    Positions travel as tile-local fixed point with 2^-19 cell resolution
+//    (tiles of 16): a 64-bit main word of three 21-bit coordinates plus an
+//    8-bit extension of 2 more bits per axis; between the levels the
+//    extension shares a 16-bit side word with the tile-within-bucket digit,
+//    which is all the level-2 histogram reads.  The paint
 //    then runs one workgroup per work item (<= cap particles of one tile):
 //    the 4x4x4 stencils of both grids go into LDS footprints ((T+3)^3
 //    plain, (T+4)^3 dual cells) as int32 fixed point.  The footprints are
@@ -32,12 +34,11 @@
 //    particles the wave holds.  Non-zero footprint cells are flushed into
 //    the int64 grids with global integer atomics.
 //    Integer addition is associative, so the grids are bit-identical run to
-//    run whatever the scheduling.  Each weight is rounded once to 2^-S, S
-//    chosen so that cap particles cannot overflow int32.  Round to nearest
-//    is unbiased for large terms, but most PCS products are small and those
-//    below half a unit always round down: a particle paints on average about
-//    1 - 2^-(S+1) of its unit mass, a deficit the k-space kernel removes by
-//    normalising each grid by its painted total (its DC mode).
+//    run whatever the scheduling.  Each tile rounds its weights to the
+//    finest fixed point its particle count allows without int32 overflow
+//    (2^-19 particle for 8192-particle work items, up to 2^-25), with a
+//    deterministic dither that makes every rounding unbiased, so no term is
+//    lost on average however small; the grids hold one common unit.
 //    Grids with more tiles than even the wide two levels address (over
 //    2^21) fall back to a single-level sort whose ranks come from global
 //    atomics.
@@ -162,16 +163,21 @@ extern "C" int fgk_widen_f32_f64(double *dst, const float *src, long n, fgk_stre
 #define MAX_D2 128                       // tiles per coarse bucket
 #define WIDE_D1 2048                     // the same for grids with more tiles
 #define WIDE_D2 1024
-// packed particle word: tile-within-bucket digit in the top bits, then
-// three PB-bit tile-local positions; PB = 19 with a 7-bit digit (two levels
-// of MAX_D1 x MAX_D2), 18 with a 10-bit digit (WIDE_D1 x WIDE_D2)
-#define POS_BITS 19
-#define POS_BITS_WIDE 18
+// A particle travels through the sort as its tile-local position in fixed
+// point, li + d with F fraction bits per axis (F = 23 - log2 T: 2^-19 cell
+// for tiles of 16): the top POS_MAIN bits of each axis in a 64-bit main word
+// (x, y, z from the top), the low POS_EXT bits of the three axes in an 8-bit
+// extension.  Between the two sort levels the extension shares a 16-bit
+// side word with the tile-within-bucket digit (digit << 6 | extension, at
+// most 10 digit bits), which the level-2 histogram reads alone.
+#define POS_MAIN 21
+#define POS_EXT 2
+#define POS_MASK ((1u << POS_MAIN) - 1u)
+#define AUX_SHIFT (3 * POS_EXT)
 
 struct KeyGeo {
     double Hinv;
-    int Ng, T, nt, F;
-    int pb, fs;          // position bits per axis, shift of the level-2 digit (3 pb)
+    int Ng, T, nt, F;    // F: fraction bits of the packed tile-local positions
     int cg0, cg1, cg2;   // tiles per coarse bucket per axis (powers of two)
     int nc0, nc1, nc2;   // coarse buckets per axis
     int lg0, lg1, lg2;   // log2 of cg0, cg1, cg2
@@ -200,7 +206,9 @@ struct fgk_paintplan {
     uint32_t *rank32;    // [Np_max] single-level path
     uint64_t *zero_ptr;  // cleared by the paint kernel (fgk_paint_set_zero), 8-byte words
     long zero_words;
-    uint64_t *sorted1, *sorted2;
+    uint64_t *sorted1, *sorted2;   // main words after level 1, level 2
+    uint16_t *aux1;                // level-2 digit and extension after level 1
+    uint8_t *ext2;                 // extension after level 2
 };
 
 // One catalog of a batched paint (fgk_paintbatch): its positions, grid
@@ -227,8 +235,9 @@ __device__ __forceinline__ int batch_job(long e, const long *__restrict__ off, i
 }
 
 // wrapped plain-grid PCS base cell of one coordinate, its tile and the
-// tile-local position li + d as POS_BITS-bit fixed point (F fraction bits).
-// The cell index is taken in FP64 as the reference does (src/fft.c:68-76).
+// tile-local position li + d in fixed point with F fraction bits (at most
+// 26 bits in all).  The cell index is taken in FP64 as the reference does
+// (src/fft.c:68-76).
 __device__ __forceinline__ void axis_key(float xf, const KeyGeo &g, int &tc, uint32_t &q) {
     const double u = (double)xf * g.Hinv;
     const double fl = floor(u);
@@ -245,8 +254,11 @@ __device__ __forceinline__ void axis_key(float xf, const KeyGeo &g, int &tc, uin
     q = (uint32_t)(qq < qmax ? qq : qmax);
 }
 
+// coarse digit (bucket), fine digit (tile within the bucket), main word and
+// extension of one particle
 __device__ __forceinline__ void particle_key(float xv, float yv, float zv, const KeyGeo &g,
-                                             uint32_t &coarse, uint64_t &pk) {
+                                             uint32_t &coarse, uint32_t &fine, uint64_t &pk,
+                                             uint32_t &ext) {
     int tx, ty, tz;
     uint32_t qx, qy, qz;
     axis_key(xv, g, tx, qx);
@@ -254,10 +266,12 @@ __device__ __forceinline__ void particle_key(float xv, float yv, float zv, const
     axis_key(zv, g, tz, qz);
     const int cx = tx >> g.lg0, cy = ty >> g.lg1, cz = tz >> g.lg2;
     coarse = (uint32_t)((cx * g.nc1 + cy) * g.nc2 + cz);
-    const uint32_t fine = (uint32_t)((((tx & (g.cg0 - 1)) << g.lg1 | (ty & (g.cg1 - 1))) << g.lg2) |
-                                     (tz & (g.cg2 - 1)));
-    pk = ((uint64_t)fine << g.fs) | ((uint64_t)qx << (2 * g.pb)) | ((uint64_t)qy << g.pb) |
-         (uint64_t)qz;
+    fine = (uint32_t)((((tx & (g.cg0 - 1)) << g.lg1 | (ty & (g.cg1 - 1))) << g.lg2) |
+                      (tz & (g.cg2 - 1)));
+    pk = ((uint64_t)(qx >> POS_EXT) << (2 * POS_MAIN)) | ((uint64_t)(qy >> POS_EXT) << POS_MAIN) |
+         (uint64_t)(qz >> POS_EXT);
+    constexpr uint32_t em = (1u << POS_EXT) - 1u;
+    ext = ((qx & em) << (2 * POS_EXT)) | ((qy & em) << POS_EXT) | (qz & em);
 }
 
 // Rank of this lane among the valid lanes of its wave holding the same
@@ -359,8 +373,11 @@ __device__ __forceinline__ bool chunk_range2(long j, const uint32_t *chunk0, con
     return true;
 }
 
-// LEVEL: 1 reads float positions (digit = coarse bucket); 2 reads packed
-// words of sorted1 (digit = tile within bucket).  COUNT: histogram only.
+// LEVEL: 1 reads float positions (digit = coarse bucket) and writes main
+// words (out) and side words (out_aux: fine digit << 6 | extension); 2
+// reads them (digit = tile within bucket, from the side word alone for the
+// histogram) and writes main words and extensions (out_ext).  COUNT:
+// histogram only.
 // Batched level 1 (jobs != nullptr): chunk j belongs to the job whose chunk
 // range jc1[job] <= j < jc1[job+1] holds it, whose positions and geometry
 // replace x, y, z, Np and g.
@@ -369,10 +386,12 @@ __device__ __forceinline__ bool chunk_range2(long j, const uint32_t *chunk0, con
 template <int LEVEL, int COUNT, int NB>
 __global__ void __launch_bounds__(SORT_THREADS)
 k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
-       long Np, KeyGeo g, const uint64_t *__restrict__ in, const uint32_t *__restrict__ chunk0,
-       const uint32_t *__restrict__ base1, const uint32_t *__restrict__ T1, int nb1, int nd,
-       uint32_t *__restrict__ H, const uint32_t *__restrict__ O, uint64_t *__restrict__ out,
-       const PaintJob *__restrict__ jobs, const long *__restrict__ jc1, int njobs) {
+       long Np, KeyGeo g, const uint64_t *__restrict__ in, const uint16_t *__restrict__ in_aux,
+       const uint32_t *__restrict__ chunk0, const uint32_t *__restrict__ base1,
+       const uint32_t *__restrict__ T1, int nb1, int nd, uint32_t *__restrict__ H,
+       const uint32_t *__restrict__ O, uint64_t *__restrict__ out, uint16_t *__restrict__ out_aux,
+       uint8_t *__restrict__ out_ext, const PaintJob *__restrict__ jobs,
+       const long *__restrict__ jc1, int njobs) {
     constexpr int ND = 1 << NB;
     using dig_t = typename std::conditional<(NB > 8), uint16_t, uint8_t>::type;
     __shared__ uint32_t hist[ND];
@@ -410,8 +429,8 @@ k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__
     __syncthreads();
     const long st = s_start;
     const int n = s_n;
-    uint32_t dk[SORT_PER_THREAD], rk[SORT_PER_THREAD];
-    uint64_t vk[SORT_PER_THREAD];
+    uint32_t dk[SORT_PER_THREAD], rk[SORT_PER_THREAD], ak[SORT_PER_THREAD];
+    uint64_t vk[COUNT ? 1 : SORT_PER_THREAD];
     float xs[LEVEL == 1 ? SORT_PER_THREAD : 1], ys[LEVEL == 1 ? SORT_PER_THREAD : 1],
         zs[LEVEL == 1 ? SORT_PER_THREAD : 1];
     // all loads first, so that they are in flight together
@@ -424,7 +443,8 @@ k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__
                 ys[k] = y[st + i];
                 zs[k] = z[st + i];
             } else {
-                vk[k] = in[st + i];
+                ak[k] = in_aux[st + i];
+                if constexpr (!COUNT) vk[k] = in[st + i];
             }
         }
     }
@@ -432,24 +452,28 @@ k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__
     for (int k = 0; k < SORT_PER_THREAD; ++k) {
         const int i = k * SORT_THREADS + threadIdx.x;
         const bool valid = i < n;
-        uint32_t d = 0;
+        uint32_t d = 0, a = 0;
         uint64_t pk = 0;
         if (valid) {
             if (LEVEL == 1) {
-                particle_key(xs[k], ys[k], zs[k], g, d, pk);
+                uint32_t fine, ext;
+                particle_key(xs[k], ys[k], zs[k], g, d, fine, pk, ext);
+                a = (fine << AUX_SHIFT) | ext;
             } else {
-                pk = vk[k];
-                d = (uint32_t)(pk >> g.fs);
+                a = ak[k];
+                d = a >> AUX_SHIFT;
+                if constexpr (!COUNT) pk = vk[k];
             }
         }
-        if (COUNT) {
+        if constexpr (COUNT) {
             wave_count(d, valid, hist);
             rk[k] = 0;
         } else {
             rk[k] = wave_rank<NB>(d, valid, hist);
+            vk[k] = pk;
         }
         dk[k] = d;
-        vk[k] = pk;
+        ak[k] = LEVEL == 1 ? a : (a & ((1u << AUX_SHIFT) - 1u));
     }
     __syncthreads();
     if (COUNT) {
@@ -465,6 +489,7 @@ k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__
             const uint32_t pos = off[dk[k]] + rk[k];
             stage[pos] = vk[k];
             sdig[pos] = (dig_t)dk[k];
+            rk[k] = pos;                         // staged position, for the side words
         }
     }
     __syncthreads();
@@ -473,6 +498,22 @@ k_sort(const float *__restrict__ x, const float *__restrict__ y, const float *__
     __syncthreads();
     for (int pos = threadIdx.x; pos < n; pos += blockDim.x)
         out[off[sdig[pos]] + pos] = stage[pos];
+    __syncthreads();
+    // the side words (level 1) or extensions (level 2), staged the same way
+    uint32_t *stage32 = (uint32_t *)stage;
+#pragma unroll
+    for (int k = 0; k < SORT_PER_THREAD; ++k) {
+        const int i = k * SORT_THREADS + threadIdx.x;
+        if (i < n) stage32[rk[k]] = ak[k];
+    }
+    __syncthreads();
+    if (LEVEL == 1) {
+        for (int pos = threadIdx.x; pos < n; pos += blockDim.x)
+            out_aux[off[sdig[pos]] + pos] = (uint16_t)stage32[pos];
+    } else {
+        for (int pos = threadIdx.x; pos < n; pos += blockDim.x)
+            out_ext[off[sdig[pos]] + pos] = (uint8_t)stage32[pos];
+    }
 }
 
 // S[b][d] = sum of H[j][d] over the RB chunks j of block b
@@ -708,9 +749,9 @@ k_count_global(const float *__restrict__ x, const float *__restrict__ y,
                uint32_t *__restrict__ rank) {
     const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const bool valid = p < Np;
-    uint32_t t = 0;
+    uint32_t t = 0, fine, ext;
     uint64_t pk;
-    if (valid) particle_key(x[p], y[p], z[p], g, t, pk);
+    if (valid) particle_key(x[p], y[p], z[p], g, t, fine, pk, ext);
     const uint32_t r = wave_rank<24>(t, valid, tot);
     if (valid) rank[p] = r;
 }
@@ -718,13 +759,16 @@ k_count_global(const float *__restrict__ x, const float *__restrict__ y,
 __global__ void k_scatter_global(const float *__restrict__ x, const float *__restrict__ y,
                                  const float *__restrict__ z, long Np, KeyGeo g,
                                  const uint32_t *__restrict__ base,
-                                 const uint32_t *__restrict__ rank, uint64_t *__restrict__ out) {
+                                 const uint32_t *__restrict__ rank, uint64_t *__restrict__ out,
+                                 uint8_t *__restrict__ out_ext) {
     const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (p >= Np) return;
-    uint32_t t;
+    uint32_t t, fine, ext;
     uint64_t pk;
-    particle_key(x[p], y[p], z[p], g, t, pk);
-    out[base[t] + rank[p]] = pk;
+    particle_key(x[p], y[p], z[p], g, t, fine, pk, ext);
+    const uint32_t q = base[t] + rank[p];
+    out[q] = pk;
+    out_ext[q] = (uint8_t)ext;
 }
 
 // Six times the PCS weights of src/fft.c:83-88 in FP32 (they sum to 6).
@@ -799,8 +843,10 @@ __device__ __forceinline__ float row_dither(float u0, float wab) {
 // Appleby 2011) of its packed sort word and its tile slot (the word holds
 // only tile-local coordinates).  They depend only on the particle's
 // quantised position, so the grids stay bit-identical run to run.
-__device__ __forceinline__ void dither_pair(uint64_t pk, uint32_t tile, float &ua, float &ub) {
-    uint32_t h = (uint32_t)pk ^ ((uint32_t)(pk >> 32) * 0x9E3779B9u) ^ (tile * 0x7FEB352Du);
+__device__ __forceinline__ void dither_pair(uint64_t pk, uint32_t ext, uint32_t tile, float &ua,
+                                            float &ub) {
+    uint32_t h = (uint32_t)pk ^ ((uint32_t)(pk >> 32) * 0x9E3779B9u) ^ (tile * 0x7FEB352Du) ^
+                 (ext * 0x2545F491u);
     h ^= h >> 16;
     h *= 0x85EBCA6Bu;
     h ^= h >> 13;
@@ -921,28 +967,29 @@ __device__ __forceinline__ void stencil_lds(int *__restrict__ fp, int bi, int bj
     }
 }
 
-// tile-local cell li and fraction d of one axis of a packed particle word
+// tile-local cell li and fraction d of one axis of a packed particle
+// (main word and extension, F fraction bits; exact in FP32 for F <= 24)
 __device__ __forceinline__ void unpack_axis(uint32_t q, int F, float fsc, int &li, float &d) {
     li = (int)(q >> F);
     d = (float)(q & ((1u << F) - 1u)) * fsc;
 }
 
-template <int PB>
-__device__ __forceinline__ uint32_t pk_axis(uint64_t pk, int a) {
-    return (uint32_t)(pk >> ((2 - a) * PB)) & ((1u << PB) - 1u);
+__device__ __forceinline__ uint32_t pk_axis(uint64_t pk, uint32_t ext, int a) {
+    const uint32_t hi = (uint32_t)(pk >> ((2 - a) * POS_MAIN)) & POS_MASK;
+    return (hi << POS_EXT) | ((ext >> ((2 - a) * POS_EXT)) & ((1u << POS_EXT) - 1u));
 }
 
 // one grid's stencil of one packed particle word into an LDS footprint:
 // DUAL = 0: plain grid, nodes li-1..li+2, footprint origin tile origin - 1;
 // DUAL = 1: dual grid (offset 1/2), base li or li+1, origin tile origin - 2
-template <int NBK, int DUAL, int PB>
-__device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, int F, float fsc,
-                                           float unit, float u, int lane) {
+template <int NBK, int DUAL>
+__device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, uint32_t ext, int F,
+                                           float fsc, float unit, float u, int lane) {
     int li[3];
     float d[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        unpack_axis(pk_axis<PB>(pk, a), F, fsc, li[a], d[a]);
+        unpack_axis(pk_axis(pk, ext, a), F, fsc, li[a], d[a]);
         if (DUAL) {
             if (d[a] >= 0.5f) { d[a] -= 0.5f; li[a] += 1; }
             else d[a] += 0.5f;
@@ -968,9 +1015,10 @@ __device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, in
 // batch; the job owning it (ji0[job] <= item < ji0[job+1], ja <= job < jb)
 // supplies the geometry, grids (set 0: GA/GB, set 1: GA2/GB2) and tile
 // tables (tilebase, items + job * sl, MAX_D2 slots per coarse bucket).
-template <int T, int PB>
+template <int T>
 __global__ void __launch_bounds__(PAINT_THREADS, 8)
-k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ tilebase,
+k_paint_tiles(const uint64_t *__restrict__ sorted, const uint8_t *__restrict__ sorted_ext,
+              const uint32_t *__restrict__ tilebase,
               const uint32_t *__restrict__ items, long nslots, KeyGeo g, int nb2,
               int cap, int interlace, unsigned long long *__restrict__ GA,
               unsigned long long *__restrict__ GB, const PaintJob *__restrict__ jobs,
@@ -1016,10 +1064,11 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ 
     const float unit = FIX_C0 * (float)(1 << (S - FIX_S0));
     for (long p = p0 + threadIdx.x; p < p1; p += PAINT_THREADS) {
         const uint64_t pk = sorted[p];
+        const uint32_t ext = sorted_ext[p];
         float ua, ub;
-        dither_pair(pk, (uint32_t)t, ua, ub);
-        paint_grid<brick_n(SA), 0, PB>(fa, pk, F, fsc, unit, ua, lane);
-        if (interlace) paint_grid<brick_n(SB), 1, PB>(fb, pk, F, fsc, unit, ub, lane);
+        dither_pair(pk, ext, (uint32_t)t, ua, ub);
+        paint_grid<brick_n(SA), 0>(fa, pk, ext, F, fsc, unit, ua, lane);
+        if (interlace) paint_grid<brick_n(SB), 1>(fb, pk, ext, F, fsc, unit, ub, lane);
     }
     __syncthreads();
     const long b1 = t / nb2, f = t - b1 * nb2;
@@ -1096,8 +1145,10 @@ static KeyGeo tile_geometry(int Ng, int T, int allow_wide, int &mode) {
         cg[0] = cg[1] = cg[2] = 1;
         nc[0] = nc[1] = nc[2] = nt;
     }
-    const int pb = mode == 2 ? POS_BITS_WIDE : POS_BITS;
-    KeyGeo g = {0.0,   Ng,    T,     nt,    pb - ceil_log2(T), pb, 3 * pb,
+    const int F = POS_MAIN + POS_EXT - ceil_log2(T) < 24 ? POS_MAIN + POS_EXT - ceil_log2(T) : 24;
+    // (the digit and extension share 16 bits between the sort levels)
+    static_assert(AUX_SHIFT + 10 <= 16, "side word");
+    KeyGeo g = {0.0,   Ng,    T,     nt,    F,
                 cg[0], cg[1], cg[2], nc[0], nc[1],
                 nc[2], ceil_log2(cg[0]), ceil_log2(cg[1]), ceil_log2(cg[2]),
                 ((1ull << 32) + (uint64_t)T - 1) / (uint64_t)T};
@@ -1172,8 +1223,12 @@ extern "C" int fgk_paint_plan_create(fgk_paintplan **out, int Ng, int tile, long
     rc |= fgk_malloc((void **)&pp->T2, (size_t)pp->nslots * u);
     rc |= fgk_malloc((void **)&pp->tilebase, (size_t)(pp->nslots + 1) * u);
     rc |= fgk_malloc((void **)&pp->items, (size_t)(pp->nslots + 1) * u);
-    if (pp->two_level) rc |= fgk_malloc((void **)&pp->sorted1, (size_t)Np_max * sizeof(uint64_t));
+    if (pp->two_level) {
+        rc |= fgk_malloc((void **)&pp->sorted1, (size_t)Np_max * sizeof(uint64_t));
+        rc |= fgk_malloc((void **)&pp->aux1, (size_t)Np_max * sizeof(uint16_t));
+    }
     rc |= fgk_malloc((void **)&pp->sorted2, (size_t)Np_max * sizeof(uint64_t));
+    rc |= fgk_malloc((void **)&pp->ext2, (size_t)Np_max * sizeof(uint8_t));
     if (rc) {
         fgk_paint_plan_destroy(pp);
         return -1;
@@ -1190,6 +1245,8 @@ extern "C" void fgk_paint_plan_destroy(fgk_paintplan *pp) {
     for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; ++i) fgk_free(bufs[i]);
     fgk_free(pp->sorted1);
     fgk_free(pp->sorted2);
+    fgk_free(pp->aux1);
+    fgk_free(pp->ext2);
     free(pp);
 }
 
@@ -1283,7 +1340,8 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
         case 2:
             if (Np > 0) {
                 hipLaunchKernelGGL(k_scatter_global, dim3(blocks_for(Np, 256)), dim3(256), 0, s,
-                                   x, y, z, Np, g, pp->tilebase, pp->rank32, pp->sorted2);
+                                   x, y, z, Np, g, pp->tilebase, pp->rank32, pp->sorted2,
+                                   pp->ext2);
                 FGK_LAUNCHED();
             }
             return 0;
@@ -1298,9 +1356,10 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
         switch (stage) {
         case 0:   // level 1 histograms
             SORT_LAUNCH(1, 1, nw1, x, y, z, Np, g, (const uint64_t *)nullptr,
-                        (const uint32_t *)nullptr, (const uint32_t *)nullptr,
-                        (const uint32_t *)nullptr, pp->nb1, pp->nb1, pp->H1,
-                        (const uint32_t *)nullptr, (uint64_t *)nullptr);
+                        (const uint16_t *)nullptr, (const uint32_t *)nullptr,
+                        (const uint32_t *)nullptr, (const uint32_t *)nullptr, pp->nb1, pp->nb1,
+                        pp->H1, (const uint32_t *)nullptr, (uint64_t *)nullptr,
+                        (uint16_t *)nullptr, (uint8_t *)nullptr);
             return 0;
         case 1:   // level 1 offsets and the level 2 chunk layout
             hipLaunchKernelGGL(k_set_seg1, dim3(1), dim3(64), 0, s, pp->seg1,
@@ -1311,15 +1370,16 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
                                (long)pp->nb1);
         case 2:
             SORT_LAUNCH(1, 0, nw1, x, y, z, Np, g, (const uint64_t *)nullptr,
-                        (const uint32_t *)nullptr, (const uint32_t *)nullptr,
-                        (const uint32_t *)nullptr, pp->nb1, pp->nb1, (uint32_t *)nullptr, pp->O1,
-                        pp->sorted1);
+                        (const uint16_t *)nullptr, (const uint32_t *)nullptr,
+                        (const uint32_t *)nullptr, (const uint32_t *)nullptr, pp->nb1, pp->nb1,
+                        (uint32_t *)nullptr, pp->O1, pp->sorted1, pp->aux1, (uint8_t *)nullptr);
             return 0;
         case 3: { // level 2 histograms (every capacity chunk; empty ones write zeros)
             const long nw2 = nw1 + (long)pp->nb1 * RB;
             SORT_LAUNCH(2, 1, nw2, (const float *)nullptr, (const float *)nullptr,
-                        (const float *)nullptr, Np, g, pp->sorted1, pp->chunk0, pp->base1, pp->T1,
-                        pp->nb1, pp->nb2, pp->H2, (const uint32_t *)nullptr, (uint64_t *)nullptr);
+                        (const float *)nullptr, Np, g, pp->sorted1, pp->aux1, pp->chunk0,
+                        pp->base1, pp->T1, pp->nb1, pp->nb2, pp->H2, (const uint32_t *)nullptr,
+                        (uint64_t *)nullptr, (uint16_t *)nullptr, (uint8_t *)nullptr);
             return 0;
         }
         case 4: {
@@ -1357,8 +1417,9 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
         case 5: {
             const long nw2 = nw1 + (long)pp->nb1 * RB;
             SORT_LAUNCH(2, 0, nw2, (const float *)nullptr, (const float *)nullptr,
-                        (const float *)nullptr, Np, g, pp->sorted1, pp->chunk0, pp->base1, pp->T1,
-                        pp->nb1, pp->nb2, (uint32_t *)nullptr, pp->O2, pp->sorted2);
+                        (const float *)nullptr, Np, g, pp->sorted1, pp->aux1, pp->chunk0,
+                        pp->base1, pp->T1, pp->nb1, pp->nb2, (uint32_t *)nullptr, pp->O2,
+                        pp->sorted2, (uint16_t *)nullptr, pp->ext2);
             return 0;
         }
         default:
@@ -1371,20 +1432,10 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
         const dim3 grid((unsigned)max_items), block(PAINT_THREADS);
         unsigned long long *A = (unsigned long long *)gridA, *B = (unsigned long long *)gridB;
 #define PAINT_T(TT)                                                                           \
-    do {                                                                                      \
-        if (pp->wide)                                                                         \
-            hipLaunchKernelGGL((k_paint_tiles<TT, POS_BITS_WIDE>), grid, block, lds, s,       \
-                               pp->sorted2, pp->tilebase, pp->items, pp->nslots, g, pp->nb2,  \
-                               pp->cap, pp->interlace, A, B, (const PaintJob *)nullptr,       \
-                               (const long *)nullptr, 0, 0, 0l, 0, pp->zero_ptr,              \
-                               pp->zero_words);                                               \
-        else                                                                                  \
-            hipLaunchKernelGGL((k_paint_tiles<TT, POS_BITS>), grid, block, lds, s,            \
-                               pp->sorted2, pp->tilebase, pp->items, pp->nslots, g, pp->nb2,  \
-                               pp->cap, pp->interlace, A, B, (const PaintJob *)nullptr,       \
-                               (const long *)nullptr, 0, 0, 0l, 0, pp->zero_ptr,              \
-                               pp->zero_words);                                               \
-    } while (0)
+    hipLaunchKernelGGL((k_paint_tiles<TT>), grid, block, lds, s, pp->sorted2, pp->ext2,       \
+                       pp->tilebase, pp->items, pp->nslots, g, pp->nb2, pp->cap,              \
+                       pp->interlace, A, B, (const PaintJob *)nullptr, (const long *)nullptr, \
+                       0, 0, 0l, 0, pp->zero_ptr, pp->zero_words)
         switch (pp->T) {
         case 16: PAINT_T(16); break;
         case 12: PAINT_T(12); break;
@@ -1492,6 +1543,8 @@ struct fgk_paintbatch {
     uint32_t *H2, *O2, *S2, *B2, *T2;
     uint32_t *tilebase, *items;
     uint64_t *sorted1, *sorted2;
+    uint16_t *aux1;
+    uint8_t *ext2;
     int table_ok;                  // djobs holds hjobs
     int sets;                      // grid sets of every job (1 or 2)
     int set;                       // set the paint stage writes
@@ -1507,6 +1560,8 @@ extern "C" void fgk_paint_batch_destroy(fgk_paintbatch *pb) {
     for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; ++i) fgk_free(bufs[i]);
     fgk_free(pb->sorted1);
     fgk_free(pb->sorted2);
+    fgk_free(pb->aux1);
+    fgk_free(pb->ext2);
     fgk_free(pb->djobs);
     fgk_free(pb->dc1);
     fgk_free(pb->di0);
@@ -1640,6 +1695,8 @@ extern "C" int fgk_paint_batch_create(fgk_paintbatch **out, int njobs,
     rc |= fgk_malloc((void **)&pb->items, (size_t)njobs * BATCH_SL * u);
     rc |= fgk_malloc((void **)&pb->sorted1, (size_t)Np_tot * sizeof(uint64_t));
     rc |= fgk_malloc((void **)&pb->sorted2, (size_t)Np_tot * sizeof(uint64_t));
+    rc |= fgk_malloc((void **)&pb->aux1, (size_t)Np_tot * sizeof(uint16_t));
+    rc |= fgk_malloc((void **)&pb->ext2, (size_t)Np_tot * sizeof(uint8_t));
     uint32_t *hb = (uint32_t *)malloc((size_t)(njobs + 1) * u);
     if (!rc && !hb) { fgk_set_error("out of host memory"); rc = -1; }
     if (!rc) {
@@ -1722,9 +1779,10 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
         hipLaunchKernelGGL((k_sort<1, 1, 8>), dim3((unsigned)pb->NW1), dim3(SORT_THREADS), 0, s,
                            (const float *)nullptr, (const float *)nullptr,
                            (const float *)nullptr, 0l, g0, (const uint64_t *)nullptr,
-                           (const uint32_t *)nullptr, (const uint32_t *)nullptr,
-                           (const uint32_t *)nullptr, 0, MAX_D1, pb->H1,
-                           (const uint32_t *)nullptr, (uint64_t *)nullptr,
+                           (const uint16_t *)nullptr, (const uint32_t *)nullptr,
+                           (const uint32_t *)nullptr, (const uint32_t *)nullptr, 0, MAX_D1,
+                           pb->H1, (const uint32_t *)nullptr, (uint64_t *)nullptr,
+                           (uint16_t *)nullptr, (uint8_t *)nullptr,
                            (const PaintJob *)pb->djobs, (const long *)pb->dc1, nj);
         FGK_LAUNCHED();
         return 0;
@@ -1749,17 +1807,20 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
         hipLaunchKernelGGL((k_sort<1, 0, 8>), dim3((unsigned)pb->NW1), dim3(SORT_THREADS), 0, s,
                            (const float *)nullptr, (const float *)nullptr,
                            (const float *)nullptr, 0l, g0, (const uint64_t *)nullptr,
-                           (const uint32_t *)nullptr, (const uint32_t *)nullptr,
-                           (const uint32_t *)nullptr, 0, MAX_D1, (uint32_t *)nullptr, pb->O1,
-                           pb->sorted1, (const PaintJob *)pb->djobs, (const long *)pb->dc1, nj);
+                           (const uint16_t *)nullptr, (const uint32_t *)nullptr,
+                           (const uint32_t *)nullptr, (const uint32_t *)nullptr, 0, MAX_D1,
+                           (uint32_t *)nullptr, pb->O1, pb->sorted1, pb->aux1,
+                           (uint8_t *)nullptr, (const PaintJob *)pb->djobs,
+                           (const long *)pb->dc1, nj);
         FGK_LAUNCHED();
         return 0;
     case 3:
         hipLaunchKernelGGL((k_sort<2, 1, 7>), dim3((unsigned)pb->NW2), dim3(SORT_THREADS), 0, s,
                            (const float *)nullptr, (const float *)nullptr,
-                           (const float *)nullptr, 0l, g0, pb->sorted1, pb->chunk0, pb->base1,
-                           pb->T1, (int)nseg2, MAX_D2, pb->H2, (const uint32_t *)nullptr,
-                           (uint64_t *)nullptr, (const PaintJob *)nullptr, (const long *)nullptr,
+                           (const float *)nullptr, 0l, g0, pb->sorted1, pb->aux1, pb->chunk0,
+                           pb->base1, pb->T1, (int)nseg2, MAX_D2, pb->H2,
+                           (const uint32_t *)nullptr, (uint64_t *)nullptr, (uint16_t *)nullptr,
+                           (uint8_t *)nullptr, (const PaintJob *)nullptr, (const long *)nullptr,
                            0);
         FGK_LAUNCHED();
         return 0;
@@ -1784,9 +1845,10 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
     case 5:
         hipLaunchKernelGGL((k_sort<2, 0, 7>), dim3((unsigned)pb->NW2), dim3(SORT_THREADS), 0, s,
                            (const float *)nullptr, (const float *)nullptr,
-                           (const float *)nullptr, 0l, g0, pb->sorted1, pb->chunk0, pb->base1,
-                           pb->T1, (int)nseg2, MAX_D2, (uint32_t *)nullptr, pb->O2, pb->sorted2,
-                           (const PaintJob *)nullptr, (const long *)nullptr, 0);
+                           (const float *)nullptr, 0l, g0, pb->sorted1, pb->aux1, pb->chunk0,
+                           pb->base1, pb->T1, (int)nseg2, MAX_D2, (uint32_t *)nullptr, pb->O2,
+                           pb->sorted2, (uint16_t *)nullptr, pb->ext2, (const PaintJob *)nullptr,
+                           (const long *)nullptr, 0);
         FGK_LAUNCHED();
         return 0;
     case 6: {
@@ -1804,7 +1866,7 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
             const size_t lds = (size_t)paint_lds_bytes(pb->side[i], pb->interlace_any);
             const dim3 grid((unsigned)nitems), block(PAINT_THREADS);
 #define PAINT_TB(TT)                                                                          \
-    hipLaunchKernelGGL((k_paint_tiles<TT, POS_BITS>), grid, block, lds, s, pb->sorted2,      \
+    hipLaunchKernelGGL((k_paint_tiles<TT>), grid, block, lds, s, pb->sorted2, pb->ext2,      \
                        pb->tilebase,                                                          \
                        pb->items, 0l, g0, MAX_D2, pb->cap, 0,                                 \
                        (unsigned long long *)nullptr, (unsigned long long *)nullptr,          \

@@ -4,20 +4,21 @@ Parity rule for the mixed-precision path (fixed-point paint, FP32 FFT, FP64
 k-space), on every k bin below k_Nyquist with modes:
   * mode counts bit-exact (per bin and per wedge);
   * |dP_l / P_l| <= 1e-5 for l = 0, 2, 4 and every mu wedge, wherever
-    |P_l| >= EPS |P_0| + EPS_SHOT P_shot in that bin;
-  * |dP_l| <= ABS |P_0| + ABS_SHOT P_shot in every bin, the near-zero ones
-    included.
-Two error floors set the coefficients:
-  * the FP32 FFT carries a per-mode relative error of about 1.1e-7 rms
-    (measured against an FP64 transform of the same fixed-point grids);
-    averaged into a bin it leaves an additive error of up to about
-    2e-7 (P_0 + P_shot) on P_l, so a relative 1e-5 is resolvable where
-    |P_l| >= EPS = 0.02 of P_0;
-  * the paint packs each particle's position within its tile to 2^-15 cell,
-    which moves the shot-noise part of P_l by up to about 1e-5 P_shot in
-    the lowest-k bins of sparse catalogs (reproduced on the CPU by the
-    oracle with positions rounded the same way), hence ABS_SHOT = 1.5e-5
-    and EPS_SHOT = ABS_SHOT / 1e-5.
+    |P_l| >= EPS (|P_0| + P_shot) + EPS_PEAK P_peak in that bin (P_peak: the
+    largest |P_0| below k_Nyquist);
+  * |dP_l| <= ABS (|P_0| + P_shot) + ABS_PEAK P_peak in every bin, the
+    near-zero ones included.
+The coefficients come from the FP32 FFT, measured against an FP64 transform
+of the same fixed-point grids:
+  * its per-mode error is about 1.1e-7 of the mode's amplitude, which leaves
+    up to about 3e-7 (|P_0| + P_shot) on P_l in a bin; a relative 1e-5 is
+    resolvable where |P_l| >= EPS = 0.05 of that (0.02 on the clustered
+    catalogs of configs 2 and 3, where it holds too);
+  * its rounding is relative to the largest values it combines, so a bin
+    whose power lies orders of magnitude below the peak (the high-k bins of
+    folded grids) carries up to about 1e-8 P_peak: EPS_PEAK = 1e-3.
+The paint's share is below these: unbiased dithered rounding at 2^-19 to
+2^-25 particle, positions packed to 2^-19 cell within their tile.
 The FP64 drop-in path is held to 1e-10 relative everywhere.
 
 The oracle runs threaded (bit-identical to one thread) so that the 2^27
@@ -31,9 +32,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 REL = 1e-5
-EPS, ABS = 0.02, 1e-6
-ABS_SHOT = 1.5e-5
-EPS_SHOT = ABS_SHOT / REL
+EPS, EPS_PEAK = 0.05, 1e-3
+ABS, ABS_PEAK = 2e-6, 1e-8
 
 
 def say(msg):
@@ -56,14 +56,15 @@ def _oracle(orc, x, y, z, V, Ng, L, interlace, dK, Nmu, los=(0, 0, 1)):
         orc.set_threads(1)
 
 
-def check_bins(out, ref, Ng, Pshot, nl=3, what=""):
+def check_bins(out, ref, Ng, Pshot, nl=3, what="", eps=EPS):
     """The parity rule of this module; returns the worst figures."""
     assert np.array_equal(out["N"], ref["N"]), f"{what}: mode counts differ"
     nb = len(ref["N"])
     sel = (ref["N"] > 0) & (np.arange(nb) < Ng // 2)
     P0 = np.abs(ref["Pl"][sel, 0])
-    keep = EPS * P0 + EPS_SHOT * Pshot
-    bound = ABS * P0 + ABS_SHOT * Pshot
+    peak = P0.max()
+    keep = eps * (P0 + Pshot) + EPS_PEAK * peak
+    bound = ABS * (P0 + Pshot) + ABS_PEAK * peak
     worst = {}
     for l in range(nl):
         r, o = ref["Pl"][sel, l], out["Pl"][sel, l]
@@ -80,13 +81,13 @@ def check_bins(out, ref, Ng, Pshot, nl=3, what=""):
         p0 = np.abs(ref["Pl"][:, 0])[:, None] * np.ones_like(ref["Pmu"])
         r, o, p0 = ref["Pmu"][wsel], out["Pmu"][wsel], p0[wsel]
         d = np.abs(o - r)
-        big = np.abs(r) >= EPS * p0 + EPS_SHOT * Pshot
+        big = np.abs(r) >= eps * (p0 + Pshot) + EPS_PEAK * peak
         rel = d[big] / np.abs(r[big]) if big.any() else np.zeros(1)
-        ab = d / (ABS * p0 + ABS_SHOT * Pshot)
+        ab = d / (ABS * (p0 + Pshot) + ABS_PEAK * peak)
         worst["wedges"] = (float(rel.max()), float(ab.max()), int(big.sum()), int(wsel.sum()))
         assert rel.max() <= REL, f"{what}: wedges |dP/P| {rel.max():.3e}"
         assert ab.max() <= 1, f"{what}: wedges |dP| at {ab.max():.3f} of its bound"
-    say(f"{what}: (max |dP/P| over kept bins, max |dP| / bound, kept, bins) {worst}")
+    say(f"{what} (eps {eps}): (max |dP/P| over kept bins, max |dP| / bound, kept, bins) {worst}")
     return worst
 
 
@@ -145,6 +146,7 @@ def test_config2_clustered_vs_oracle(torch_cuda, lib, orc):
     out = p.run(*[a.data_ptr() for a in t], Np, V)
     p.close()
     check_bins(out, ref, Ng, V / Np, what="config 2")
+    check_bins(out, ref, Ng, V / Np, what="config 2", eps=0.02)
 
 
 @pytest.mark.timeout(900)
@@ -190,6 +192,7 @@ def test_config3_headline_vs_oracle(torch_cuda, lib, orc):
     # the mixed form: the module's rule; a rerun gives the same bins to the
     # rounding of the FP64 bin sums
     check_bins(out, ref, Ng, V / Np, what="config 3")
+    check_bins(out, ref, Ng, V / Np, what="config 3", eps=0.02)
     assert np.array_equal(again["N"], out["N"])
     ok = out["N"] > 0
     scale = (np.abs(out["Pl"][ok, 0]) + V / Np)[:, None]
@@ -207,8 +210,6 @@ def _batch_subset():
     return pick
 
 
-@pytest.mark.xfail(reason="fold-4 catalogs: tile-local positions packed to 2^-15 cell",
-                   strict=False)
 @pytest.mark.timeout(900)
 def test_config4_batch_subset_vs_oracle(torch_cuda, lib, orc):
     """The batched uneven workload on a declared subset: seeds 11, 36, 62

@@ -49,16 +49,15 @@ def _paint_vs_oracle(torch, orc, x, y, z, L, Ng, tile):
     ra = g.f[:, :, :Ng] / unit
     g.p2g(c, (0.5, 0.5, 0.5))
     rb = g.f[:, :, :Ng] / unit
-    # Stated tolerance per stencil term: dithered weight rounding (< 2^-S), the packed
-    # tile-local position (F = PB - log2(T) fraction bits, PB = 19, or 18 for
-    # grids of more than 32^3 tiles, which take the wide sort; so at most
-    # 2^-(F+1) cell away, moving a weight product by <= 0.67 * 2^-(F+1)),
-    # and FP32 weight evaluation (1e-7).  The oracle sees the same float32
-    # positions.  A cell's bound is that times the number of particles
-    # whose 4x4x4 stencil covers it (counted here with numpy).
+    # Stated tolerance per stencil term: dithered weight rounding (< 2^-S),
+    # the packed tile-local position (F = min(23 - log2(T), 24) fraction
+    # bits, so at most 2^-(F+1) cell away, moving a weight product by
+    # <= 0.67 * 2^-(F+1)), and FP32 weight evaluation (1e-7).  The oracle
+    # sees the same float32 positions.  A cell's bound is that times the
+    # number of particles whose 4x4x4 stencil covers it (counted here with
+    # numpy).
     T = min(tile, Ng)
-    nt = -(-Ng // T)
-    F = (19 if nt ** 3 <= 32768 else 18) - int(np.ceil(np.log2(T)))
+    F = min(23 - int(np.ceil(np.log2(T))), 24)
     term = 2.0 ** -S + 0.34 * 2.0 ** -F + 1e-7
     for grid, ref, off in ((a, ra, 0.0), (b, rb, 0.5)):
         cnt = np.zeros((Ng, Ng, Ng))
