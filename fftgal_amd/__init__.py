@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(_HERE, "libfftgal_amd.so")
+# FFTGAL_LIB selects another build of the same library (the host-sanitizer
+# build of fftgal_amd/csrc/Makefile's `asan` target)
+LIBPATH = os.environ.get("FFTGAL_LIB") or os.path.join(_HERE, "libfftgal_amd.so")
 _LIB = None
 
 dptr = C.POINTER(C.c_double)

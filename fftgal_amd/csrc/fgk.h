@@ -88,6 +88,8 @@ int  fgk_fft_fourstep(const fgk_fftplan *p);
 /* tuning knobs: columns per workgroup for the strided passes, rows per
  * workgroup for the row pass (0 = default); generic kernels only */
 int  fgk_fft_set_tuning(fgk_fftplan *p, int cols_per_wg, int rows_per_wg);
+/* the generic kernels' launch shape the plan uses now (for its precision) */
+int  fgk_fft_launch_shape(const fgk_fftplan *p, int *cols, int *rows);
 
 /* ---- painting (fgk_paint.hip) ---- */
 /* FP64 PCS paint with FP64 global atomics (reference order of operations):

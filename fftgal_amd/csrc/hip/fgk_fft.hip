@@ -770,6 +770,16 @@ extern "C" void fgk_fft_plan_destroy(fgk_fftplan *p) {
     free(p);
 }
 
+static int cols_per_wg(const fgk_fftplan *p, size_t elem);
+static int rows_per_wg(const fgk_fftplan *p, size_t elem, int extra_row);
+
+extern "C" int fgk_fft_launch_shape(const fgk_fftplan *p, int *cols, int *rows) {
+    const size_t elem = p->prec == 8 ? sizeof(cpx<double>) : sizeof(cpx<float>);
+    if (cols) *cols = cols_per_wg(p, elem);
+    if (rows) *rows = rows_per_wg(p, elem, 1);
+    return 0;
+}
+
 extern "C" int fgk_fft_set_tuning(fgk_fftplan *p, int cols_per_wg, int rows_per_wg) {
     if (cols_per_wg && (cols_per_wg & (cols_per_wg - 1))) {
         fgk_set_error("cols_per_wg must be a power of two");

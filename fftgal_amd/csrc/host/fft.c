@@ -13,6 +13,7 @@
 #include <math.h>
 #include <time.h>
 #include "internal.h"
+#include "tune.h"
 #include "../../../include/fft.h"
 #include "../../../include/fftgal_ext.h"
 
@@ -82,12 +83,65 @@ void fftgal_dev_push(fft_t *g) {
     }
 }
 
+/* The role of FFTW_MEASURE planning (src/fft.c:36-43): time the FP64
+ * transform pair on the grid's own device buffer with the generic kernels'
+ * default launch shape and with half its columns / rows per workgroup, keep
+ * the fastest (the buffer is zeroed afterwards, as the reference's paint
+ * zeroes after planning, src/fft.c:64). */
+static void measure_fft_shape(fft_dev *d, long n, fftgal_tuning *best) {
+    fftgal_tune_lookup(NULL, 0, FFTGAL_TUNE_FP64_GRID, best);
+    int C = 0, R = 0;
+    DI_CHECK(fgk_fft_launch_shape(d->plan, &C, &R));
+    const int cand[4][2] = {{0, 0}, {C / 2, 0}, {0, R / 2}, {C / 2, R / 2}};
+    fgk_event_t e0 = NULL, e1 = NULL;
+    DI_CHECK(fgk_event_create(&e0));
+    DI_CHECK(fgk_event_create(&e1));
+    float tbest = 1e30f;
+    for (int c = 0; c < 4; ++c) {
+        if ((c & 1) && C < 2) continue;
+        if ((c & 2) && R < 2) continue;
+        DI_CHECK(fgk_fft_set_tuning(d->plan, cand[c][0], cand[c][1]));
+        float ms = 0.0f;
+        for (int rep = 0; rep < 3; ++rep) {   /* the first pair warms up */
+            DI_CHECK(fgk_event_record(e0, d->s));
+            DI_CHECK(fgk_fft_r2c_f64(d->plan, d->d_f, 1.0, d->s));
+            DI_CHECK(fgk_fft_c2r_f64(d->plan, d->d_f, 1.0, d->s));
+            DI_CHECK(fgk_event_record(e1, d->s));
+            DI_CHECK(fgk_event_sync(e1));
+            float m = 0.0f;
+            DI_CHECK(fgk_event_elapsed_ms(&m, e0, e1));
+            if (rep) ms += m;
+        }
+        if (ms < tbest) {
+            tbest = ms;
+            best->fft_cols = cand[c][0];
+            best->fft_rows = cand[c][1];
+        }
+    }
+    DI_CHECK(fgk_fft_set_tuning(d->plan, best->fft_cols, best->fft_rows));
+    DI_CHECK(fgk_fill_zero(d->d_f, (size_t)n * sizeof(double), d->s));
+    DI_CHECK(fgk_stream_sync(d->s));
+    fgk_event_destroy(e0);
+    fgk_event_destroy(e1);
+}
+
 /* src/fft.c:13-47 */
 fft_t *fft_init(int Ng, double L, char *wisdom) {
     if (fftgal_verbose())
         fprintf(stderr, "fft_init() making a %d^3 grid in a %g^3 box\n", Ng, L);
     DI_ASSERT(Ng >= 2 && fgk_fft_supported(Ng),
               "fft_init: Ng must be even with prime factors in {2,3,5,7,11,13}");
+    /* a wisdom file that has no line for this grid will be written after
+     * measuring: fail before any device work if it cannot be (the reference
+     * asserts on the export, src/fft.c:40) */
+    if (wisdom && strcmp(wisdom, "FFTW_ESTIMATE") != 0) {
+        fftgal_tuning t;
+        if (!fftgal_tune_lookup(wisdom, Ng, FFTGAL_TUNE_FP64_GRID, &t)) {
+            FILE *fp = fopen(wisdom, "a");
+            DI_ASSERT(fp != NULL, "fft_init: cannot write the wisdom file");
+            fclose(fp);
+        }
+    }
     DI_CHECK(fgk_init());
     fft_t *self = (fft_t *)malloc(sizeof(fft_t));
     DI_ASSERT(self != NULL, "fft_init: out of memory");
@@ -118,8 +172,22 @@ fft_t *fft_init(int Ng, double L, char *wisdom) {
     d->resident = env_resident();
     self->x2k = self->k2x = d;
 
-    if (wisdom && strcmp(wisdom, "FFTW_ESTIMATE") != 0 && fftgal_verbose())
-        fprintf(stderr, "fft_init() FP64 path has no tuning; wisdom %s unused\n", wisdom);
+    if (wisdom && strcmp(wisdom, "FFTW_ESTIMATE") != 0) {
+        fftgal_tuning t;
+        if (fftgal_tune_lookup(wisdom, Ng, FFTGAL_TUNE_FP64_GRID, &t)) {
+            DI_CHECK(fgk_fft_set_tuning(d->plan, t.fft_cols, t.fft_rows));
+            if (fftgal_verbose()) fprintf(stderr, "fft_init() imported wisdom from %s\n", wisdom);
+        } else {
+            const double t0 = fftgal_now();
+            measure_fft_shape(d, self->Ng3_pad, &t);
+            DI_ASSERT(fftgal_tune_store(wisdom, Ng, FFTGAL_TUNE_FP64_GRID, &t) == 0,
+                      "fft_init: cannot write the wisdom file");
+            if (fftgal_verbose()) {
+                fprintf(stderr, "fft_init() exported wisdom to %s\n", wisdom);
+                fprintf(stderr, "fft_init() %.2fs on measuring\n", fftgal_now() - t0);
+            }
+        }
+    }
     return self;
 }
 

@@ -14,6 +14,11 @@ static void defaults(int Ng, fftgal_tuning *t) {
     t->bin_bk = 0;
 }
 
+/* the interlace field of a line: 0, 1, or the FP64 drop-in grid's marker */
+static int mode_of(int interlace) {
+    return interlace == FFTGAL_TUNE_FP64_GRID ? FFTGAL_TUNE_FP64_GRID : (interlace ? 1 : 0);
+}
+
 static int usable(const char *path) {
     return path && *path && strcmp(path, "FFTW_ESTIMATE") != 0;
 }
@@ -29,7 +34,7 @@ int fftgal_tune_lookup(const char *path, int Ng, int interlace, fftgal_tuning *t
         int n, il, a, b, c, g = 0, k = 0;
         if (line[0] == '#') continue;
         if (sscanf(line, "%d %d %d %d %d %d %d", &n, &il, &a, &b, &c, &g, &k) >= 5 && n == Ng &&
-            il == (interlace ? 1 : 0)) {
+            il == mode_of(interlace)) {
             t->paint_tile = a;
             t->fft_cols = b;
             t->fft_rows = c;
@@ -52,7 +57,7 @@ int fftgal_tune_store(const char *path, int Ng, int interlace, const fftgal_tuni
         while (fgets(line, sizeof line, fp)) {
             int n, il;
             if (line[0] != '#' && sscanf(line, "%d %d", &n, &il) == 2 && n == Ng &&
-                il == (interlace ? 1 : 0))
+                il == mode_of(interlace))
                 continue;
             char **k2 = (char **)realloc(keep, (nkeep + 1) * sizeof(char *));
             if (!k2) break;
@@ -67,7 +72,7 @@ int fftgal_tune_store(const char *path, int Ng, int interlace, const fftgal_tuni
         if (nkeep == 0)
             fprintf(fp, "# Ng interlace paint_tile fft_cols fft_rows fft_generic bin_bk\n");
         for (size_t i = 0; i < nkeep; ++i) fputs(keep[i], fp);
-        fprintf(fp, "%d %d %d %d %d %d %d\n", Ng, interlace ? 1 : 0, t->paint_tile, t->fft_cols,
+        fprintf(fp, "%d %d %d %d %d %d %d\n", Ng, mode_of(interlace), t->paint_tile, t->fft_cols,
                 t->fft_rows, t->fft_generic, t->bin_bk);
         fclose(fp);
     }

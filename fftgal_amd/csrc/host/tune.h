@@ -8,9 +8,13 @@
  * (0 in fft_cols / fft_rows = automatic choice inside the FFT plan; the
  * last two fields may be absent: 0 = specialised FFT kernels where the
  * grid side has them, 0 = default bin block).  fftgal_power_autotune
- * (batch.c) measures the candidates and writes the line. */
+ * (batch.c) measures the candidates and writes the line.  interlace = 2
+ * marks the FP64 grid of the drop-in fft_init, whose line holds only the
+ * FFT launch shape (fft_cols, fft_rows), measured by fft_init itself. */
 #ifndef FFTGAL_TUNE_H
 #define FFTGAL_TUNE_H
+
+#define FFTGAL_TUNE_FP64_GRID 2
 
 typedef struct fftgal_tuning {
     int paint_tile;

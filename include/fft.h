@@ -43,9 +43,11 @@ typedef struct fft {
     double offset[3];  /* grid wrt boundary [0,L)^3 in grid unit */
 } fft_t;
 
-/* src/fft.c:13-47.  wisdom: "FFTW_ESTIMATE" or a tuning cache path (read
- * if present, otherwise written after tuning; the FP64 path has nothing to
- * tune and only records the grid size) */
+/* src/fft.c:13-47.  wisdom: "FFTW_ESTIMATE" (default launch shapes) or a
+ * tuning cache path: a line for this Ng is read if the file holds one;
+ * otherwise the FP64 transform's launch shapes are timed on the new grid
+ * (as FFTW_MEASURE) and the fastest is written to the file, and an
+ * unwritable path aborts, as the reference's failed wisdom export does */
 fft_t *fft_init(int Ng, double L, char *wisdom);
 /* src/fft.c:50-54 */
 void fft_free(fft_t *self);

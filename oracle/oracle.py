@@ -44,7 +44,9 @@ def load():
     global _LIB
     if _LIB is not None:
         return _LIB
-    path = os.path.join(_HERE, "liboracle.so")
+    # FFTGAL_ORACLE_LIB selects another build (the sanitizer build of
+    # oracle/Makefile's `asan` target)
+    path = os.environ.get("FFTGAL_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
     if not os.path.exists(path):
         subprocess.check_call(["make", "-s", "-C", _HERE])
     lib = C.CDLL(path)

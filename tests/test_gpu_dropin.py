@@ -399,3 +399,33 @@ def test_subvolume_flow_vs_oracle(torch_cuda, lib, orc, kind, Nsub):
         assert np.abs(Pg - Po).max() <= 1e-9 * np.abs(Po).max()
     orc.load().oracle_gal_free(so.ptr)
     lib.gal_free(sg.ptr)
+
+
+def test_fft_init_wisdom_measure_export_import(torch_cuda, lib, orc, tmp_path):
+    """fft_init honours `wisdom` like src/fft.c:30-44: a file without a line
+    for this grid is measured and written, a second call imports the line,
+    and the transform is the same FP64 FFT either way."""
+    import fftgal_amd as fa
+    Ng, L = 48, 100.0
+    w = tmp_path / "48.wsdm"
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((Ng, Ng, Ng))
+    ref = orc.Grid(Ng, L)
+    ref.f[:, :, :Ng] = x
+    ref.x2k(0)
+    scale = np.abs(ref.fk()).max()
+    for k in range(2):
+        g = fa.Grid(Ng, L, wisdom=str(w))
+        g.f[:, :, :Ng] = x
+        g.x2k(0)
+        assert np.abs(g.fk() - ref.fk()).max() <= 1e-13 * scale
+        lines = [ln.split() for ln in w.read_text().splitlines()
+                 if ln.strip() and not ln.startswith("#")]
+        assert len(lines) == 1 and lines[0][:2] == ["48", "2"], lines
+        g.free()
+    # a line for another grid side leaves this one to be measured and appended
+    fa.Grid(30, L, wisdom=str(w)).free()
+    sides = sorted(int(ln.split()[0]) for ln in w.read_text().splitlines()
+                   if ln.strip() and not ln.startswith("#"))
+    assert sides == [30, 48]
+

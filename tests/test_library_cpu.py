@@ -186,3 +186,18 @@ def test_loadqpm_rejects_bad_line(lib, tmp_path):
                        cwd=os.path.dirname(INC), timeout=120)
     assert r.returncode != 0
     assert "line 2" in r.stderr
+
+
+def test_fft_init_unwritable_wisdom_aborts(lib, tmp_path):
+    """The reference asserts on a wisdom file it cannot export
+    (src/fft.c:40); fft_init checks that it can write the file before any
+    device work, so the abort shows without a GPU (run in a child process)."""
+    import subprocess
+    import sys
+    bad = tmp_path / "no_such_dir" / "64.wsdm"
+    code = ("import fftgal_amd as fa; fa.Grid(64, 10.0, wisdom=%r); print('returned')" % str(bad))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                       cwd=os.path.dirname(INC), timeout=120)
+    assert r.returncode != 0 and "returned" not in r.stdout
+    assert "cannot write the wisdom file" in r.stderr
+    assert not bad.parent.exists()
