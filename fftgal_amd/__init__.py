@@ -83,6 +83,7 @@ API = {
     "fftgal_NK": (C.c_int, [C.c_int, C.c_double, C.c_double]),
     "Pl_bins": (C.c_int, [C.POINTER(FftT), dptr, C.c_double, dptr, lptr, dptr]),
     "Pmu_bins": (C.c_int, [C.POINTER(FftT), dptr, C.c_double, C.c_int, dptr, lptr, dptr]),
+    "Pl_bins_noalloc": (C.c_int, [C.POINTER(FftT), dptr, C.c_double, dptr, lptr, dptr]),
     # include/fftgal_batch.h
     "fftgal_last_error": (C.c_char_p, []),
     "fftgal_power_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(PowerDesc)]),

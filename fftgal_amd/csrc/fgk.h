@@ -201,6 +201,10 @@ size_t fgk_bins_counts(const fgk_binspec *b);
 /* geometry only: K sums and counts (b->geometry must be 1); P sums untouched */
 int  fgk_bin_geometry(const fgk_binspec *b, double *bins_dev, int64_t *counts_dev,
                       fgk_stream_t s);
+/* the fixed device scratch of the bin sums (1 MiB, one copy per device):
+ * its device address on the current device and its size */
+#define FGK_BIN_SCRATCH_BYTES (1u << 20)
+int  fgk_bin_scratch(void **ptr, size_t *bytes);
 /* bin the FP64 grid holding P(k) in the real slots (after P()) */
 int  fgk_bin_pgrid_f64(const double *f, const fgk_binspec *b, double *bins_dev,
                        int64_t *counts_dev, fgk_stream_t s);

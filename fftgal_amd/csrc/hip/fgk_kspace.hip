@@ -21,6 +21,7 @@
 // so mode counts match the reference exactly.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <mutex>
 #include "../fgk.h"
 #include "fgk_common.h"
 
@@ -872,6 +873,44 @@ extern "C" int fgk_bin_pgrid_f64(const double *f, const fgk_binspec *b, double *
                        (const double *)nullptr, 0.0, 0.0, 0.0, 0.0, bins_dev,
                        (unsigned long long *)counts_dev, (const fgk_binjob *)nullptr);
     FGK_LAUNCHED();
+    return 0;
+}
+
+// Fixed bin scratch of the allocation-free Pl_bins_noalloc: the FP64 sums
+// and int64 counts of fgk_bin_pgrid_f64 for up to FGK_BIN_SCRATCH_BYTES.
+// Each device holds its own copy for the life of the process.  Host code
+// never passes `scratch` itself to a kernel (on the host that name is a
+// stand-in, not a device address): fgk_bin_scratch looks the device address
+// up once per device with hipGetSymbolAddress and hands that out.
+static __device__ __align__(256) unsigned char scratch[FGK_BIN_SCRATCH_BYTES];
+
+#define FGK_SCRATCH_DEVICES 64
+static void *g_scratch_ptr[FGK_SCRATCH_DEVICES];
+static std::mutex g_scratch_mu;
+
+extern "C" int fgk_bin_scratch(void **ptr, size_t *bytes) {
+    *ptr = nullptr;
+    *bytes = 0;
+    int dev = -1;
+    FGK_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= FGK_SCRATCH_DEVICES) {
+        fgk_set_error("fgk_bin_scratch: device %d beyond the %d this library tracks", dev,
+                      FGK_SCRATCH_DEVICES);
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fgk_set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                          hipGetErrorString(e));
+            return -1;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    *bytes = FGK_BIN_SCRATCH_BYTES;
     return 0;
 }
 

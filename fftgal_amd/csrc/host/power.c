@@ -105,6 +105,66 @@ int Pl_bins(fft_t *grid, double los[3], double dK, double *K, long *N, double *P
     return NK;
 }
 
+/* Pl_bins with its device sums in the library's fixed bin scratch instead
+ * of per-call allocations: no hipMalloc/hipFree and no host malloc.  The
+ * raw sums are copied straight into the caller's K, N and P and normalised
+ * there.  Errors are returned (-1, fftgal_last_error()), not aborted on. */
+int Pl_bins_noalloc(fft_t *grid, double los[3], double dK, double *K, long *N, double *P) {
+    if (!grid || !K || !N || !P || !(dK > 0)) {
+        fgk_set_error("Pl_bins_noalloc: null argument or dK <= 0");
+        return -1;
+    }
+    fft_dev *d = fft_dev_of(grid);
+    const int Ng = grid->Ng;
+    const double KF = 2 * M_PI / grid->L;
+    fgk_binspec spec;
+    memset(&spec, 0, sizeof spec);
+    spec.Ng = Ng;
+    spec.NK = fftgal_nk(Ng, grid->L, dK);
+    spec.Nmu = 0;
+    spec.dKinv = KF / dK;
+    spec.do_pl = 1;
+    spec.geometry = 1;
+    const int NK = spec.NK;
+    /* layout with Nmu = 0: Pl sums [NK][4], then K sums [NK]; counts [NK] */
+    const size_t nb = fgk_bins_doubles(&spec), nc = fgk_bins_counts(&spec);
+    const size_t cnt_off = (nb * sizeof(double) + 255) & ~(size_t)255;
+    void *base;
+    size_t cap;
+    if (fgk_bin_scratch(&base, &cap)) return -1;
+    /* 5 doubles and one count per bin, plus at most 255 bytes of alignment */
+    const long nk_max = (long)((cap - 256) / (5 * sizeof(double) + sizeof(int64_t)));
+    if (NK > nk_max) {
+        fgk_set_error("Pl_bins_noalloc: NK = %d bins exceed the %ld that its %zu-byte bin "
+                      "scratch holds", NK, nk_max, cap);
+        return -1;
+    }
+    hat(los);
+    spec.los[0] = los[0];
+    spec.los[1] = los[1];
+    spec.los[2] = los[2];
+    double *dbins = (double *)base;
+    int64_t *dcnt = (int64_t *)((char *)base + cnt_off);
+    if (fgk_fill_zero(base, cnt_off + nc * sizeof(int64_t), d->s)) return -1;
+    fftgal_dev_pull(grid);
+    if (fgk_bin_pgrid_f64(d->d_f, &spec, dbins, dcnt, d->s)) return -1;
+    if (fgk_d2h(P, dbins, (size_t)NK * 4 * sizeof(double), d->s)) return -1;
+    if (fgk_d2h(K, dbins + (size_t)NK * 4, (size_t)NK * sizeof(double), d->s)) return -1;
+    if (fgk_d2h(N, dcnt, (size_t)NK * sizeof(int64_t), d->s)) return -1;
+    if (fgk_stream_sync(d->s)) return -1;
+    long Ntot = 0;
+    for (int iK = 0; iK < NK; ++iK) {
+        K[iK] *= KF / N[iK];
+        for (int l = 0; l < 4; ++l) P[iK * 4 + l] *= (4 * l + 1.0) / N[iK];
+        Ntot += N[iK];
+    }
+    if (Ntot != (long)Ng * Ng * Ng - 1) {
+        fgk_set_error("Pl_bins_noalloc: sum of mode counts %ld != Ng^3-1", Ntot);
+        return -1;
+    }
+    return NK;
+}
+
 /* src/power.c:104-153 as arrays, Nmu wedges */
 int Pmu_bins(fft_t *grid, double los[3], double dK, int Nmu, double *K, long *N, double *P) {
     DI_ASSERT(Nmu >= 1, "Pmu: Nmu must be >= 1");

@@ -39,6 +39,16 @@ void P_params(fft_t *grid, long Np, double V, long Nrand);
 int fftgal_NK(int Ng, double L, double dK);
 /* Pl as arrays: K[NK], N[NK], P[NK*4] (l = 0,2,4,6); returns NK */
 int Pl_bins(fft_t *grid, double los[3], double dK, double *K, long *N, double *P);
+/* No counterpart in the reference.  Pl_bins for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): the same results, with the device bin sums held
+ * in a fixed 1 MiB scratch array that each GPU keeps for the life of the
+ * process, and no device or host allocation.  Limit: NK <= 21840 bins
+ * (fftgal_NK; e.g. Ng = 2048 down to dK = kF/12).  Returns NK, or -1 with
+ * fftgal_last_error() set (too many bins, a failed device call, a count
+ * mismatch) instead of aborting.  Every call on a GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (two threads, two grids) race. */
+int Pl_bins_noalloc(fft_t *grid, double los[3], double dK, double *K, long *N, double *P);
 /* Pmu with Nmu wedges as arrays: K, N, P each [NK*Nmu]; returns NK */
 int Pmu_bins(fft_t *grid, double los[3], double dK, int Nmu, double *K, long *N, double *P);
 
