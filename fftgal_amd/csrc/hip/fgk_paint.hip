@@ -234,12 +234,17 @@ __device__ __forceinline__ int batch_job(long e, const long *__restrict__ off, i
     return lo;
 }
 
-// wrapped plain-grid PCS base cell of one coordinate, its tile and the
-// tile-local position li + d in fixed point with F fraction bits (at most
-// 26 bits in all).  The cell index is taken in FP64 as the reference does
-// (src/fft.c:68-76).
+// Tile and tile-local position of one coordinate.  u = x / H is taken in
+// FP64 as the reference does (src/fft.c:68-76).  A particle belongs to the
+// tile holding the base node of its dual-grid stencil, floor(u - 3/2): its
+// dual stencil then covers tile-local nodes 0 .. T + 2 and its plain stencil
+// (base floor(u) - 1 = floor(u - 3/2 + 1/2)) nodes 0 .. T + 3, so a tile's
+// footprints extend past its upper faces only.  The stored position is
+// v = u - 3/2 - T tc in [0, T), fixed point with F fraction bits (at most 26
+// bits in all): the dual stencil has base floor(v) and fraction frac(v), the
+// plain one base floor(v + 1/2) and fraction frac(v + 1/2).
 __device__ __forceinline__ void axis_key(float xf, const KeyGeo &g, int &tc, uint32_t &q) {
-    const double u = (double)xf * g.Hinv;
+    const double u = __dmul_rn((double)xf, g.Hinv) - 1.5;   // exact subtraction
     const double fl = floor(u);
     const double d = u - fl;
     int gi = (int)fl;
@@ -572,11 +577,25 @@ k_segscan_wide(const uint32_t *__restrict__ S, long nblk, int nd, uint32_t *__re
     if (threadIdx.x == 0) T[d] = carry;
 }
 
+// paint work items of a tile of c particles: the dense items of a tile of
+// more than cap particles (cap particles each, the last fewer); tiles of at
+// most cap particles are painted by their pencil segment
+__host__ __device__ __forceinline__ uint32_t dense_items(uint32_t c, int cap) {
+    return c > (uint32_t)cap ? (c + cap - 1) / cap : 0u;
+}
+
+// paint workgroups of one catalog: its pencil segments and an upper bound of
+// its dense items (a dense tile of c > cap particles has ceil(c / cap) <
+// 2 c / cap items, so all of them have fewer than 2 Np / cap)
+static long paint_work(long nb1, const KeyGeo &g, long Np, int cap) {
+    return nb1 * g.cg0 * g.cg1 + 2 * ((Np + cap - 1) / cap);
+}
+
 // single-workgroup exclusive scan of v[0..n) into base[0..n] and of an
 // auxiliary per-entry count into aux[0..n]:
 //   mode 1: aux = chunks of RB-aligned level-2 work, ceil(v/SORT_CHUNK) rounded up to RB;
 //           also aux_blk = aux / RB
-//   mode 2: aux = paint items, ceil(v/cap)
+//   mode 2: aux = dense paint items (dense_items)
 __global__ void __launch_bounds__(SCAN_THREADS)
 k_digitscan(const uint32_t *__restrict__ v, long n, int mode, int cap,
             uint32_t *__restrict__ base, uint32_t *__restrict__ aux,
@@ -587,7 +606,7 @@ k_digitscan(const uint32_t *__restrict__ v, long n, int mode, int cap,
     const long i1 = i0 + per < n ? i0 + per : n;
     auto aux_of = [mode, cap](uint32_t c) -> uint32_t {
         return mode == 1 ? ((c + SORT_CHUNK - 1) / SORT_CHUNK + RB - 1) / RB * RB
-                         : (c + cap - 1) / cap;
+                         : dense_items(c, cap);
     };
     // up to RP entries per thread: loaded into registers together (one
     // memory latency rather than one per entry)
@@ -678,7 +697,7 @@ __global__ void k_colscan(const uint32_t *__restrict__ H, const uint32_t *__rest
 // Wide two-level sort, level-2 slots: per coarse bucket b (one workgroup),
 // exclusive scans of its tile counts T2[b*nd + s], s < nd, into tilebase
 // (from the bucket's first sorted position base1[b]) and paint items
-// (ceil(count / cap), from 0), and the bucket's item total into bitem[b]
+// (dense_items, from 0), and the bucket's item total into bitem[b]
 __global__ void __launch_bounds__(SCAN_THREADS)
 k_bucket_slots(const uint32_t *__restrict__ T2, const uint32_t *__restrict__ base1, int nb1,
                int nd, int cap, uint32_t *__restrict__ tilebase, uint32_t *__restrict__ items,
@@ -694,7 +713,7 @@ k_bucket_slots(const uint32_t *__restrict__ T2, const uint32_t *__restrict__ bas
     for (int i = i0; i < i1; ++i) {
         const uint32_t c = v[i];
         ca += c;
-        cb += (c + cap - 1) / cap;
+        cb += dense_items(c, cap);
     }
     sa[threadIdx.x] = ca;
     sb[threadIdx.x] = cb;
@@ -713,7 +732,7 @@ k_bucket_slots(const uint32_t *__restrict__ T2, const uint32_t *__restrict__ bas
         aux[i] = rb;
         const uint32_t c = v[i];
         ra += c;
-        rb += (c + cap - 1) / cap;
+        rb += dense_items(c, cap);
     }
     if (threadIdx.x == SCAN_THREADS - 1) {
         bitem[b] = sb[SCAN_THREADS - 1];
@@ -865,29 +884,30 @@ __device__ __forceinline__ void dither_pair(uint64_t pk, uint32_t ext, uint32_t 
 constexpr int brick_n(int side) { return (side + 3) / 4; }
 constexpr int brick_words(int side) { return brick_n(side) * brick_n(side) * brick_n(side) * 64; }
 
-// Flush a bricked LDS footprint of side^3 cells whose cell (0,0,0) sits at
-// grid cell (oi, oj, ok) into the int64 grid with one integer atomic per
-// non-zero cell.  (Plain stores of the cells no other tile reaches were
-// measured twice as slow: they share cache lines with the atomics.)  Each
-// wave takes groups of 64/side whole rows along k (so a wave covers runs of
-// side consecutive grid cells); a lane's k offset is fixed, so the periodic
-// wrap needs no division (footprints extend at most 2 cells below 0 and
-// side cells past the grid's last index).
-template <int SIDE>
+// Flush the planes k < NZ of a bricked LDS footprint of SIDE^2 rows along k,
+// whose cell (0,0,0) sits at grid cell (oi, oj, ok), into the int64 grid
+// with one integer atomic per non-zero cell.  (Plain stores of the cells no
+// other tile reaches were measured twice as slow: they share cache lines
+// with the atomics.)  Each wave takes groups of 64/NZ whole rows (runs of NZ
+// consecutive grid cells); a lane's k offset is fixed, so the periodic wrap
+// needs no division.  With NZ = T = 16 and ok a multiple of 16 every row is
+// one aligned 128-byte line; a row of a whole footprint spans three lines,
+// and the memory-side atomic unit charges per line as well as per atomic.
+template <int SIDE, int NZ>
 __device__ void flush_footprint(const int *__restrict__ fp, int oi, int oj, int ok, int Ng,
                                 int shift, unsigned long long *__restrict__ G) {
-    constexpr int RPG = 64 / SIDE;             // rows per wave step
+    constexpr int RPG = 64 / NZ;               // rows per wave step
     constexpr int NBK = brick_n(SIDE);
     typedef __attribute__((address_space(1))) unsigned long long gu64;
     gu64 *g = (gu64 *)G;
     const int lane = __lane_id();
     const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    const int ro = lane / SIDE, fk = lane - ro * SIDE;
+    const int ro = lane / NZ, fk = lane - ro * NZ;
     if (ro >= RPG) return;
-    // a footprint starts at most 2 cells below 0 and ends at most SIDE past
-    // the grid's last index; grids smaller than that wrap more than once
+    // a footprint ends at most SIDE cells past the grid's last index; grids
+    // smaller than that wrap more than once
     auto wrap = [Ng](int v) {
-        v = v < 0 ? v + Ng : (v >= Ng ? v - Ng : v);
+        v = v >= Ng ? v - Ng : v;
         if ((unsigned)v >= (unsigned)Ng) v = ((v % Ng) + Ng) % Ng;
         return v;
     };
@@ -903,6 +923,21 @@ __device__ void flush_footprint(const int *__restrict__ fp, int oi, int oj, int 
         __hip_atomic_fetch_add(g + ((uint64_t)row * (uint32_t)Ng + (uint32_t)gk),
                                (unsigned long long)((long long)v << shift), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// After a tile of a pencil is flushed (planes k < T): move its planes
+// T .. SIDE-1, the last brick layer (T is a multiple of 4), to layer 0, where
+// they overlap the next tile along k, and clear the other layers.
+template <int SIDE, int T>
+__device__ void carry_down(int *__restrict__ fp) {
+    constexpr int NBK = brick_n(SIDE), L = T / 4;
+    static_assert(T % 4 == 0 && NBK == L + 1, "carry layer");
+    for (int w = threadIdx.x; w < NBK * NBK * 64; w += blockDim.x) {
+        int *col = fp + (w >> 6) * NBK * 64 + (w & 63);   // brick column (i, j), element
+        col[0] = col[L * 64];
+#pragma unroll
+        for (int l = 1; l <= L; ++l) col[l * 64] = 0;
     }
 }
 
@@ -979,9 +1014,9 @@ __device__ __forceinline__ uint32_t pk_axis(uint64_t pk, uint32_t ext, int a) {
     return (hi << POS_EXT) | ((ext >> ((2 - a) * POS_EXT)) & ((1u << POS_EXT) - 1u));
 }
 
-// one grid's stencil of one packed particle word into an LDS footprint:
-// DUAL = 0: plain grid, nodes li-1..li+2, footprint origin tile origin - 1;
-// DUAL = 1: dual grid (offset 1/2), base li or li+1, origin tile origin - 2
+// one grid's stencil of one packed particle word into an LDS footprint whose
+// cell (0,0,0) is the tile's origin: DUAL = 1, base floor(v), fraction
+// frac(v); DUAL = 0 (plain grid), base floor(v + 1/2), fraction frac(v + 1/2)
 template <int NBK, int DUAL>
 __device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, uint32_t ext, int F,
                                            float fsc, float unit, float u, int lane) {
@@ -989,11 +1024,9 @@ __device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, ui
     float d[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        unpack_axis(pk_axis(pk, ext, a), F, fsc, li[a], d[a]);
-        if (DUAL) {
-            if (d[a] >= 0.5f) { d[a] -= 0.5f; li[a] += 1; }
-            else d[a] += 0.5f;
-        }
+        uint32_t q = pk_axis(pk, ext, a);
+        if (!DUAL) q += 1u << (F - 1);
+        unpack_axis(q, F, fsc, li[a], d[a]);
     }
     float wx[4], wy[4], wz[4], wzs[4];
     pcs_w6_f32(d[0], wx);
@@ -1004,17 +1037,53 @@ __device__ __forceinline__ void paint_grid(int *__restrict__ fp, uint64_t pk, ui
     stencil_lds<NBK>(fp, li[0], li[1], li[2], wx, wy, wzs, u, lane);
 }
 
-// Paint of one work item (<= cap particles of one tile): each thread takes
-// particles PAINT_THREADS apart (coalesced loads) and adds its stencils to
-// the plain and the dual LDS footprint as int32 fixed point; the footprints
-// are then flushed.
+// particles [p0, p1) of tile slot t into the plain (fa) and dual (fb) LDS
+// footprints, one particle per thread per step (coalesced loads)
+template <int SA, int SB>
+__device__ __forceinline__ void paint_range(const uint64_t *__restrict__ sorted,
+                                            const uint8_t *__restrict__ sorted_ext, long p0,
+                                            long p1, long t, int *fa, int *fb, int interlace,
+                                            int F, float unit) {
+    const float fsc = 1.0f / (float)(1 << F);
+    const int lane = __lane_id();
+    for (long p = p0 + threadIdx.x; p < p1; p += PAINT_THREADS) {
+        const uint64_t pk = sorted[p];
+        const uint32_t ext = sorted_ext[p];
+        float ua, ub;
+        dither_pair(pk, ext, (uint32_t)t, ua, ub);
+        paint_grid<brick_n(SA), 0>(fa, pk, ext, F, fsc, unit, ua, lane);
+        if (interlace) paint_grid<brick_n(SB), 1>(fb, pk, ext, F, fsc, unit, ub, lane);
+    }
+}
+
+__device__ __forceinline__ float fix_unit(int S) {
+    return FIX_C0 * (float)(1 << (S - FIX_S0));
+}
+
+// Paint of the sorted particles into the int64 grids.  The plain footprint
+// of a tile of side T has T + 4 cells per side, the dual one T + 3, both
+// starting at the tile's origin (see axis_key); they are int32 fixed point
+// in LDS, flushed with integer atomics.  A launch holds two kinds of work:
+//   * pencil segments (work index w < nb1 * cg0 * cg1): the cg2 tiles along
+//     k of one coarse bucket and (i, j) tile column, in order.  After each
+//     tile only its planes k < T are flushed (whole aligned rows); the planes
+//     above are carried down in LDS into the next tile's footprint, and the
+//     segment's last tile flushes its whole footprints.  Tiles of more than
+//     cap particles are left to the second kind (the segment flushes only
+//     what it carries into them).  All tiles of a segment share one fixed
+//     point, chosen from the largest particle count of two neighbouring
+//     tiles: a footprint cell gets terms from particles of its tile and of
+//     the one below it.
+//   * dense items (w >= that count): <= cap particles of a tile with more
+//     than cap particles (items: exclusive prefix of those items over the
+//     tile slots); whole footprints flushed; fixed point from min(count, cap).
 // zbuf (one plan): every launched workgroup first clears its share of
 // zwords words there.
 // zbuf (batch): the launches of one paint stage clear consecutive pieces.
-// Batched (jobs != nullptr): workgroup b paints item ji0[ja] + b of the
-// batch; the job owning it (ji0[job] <= item < ji0[job+1], ja <= job < jb)
-// supplies the geometry, grids (set 0: GA/GB, set 1: GA2/GB2) and tile
-// tables (tilebase, items + job * sl, MAX_D2 slots per coarse bucket).
+// Batched (jobs != nullptr): workgroup b does work ji0[ja] + b of the batch;
+// the job owning it (ji0[job] <= work < ji0[job+1], ja <= job < jb) supplies
+// the geometry, grids (set 0: GA/GB, set 1: GA2/GB2) and tile tables
+// (tilebase, items + job * sl, MAX_D2 slots per coarse bucket).
 template <int T>
 __global__ void __launch_bounds__(PAINT_THREADS, 8)
 k_paint_tiles(const uint64_t *__restrict__ sorted, const uint8_t *__restrict__ sorted_ext,
@@ -1024,7 +1093,8 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint8_t *__restrict__ s
               unsigned long long *__restrict__ GB, const PaintJob *__restrict__ jobs,
               const long *__restrict__ ji0, int ja, int jb, long sl, int set,
               uint64_t *__restrict__ zbuf, long zwords) {
-    constexpr int SA = T + 3, SB = T + 4;
+    constexpr int SA = T + 4, SB = T + 3;
+    constexpr bool CARRY = T % 4 == 0;
     if (zbuf) {   // this workgroup's slice of the memory to clear (16-byte stores)
         const long per = ((zwords + gridDim.x - 1) / gridDim.x + 1) & ~1l;
         const long z0 = blockIdx.x * per, z1 = z0 + per < zwords ? z0 + per : zwords;
@@ -1033,12 +1103,12 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint8_t *__restrict__ s
     }
     constexpr int NA = brick_words(SA), NB = brick_words(SB);
     extern __shared__ int lds[];
-    long it = blockIdx.x;
+    long w = blockIdx.x;
     if (jobs) {
         const long e = ji0[ja] + blockIdx.x;
         const int job = ja + batch_job(e, ji0 + ja, jb - ja);
         const PaintJob &J = jobs[job];
-        it = e - ji0[job];
+        w = e - ji0[job];
         tilebase += job * sl;
         items += job * sl;
         nslots = (long)J.nb1 * MAX_D2;
@@ -1048,45 +1118,91 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint8_t *__restrict__ s
         GA = set ? J.GA2 : J.GA;
         GB = set ? J.GB2 : J.GB;
     }
-    long t, p0, p1;
-    if (!item_range(it, tilebase, items, nslots, cap, t, p0, p1)) return;
-    const int nb = interlace ? NB : 0;
+    const int nl = NA + (interlace ? NB : 0);
     int *fa = lds, *fb = lds + NA;
-    for (int i = threadIdx.x; i < NA + nb; i += blockDim.x) lds[i] = 0;
+    const int pen = g.cg0 * g.cg1;
+    const long nseg = nslots / nb2 * pen;
+    // grid origin of the tile in slot t
+    auto tile_origin = [&g, nb2](long t, int &oi, int &oj, int &ok) {
+        const long b1 = t / nb2, f = t - b1 * nb2;
+        const int cx = (int)(b1 / ((long)g.nc1 * g.nc2)), cy = (int)((b1 / g.nc2) % g.nc1),
+                  cz = (int)(b1 % g.nc2);
+        const int fx = (int)(f / (g.cg1 * g.cg2)), fy = (int)((f / g.cg2) % g.cg1),
+                  fz = (int)(f % g.cg2);
+        oi = (cx * g.cg0 + fx) * T;
+        oj = (cy * g.cg1 + fy) * T;
+        ok = (cz * g.cg2 + fz) * T;
+    };
+    if (w < nseg) {
+        const long b1 = w / pen;
+        const long s0 = b1 * nb2 + ((w - b1 * pen) << g.lg2);   // the segment's first slot
+        int oi, oj, ok0;
+        tile_origin(s0, oi, oj, ok0);
+        if (oi >= g.nt * T || oj >= g.nt * T) return;   // slots past the grid: empty
+        const int nz = g.cg2 < g.nt - ok0 / T ? g.cg2 : g.nt - ok0 / T;
+        long m = 0, prev = 0;
+        for (int q = 0; q < nz; ++q) {
+            long c = (long)tilebase[s0 + q + 1] - (long)tilebase[s0 + q];
+            if (c > cap) c = 0;
+            if (prev + c > m) m = prev + c;
+            prev = c;
+        }
+        if (m == 0) return;   // nothing to paint here (dense tiles are items)
+        const int S = tile_bits(m);
+        const float unit = fix_unit(S);
+        for (int i = threadIdx.x; i < nl; i += blockDim.x) lds[i] = 0;
+        __syncthreads();
+        bool carry = false;   // the carried planes may be non-zero
+        for (int q = 0; q < nz; ++q) {
+            const long t = s0 + q;
+            const long p0 = tilebase[t], c = (long)tilebase[t + 1] - p0;
+            const bool painted = c > 0 && c <= cap;
+            if (!painted && !carry) continue;   // footprints all zero
+            if (painted)
+                paint_range<SA, SB>(sorted, sorted_ext, p0, p0 + c, t, fa, fb, interlace, g.F,
+                                    unit);
+            __syncthreads();
+            const int ok = ok0 + q * T;
+            if (!CARRY || q == nz - 1) {
+                flush_footprint<SA, SA>(fa, oi, oj, ok, g.Ng, FIX_SMAX - S, GA);
+                if (interlace) flush_footprint<SB, SB>(fb, oi, oj, ok, g.Ng, FIX_SMAX - S, GB);
+                if (q == nz - 1) break;
+                __syncthreads();
+                for (int i = threadIdx.x; i < nl; i += blockDim.x) lds[i] = 0;
+            } else {
+                flush_footprint<SA, T>(fa, oi, oj, ok, g.Ng, FIX_SMAX - S, GA);
+                if (interlace) flush_footprint<SB, T>(fb, oi, oj, ok, g.Ng, FIX_SMAX - S, GB);
+                __syncthreads();
+                if constexpr (CARRY) {
+                    carry_down<SA, T>(fa);
+                    if (interlace) carry_down<SB, T>(fb);
+                }
+            }
+            __syncthreads();
+            carry = CARRY && painted;
+        }
+        return;
+    }
+    long t, p0, p1;
+    if (!item_range(w - nseg, tilebase, items, nslots, cap, t, p0, p1)) return;
+    for (int i = threadIdx.x; i < nl; i += blockDim.x) lds[i] = 0;
     __syncthreads();
-
-    const int F = g.F;
-    const float fsc = 1.0f / (float)(1 << F);
-    const int lane = __lane_id();
     // the tile's fixed point, from its particle count (all its items agree)
     const long ntile = (long)tilebase[t + 1] - (long)tilebase[t];
     const int S = tile_bits(ntile < cap ? ntile : cap);
-    const float unit = FIX_C0 * (float)(1 << (S - FIX_S0));
-    for (long p = p0 + threadIdx.x; p < p1; p += PAINT_THREADS) {
-        const uint64_t pk = sorted[p];
-        const uint32_t ext = sorted_ext[p];
-        float ua, ub;
-        dither_pair(pk, ext, (uint32_t)t, ua, ub);
-        paint_grid<brick_n(SA), 0>(fa, pk, ext, F, fsc, unit, ua, lane);
-        if (interlace) paint_grid<brick_n(SB), 1>(fb, pk, ext, F, fsc, unit, ub, lane);
-    }
+    paint_range<SA, SB>(sorted, sorted_ext, p0, p1, t, fa, fb, interlace, g.F, fix_unit(S));
     __syncthreads();
-    const long b1 = t / nb2, f = t - b1 * nb2;
-    const int cx = (int)(b1 / ((long)g.nc1 * g.nc2)), cy = (int)((b1 / g.nc2) % g.nc1),
-              cz = (int)(b1 % g.nc2);
-    const int fx = (int)(f / (g.cg1 * g.cg2)), fy = (int)((f / g.cg2) % g.cg1),
-              fz = (int)(f % g.cg2);
-    const int ti = cx * g.cg0 + fx, tj = cy * g.cg1 + fy, tk = cz * g.cg2 + fz;
-    flush_footprint<SA>(fa, ti * T - 1, tj * T - 1, tk * T - 1, g.Ng, FIX_SMAX - S, GA);
-    if (interlace)
-        flush_footprint<SB>(fb, ti * T - 2, tj * T - 2, tk * T - 2, g.Ng, FIX_SMAX - S, GB);
+    int oi, oj, ok;
+    tile_origin(t, oi, oj, ok);
+    flush_footprint<SA, SA>(fa, oi, oj, ok, g.Ng, FIX_SMAX - S, GA);
+    if (interlace) flush_footprint<SB, SB>(fb, oi, oj, ok, g.Ng, FIX_SMAX - S, GB);
 }
 
 // ------------------------------------------------------------------ host
 
 // the bricked LDS footprints of one work item
 static int paint_lds_bytes(int T, int interlace) {
-    return (brick_words(T + 3) + (interlace ? brick_words(T + 4) : 0)) * 4;
+    return (brick_words(T + 4) + (interlace ? brick_words(T + 3) : 0)) * 4;
 }
 
 static int ceil_log2(long v) {
@@ -1188,7 +1304,7 @@ extern "C" int fgk_paint_plan_create(fgk_paintplan **out, int Ng, int tile, long
     pp->bits2 = ceil_log2(pp->nb2);
     pp->nslots = (long)pp->nb1 * pp->nb2;
     // one paint workgroup per work item; a dispatch has fewer than 2^32 work-items
-    if ((pp->nslots + (Np_max + pp->cap - 1) / pp->cap + 8) * (long)PAINT_THREADS >= (1l << 32)) {
+    if ((paint_work(pp->nb1, pp->g, Np_max, pp->cap) + 8) * (long)PAINT_THREADS >= (1l << 32)) {
         fgk_set_error("paint: %ld tiles of side %d and %ld particles exceed one dispatch",
                       pp->nslots, T, Np_max);
         free(pp);
@@ -1427,7 +1543,7 @@ extern "C" int fgk_paint_stage(fgk_paintplan *pp, int stage, const float *x, con
         }
     }
     if (stage == 6) {
-        const long max_items = pp->nslots + (Np + pp->cap - 1) / pp->cap;
+        const long max_items = paint_work(pp->nb1, g, Np, pp->cap);
         const size_t lds = (size_t)paint_lds_bytes(pp->T, pp->interlace);
         const dim3 grid((unsigned)max_items), block(PAINT_THREADS);
         unsigned long long *A = (unsigned long long *)gridA, *B = (unsigned long long *)gridB;
@@ -1483,7 +1599,7 @@ extern "C" int fgk_paint_fixed(fgk_paintplan *pp, const float *x, const float *y
 
 // per catalog j (one workgroup each): exclusive scans of its slot counts
 // T2[j][s], s < nb1_j * MAX_D2, into tilebase (starting at the catalog's
-// first sorted position base1[j * MAX_D1]) and paint items (ceil(count/cap))
+// first sorted position base1[j * MAX_D1]) and dense paint items (dense_items)
 __global__ void __launch_bounds__(SCAN_THREADS)
 k_digitscan_jobs(const uint32_t *__restrict__ T2, const uint32_t *__restrict__ base1,
                  const PaintJob *__restrict__ jobs, int cap, uint32_t *__restrict__ tilebase,
@@ -1501,7 +1617,7 @@ k_digitscan_jobs(const uint32_t *__restrict__ T2, const uint32_t *__restrict__ b
     for (long i = i0; i < i1; ++i) {
         const uint32_t c = v[i];
         ca += c;
-        cb += (c + cap - 1) / cap;
+        cb += dense_items(c, cap);
     }
     sa[threadIdx.x] = ca;
     sb[threadIdx.x] = cb;
@@ -1520,7 +1636,7 @@ k_digitscan_jobs(const uint32_t *__restrict__ T2, const uint32_t *__restrict__ b
         aux[i] = rb;
         const uint32_t c = v[i];
         ra += c;
-        rb += (c + cap - 1) / cap;
+        rb += dense_items(c, cap);
     }
     if (threadIdx.x == SCAN_THREADS - 1) {
         base[n] = b0 + sa[SCAN_THREADS - 1];
@@ -1654,7 +1770,7 @@ extern "C" int fgk_paint_batch_create(fgk_paintbatch **out, int njobs,
         pb->hc1[q] = c1;
         c1 += round_up((d.Np_max + SORT_CHUNK - 1) / SORT_CHUNK, RB);
         pb->hi0[q] = i0;
-        i0 += (long)J.nb1 * (g.cg0 * g.cg1 * g.cg2) + (d.Np_max + pb->cap - 1) / pb->cap;
+        i0 += paint_work(J.nb1, g, d.Np_max, pb->cap);
     }
     pb->hc1[njobs] = c1;
     pb->hi0[njobs] = i0;
