@@ -198,8 +198,19 @@ typedef struct fgk_binspec {
 } fgk_binspec;
 size_t fgk_bins_doubles(const fgk_binspec *b);
 size_t fgk_bins_counts(const fgk_binspec *b);
-/* geometry only: K sums and counts (b->geometry must be 1); P sums untouched */
-int  fgk_bin_geometry(const fgk_binspec *b, double *bins_dev, int64_t *counts_dev,
+/* Sums of the fast path (fgk_bin_geometry, fgk_fused_bin_f32*) are exact
+ * fixed point: value i of the fgk_bins_doubles layout is the int64 pair
+ * (H, L) at 2i, 2i+1, worth (H 2^32 + L) 2^(e - FGK_BIN_FIX_BITS), e the
+ * binary exponent (ilogb) of the catalog's Pshot for power sums,
+ * FGK_BIN_K_EXP for the geometry's K sums (fgk_bin_fixed_value) */
+#define FGK_BIN_FIX_BITS 30
+#define FGK_BIN_K_EXP (-22)
+double fgk_bin_fixed_value(const int64_t *hl, int e);
+/* e of a catalog's power sums: ilogb(Pshot), 0 unless 0 < Pshot < 1e300 */
+int fgk_bin_fixed_exponent(double Pshot);
+/* geometry only: K sums (fixed point) and counts (b->geometry must be 1);
+ * P sums untouched */
+int  fgk_bin_geometry(const fgk_binspec *b, int64_t *sums_dev, int64_t *counts_dev,
                       fgk_stream_t s);
 /* the fixed device scratch of the bin sums (1 MiB, one copy per device):
  * its device address on the current device and its size */
@@ -220,7 +231,7 @@ int  fgk_bin_pgrid_f64(const double *f, const fgk_binspec *b, double *bins_dev,
 int  fgk_fused_bin_f32(const float *A, const float *B, const fgk_binspec *b,
                        const double *phase_dev, const double *winv_dev,
                        double norm, double Vinv, double Pshot, double npart,
-                       double *bins_dev, fgk_stream_t s);
+                       int64_t *sums_dev, fgk_stream_t s);
 
 /* one catalog of a batched fused bin over grids of one side: packed
  * spectra A, B (B NULL: no interlacing), its bin sums (layout as above for
@@ -228,7 +239,7 @@ int  fgk_fused_bin_f32(const float *A, const float *B, const fgk_binspec *b,
  * fgk_fused_bin_f32 */
 typedef struct fgk_binjob {
     const float *A, *B;
-    double *bins;
+    int64_t *sums;         /* fixed-point pairs */
     int NK;
     double dKinv;
     double norm, Vinv, Pshot, npart;

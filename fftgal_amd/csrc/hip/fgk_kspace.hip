@@ -268,6 +268,13 @@ extern "C" size_t fgk_bins_doubles(const fgk_binspec *b) {
     return n;
 }
 
+// (H 2^32 + L) 2^(e - FGK_BIN_FIX_BITS) of one fixed-point pair, exact up
+// to the one rounding to FP64
+extern "C" double fgk_bin_fixed_value(const int64_t *hl, int e) {
+    const __int128 y = ((__int128)hl[0] << 32) + (__int128)(uint64_t)hl[1];
+    return ldexp((double)y, e - FGK_BIN_FIX_BITS);
+}
+
 extern "C" size_t fgk_bins_counts(const fgk_binspec *b) {
     size_t W = b->Nmu > 0 ? (size_t)b->Nmu + 1 : 0;
     return b->geometry ? (size_t)b->NK * (1 + W) : 0;
@@ -283,6 +290,7 @@ struct BinArgs {
     int ncnt;         // counts per bin in LDS
     int wmax;         // LDS window capacity in bins
     int fs_n1;        // spectra planes in four-step order (0: natural)
+    int fx;           // sums: 0 FP64; 1, 2 exact fixed point (see fix_int)
     double dKinv;
     double l0, l1, l2;
 };
@@ -295,6 +303,70 @@ __device__ __forceinline__ int iplane(const BinArgs &a, int i) {
 // signed frequency of index i with the Nyquist index carrying -Ng/2
 // (src/power.c:46-51)
 __device__ __forceinline__ int kval(int i, int Ng) { return (2 * i < Ng) ? i : i - Ng; }
+
+// Exact bin sums.  With a.fx set, a value v is added as the integer
+// Y = rint(v * qs), qs = 2^(FGK_BIN_FIX_BITS - e), e the binary exponent of
+// the catalog's Pshot (FGK_BIN_K_EXP for the geometry's K sums, in units of
+// KF).  The global sums are pairs of int64 accumulators (H, L), Y adding
+// H = floor(Y / 2^32) and L = Y - H 2^32 in [0, 2^32), so every sum is an
+// exact integer whatever the order of the atomics and reruns give
+// bit-identical bins; the host forms (H_sum 2^32 + L_sum) / qs.  The
+// quantum is 2^-30 of Pshot's power of two for power sums (the rounding of
+// one mode is below 1e-9 Pshot) and 2^-52 KF for K sums.
+//   fx = 1 (fast path): the LDS window holds one int64 per value, Y from one
+//     FMA against a magic constant while |Y| < 2^50 (a block adds at most
+//     1024 such values to one entry, so the entry stays below 2^60), split
+//     into (H, L) when the window is flushed; larger values (|v| above about
+//     2^20 Pshot) go to the global pair directly.
+//   fx = 2 (geometry): the LDS window holds pairs, every value split.
+__host__ __device__ __forceinline__ int fix_exponent(double Pshot) {
+    return (Pshot > 0.0 && Pshot < 1e300) ? ilogb(Pshot) : 0;
+}
+
+__device__ __forceinline__ double fix_scale(int e) { return ldexp(1.0, FGK_BIN_FIX_BITS - e); }
+
+extern "C" int fgk_bin_fixed_exponent(double Pshot) { return fix_exponent(Pshot); }
+
+// Y of any magnitude into a pair (H, L); out of line, off the common path
+__device__ __attribute__((noinline)) void pair_add(unsigned long long *acc, double v, double qs) {
+#pragma clang fp contract(off)
+    const double Y = rint(v * qs);
+    const double H = floor(Y * 0x1p-32);
+    const unsigned long long h = (unsigned long long)(long long)H;
+    const unsigned long long l = (unsigned long long)(unsigned)(Y - H * 0x1p32);
+    if (h) atomicAdd(acc, h);
+    if (l) atomicAdd(acc + 1, l);
+}
+
+// rint(v * qs) for |v * qs| < 2^50: the FMA rounds v * qs + 1.5 2^52 once to
+// an integer (unit spacing there), whose bits less those of 1.5 2^52 are Y
+__device__ __forceinline__ unsigned long long fix_int(double v, double qs) {
+    const double M = 0x1.8p52;
+    return (unsigned long long)(__double_as_longlong(__builtin_fma(v, qs, M)) -
+                                __double_as_longlong(M));
+}
+
+// int64 words of one value in the LDS window
+__device__ __host__ __forceinline__ int lds_words(int fx) { return fx == 2 ? 2 : 1; }
+
+// add v to a bin's sums: entry lidx of its LDS window row lrow, or (lrow
+// null) entry gidx of the global sums
+__device__ __forceinline__ void put(const BinArgs &a, void *gbins, double qs, void *lrow,
+                                    int lidx, size_t gidx, double v) {
+    if (a.fx == 0) {
+        atomicAdd(lrow ? (double *)lrow + lidx : (double *)gbins + gidx, v);
+        return;
+    }
+    if (a.fx == 1 && lrow) {
+        if (fabs(v * qs) < 0x1p50) {
+            atomicAdd((unsigned long long *)lrow + lidx, fix_int(v, qs));
+            return;
+        }
+        lrow = nullptr;
+    }
+    pair_add(lrow ? (unsigned long long *)lrow + 2 * lidx
+                  : (unsigned long long *)gbins + 2 * gidx, v, qs);
+}
 
 // smallest and largest |frequency| over an index range [a, b) of one axis
 __device__ __forceinline__ void axis_range(int a, int b, int Ng, int &lo, int &hi) {
@@ -319,12 +391,13 @@ __device__ __forceinline__ void axis_range(int a, int b, int Ng, int &lo, int &h
 // its Nyquist modes) bins [blo2, blo2 + wtot - wn) -- or, when the windows
 // do not fit, straight into the global bins
 struct BinOut {
-    double *sv;
+    void *sv;                  // LDS sums: FP64, or int64 pairs (a.fx)
     unsigned int *sc;
     int blo, wn, blo2, wtot;
     bool windowed;
-    double *bins;
+    void *bins;                // global sums, the same form
     unsigned long long *cnts;
+    double qs;                 // fixed-point scale (a.fx)
 };
 
 // bin, wedge and sums of one mode of power P, given its count (1 on the
@@ -362,12 +435,12 @@ __device__ __forceinline__ bool mode_vals(const BinArgs &a, int count, double Ka
 
 // LDS window row of bin iK (its counts in *crow), or nullptr when the bin
 // lies outside the window and goes straight to the global bins
-__device__ __forceinline__ double *window_row(const BinArgs &a, const BinOut &o, int iK, int win,
-                                              unsigned int **crow) {
+__device__ __forceinline__ void *window_row(const BinArgs &a, const BinOut &o, int iK, int win,
+                                            unsigned int **crow) {
     const int slot = win ? o.wn + iK - o.blo2 : iK - o.blo;
     if (o.windowed && slot >= (win ? o.wn : 0) && slot < (win ? o.wtot : o.wn)) {
         *crow = o.sc + (size_t)slot * a.ncnt;
-        return o.sv + (size_t)slot * a.nvals;
+        return (void *)((unsigned long long *)o.sv + (size_t)slot * a.nvals * lds_words(a.fx));
     }
     return nullptr;
 }
@@ -380,23 +453,15 @@ template <bool SUMS>
 __device__ __forceinline__ void emit_bin(const BinArgs &a, const BinOut &o, int iK,
                                          const double pl[4], double kc, int count, int win) {
     const int Wm = a.Nmu > 0 ? a.Nmu + 1 : 0;
-    unsigned int *c;
-    double *v = window_row(a, o, iK, win, &c);
-    if (v) {
-        if (a.do_pl && SUMS)
-            for (int l = 0; l < 4; ++l) atomicAdd(v + l, pl[l]);
-        if (a.geometry) {
-            atomicAdd(v + (a.do_pl ? 4 : 0) + Wm, kc);
-            atomicAdd(c, (unsigned)count);
-        }
-        return;
-    }
+    unsigned int *c = nullptr;
+    void *v = window_row(a, o, iK, win, &c);
     const size_t offK = (size_t)a.NK * 4 + (size_t)a.NK * Wm;
     if (a.do_pl && SUMS)
-        for (int l = 0; l < 4; ++l) atomicAdd(o.bins + (size_t)iK * 4 + l, pl[l]);
+        for (int l = 0; l < 4; ++l) put(a, o.bins, o.qs, v, l, (size_t)iK * 4 + l, pl[l]);
     if (a.geometry) {
-        atomicAdd(o.bins + offK + iK, kc);
-        atomicAdd(o.cnts + iK, (unsigned long long)count);
+        put(a, o.bins, o.qs, v, (a.do_pl ? 4 : 0) + Wm, offK + iK, kc);
+        if (v) atomicAdd(c, (unsigned)count);
+        else atomicAdd(o.cnts + iK, (unsigned long long)count);
     }
 }
 
@@ -406,23 +471,16 @@ __device__ __forceinline__ void emit_wedge(const BinArgs &a, const BinOut &o, in
                                            double d2, double kc, int count, int win) {
     const int Wm = a.Nmu > 0 ? a.Nmu + 1 : 0;
     if (!Wm) return;
-    unsigned int *c;
-    double *v = window_row(a, o, iK, win, &c);
-    if (v) {
-        const int q = a.do_pl ? 4 : 0;
-        if (SUMS) atomicAdd(v + q + imu, d2);
-        if (a.geometry) {
-            atomicAdd(v + q + Wm + 1 + imu, kc);
-            atomicAdd(c + 1 + imu, (unsigned)count);
-        }
-        return;
-    }
+    unsigned int *c = nullptr;
+    void *v = window_row(a, o, iK, win, &c);
+    const int q = a.do_pl ? 4 : 0;
     const size_t offPmu = (size_t)a.NK * 4;
     const size_t offKmu = offPmu + (size_t)a.NK * Wm + (size_t)a.NK;
-    if (SUMS) atomicAdd(o.bins + offPmu + (size_t)iK * Wm + imu, d2);
+    if (SUMS) put(a, o.bins, o.qs, v, q + imu, offPmu + (size_t)iK * Wm + imu, d2);
     if (a.geometry) {
-        atomicAdd(o.bins + offKmu + (size_t)iK * Wm + imu, kc);
-        atomicAdd(o.cnts + a.NK + (size_t)iK * Wm + imu, (unsigned long long)count);
+        put(a, o.bins, o.qs, v, q + Wm + 1 + imu, offKmu + (size_t)iK * Wm + imu, kc);
+        if (v) atomicAdd(c + 1 + imu, (unsigned)count);
+        else atomicAdd(o.cnts + a.NK + (size_t)iK * Wm + imu, (unsigned long long)count);
     }
 }
 
@@ -539,34 +597,65 @@ __device__ __forceinline__ void bin_window(const BinArgs &a, int lo0, int hi0, i
 
 __device__ __forceinline__ void window_clear(const BinArgs &a, const BinOut &o) {
     if (o.windowed) {
-        for (int t = threadIdx.x; t < o.wtot * a.nvals; t += blockDim.x) o.sv[t] = 0.0;
+        for (int t = threadIdx.x; t < lds_words(a.fx) * o.wtot * a.nvals; t += blockDim.x)
+            ((unsigned long long *)o.sv)[t] = 0ull;   // also FP64 +0
         for (int t = threadIdx.x; t < o.wtot * a.ncnt; t += blockDim.x) o.sc[t] = 0u;
     }
     __syncthreads();
 }
 
-// one global atomic per non-zero LDS window entry
-__device__ __forceinline__ void window_flush(const BinArgs &a, const BinOut &o) {
-    if (!o.windowed) return;
-    __syncthreads();
+// global index of entry q of LDS window row w
+__device__ __forceinline__ size_t flush_index(const BinArgs &a, const BinOut &o, int w, int q) {
     const int Wm = a.Nmu > 0 ? a.Nmu + 1 : 0;
     const size_t offPmu = (size_t)a.NK * 4;
     const size_t offK = offPmu + (size_t)a.NK * Wm;
     const size_t offKmu = offK + (size_t)a.NK;
-    for (int t = threadIdx.x; t < o.wtot * a.nvals; t += blockDim.x) {
-        const double v = o.sv[t];
-        if (v == 0.0) continue;
-        const int w = t / a.nvals, q = t - w * a.nvals;
-        const int iK = w < o.wn ? o.blo + w : o.blo2 + w - o.wn;
-        int qq = q;
-        if (a.do_pl) {
-            if (qq < 4) { atomicAdd(o.bins + (size_t)iK * 4 + qq, v); continue; }
-            qq -= 4;
+    const int iK = w < o.wn ? o.blo + w : o.blo2 + w - o.wn;
+    if (a.do_pl) {
+        if (q < 4) return (size_t)iK * 4 + q;
+        q -= 4;
+    }
+    if (q < Wm) return offPmu + (size_t)iK * Wm + q;
+    q -= Wm;
+    if (q == 0) return offK + iK;
+    return offKmu + (size_t)iK * Wm + (q - 1);
+}
+
+// one global atomic per non-zero LDS window entry (fixed point: per
+// non-zero half of the pair; integer sums need no conversion)
+__device__ __forceinline__ void window_flush(const BinArgs &a, const BinOut &o) {
+    if (!o.windowed) return;
+    __syncthreads();
+    const int Wm = a.Nmu > 0 ? a.Nmu + 1 : 0;
+    if (a.fx == 2) {
+        const unsigned long long *sv = (const unsigned long long *)o.sv;
+        unsigned long long *gb = (unsigned long long *)o.bins;
+        for (int t = threadIdx.x; t < 2 * o.wtot * a.nvals; t += blockDim.x) {
+            const unsigned long long v = sv[t];
+            if (!v) continue;
+            const int e = t >> 1, w = e / a.nvals;
+            atomicAdd(gb + 2 * flush_index(a, o, w, e - w * a.nvals) + (t & 1), v);
         }
-        if (qq < Wm) { atomicAdd(o.bins + offPmu + (size_t)iK * Wm + qq, v); continue; }
-        qq -= Wm;
-        if (qq == 0) { atomicAdd(o.bins + offK + iK, v); continue; }
-        atomicAdd(o.bins + offKmu + (size_t)iK * Wm + (qq - 1), v);
+    } else if (a.fx == 1) {
+        // S = H 2^32 + L, L = the low 32 bits (two's complement)
+        const long long *sv = (const long long *)o.sv;
+        unsigned long long *gb = (unsigned long long *)o.bins;
+        for (int t = threadIdx.x; t < o.wtot * a.nvals; t += blockDim.x) {
+            const long long v = sv[t];
+            if (!v) continue;
+            const int w = t / a.nvals;
+            unsigned long long *g = gb + 2 * flush_index(a, o, w, t - w * a.nvals);
+            const unsigned long long h = (unsigned long long)(v >> 32), l = (unsigned long long)v & 0xffffffffull;
+            if (h) atomicAdd(g, h);
+            if (l) atomicAdd(g + 1, l);
+        }
+    } else {
+        for (int t = threadIdx.x; t < o.wtot * a.nvals; t += blockDim.x) {
+            const double v = ((const double *)o.sv)[t];
+            if (v == 0.0) continue;
+            const int w = t / a.nvals;
+            atomicAdd((double *)o.bins + flush_index(a, o, w, t - w * a.nvals), v);
+        }
     }
     if (a.geometry) {
         for (int t = threadIdx.x; t < o.wtot * a.ncnt; t += blockDim.x) {
@@ -581,8 +670,8 @@ __device__ __forceinline__ void window_flush(const BinArgs &a, const BinOut &o) 
 }
 
 __device__ __forceinline__ BinOut make_out(const BinArgs &a, double *sh, int blo, int bhi,
-                                           int blo2, int bhi2, double *bins,
-                                           unsigned long long *cnts) {
+                                           int blo2, int bhi2, void *bins,
+                                           unsigned long long *cnts, double qs) {
     BinOut o;
     o.blo = blo;
     o.wn = bhi - blo + 1;
@@ -590,9 +679,10 @@ __device__ __forceinline__ BinOut make_out(const BinArgs &a, double *sh, int blo
     o.wtot = o.wn + (bhi2 >= blo2 ? bhi2 - blo2 + 1 : 0);
     o.windowed = o.wtot <= a.wmax;
     o.sv = sh;
-    o.sc = (unsigned int *)(sh + (size_t)a.wmax * a.nvals);
+    o.sc = (unsigned int *)(sh + (size_t)a.wmax * a.nvals * lds_words(a.fx));
     o.bins = bins;
     o.cnts = cnts;
+    o.qs = qs;
     return o;
 }
 
@@ -609,7 +699,7 @@ __device__ __forceinline__ BinOut make_out(const BinArgs &a, double *sh, int blo
         const fgk_binjob &J = jobs[blockIdx.y]; \
         A = J.A;                           \
         B = J.B;                           \
-        bins = J.bins;                     \
+        bins = J.sums;                     \
         a.NK = J.NK;                       \
         a.dKinv = J.dKinv;                 \
         norm = J.norm;                     \
@@ -625,10 +715,11 @@ template <int MODE, int BKT>
 __global__ void __launch_bounds__(BIN_THREADS, (BKT <= 32 ? 5 : 1))
 k_bin(const double *__restrict__ fP, const float *__restrict__ A, const float *__restrict__ B,
       BinArgs a, const double *__restrict__ ph, const double *__restrict__ winv, double norm,
-      double Vinv, double Pshot, double npart, double *__restrict__ bins,
+      double Vinv, double Pshot, double npart, void *__restrict__ bins,
       unsigned long long *__restrict__ cnts, const fgk_binjob *__restrict__ jobs) {
 #pragma clang fp contract(off)
     extern __shared__ double sh[];
+    a.fx = MODE == 1 ? 1 : (MODE == 2 ? 2 : 0);   // as the host sized the LDS; constant here
     if (MODE == 1 && jobs) BIN_JOB(jobs);
     constexpr int BI = BinBlock<BKT>::BI, BJ = BinBlock<BKT>::BJ, BK = BinBlock<BKT>::BK;
     const int Ng = a.Ng;
@@ -649,7 +740,9 @@ k_bin(const double *__restrict__ fP, const float *__restrict__ A, const float *_
     axis_range(j0, j1, Ng, lo1, hi1);
     axis_range(k0, k1, Ng, lo2, hi2);
     bin_window(a, lo0, hi0, lo1, hi1, lo2, hi2, blo, bhi);
-    const BinOut o = make_out(a, sh, blo, bhi, 0, -1, bins, cnts);
+    const BinOut o = make_out(a, sh, blo, bhi, 0, -1, bins, cnts,
+                              a.fx ? fix_scale(MODE == 1 ? fix_exponent(Pshot) : FGK_BIN_K_EXP)
+                                   : 0.0);
     window_clear(a, o);
 
     // each thread takes KPT consecutive k of the rows it visits
@@ -752,10 +845,11 @@ k_bin(const double *__restrict__ fP, const float *__restrict__ A, const float *_
 __global__ void __launch_bounds__(PI * PJ)
 k_bin_plane(const float *__restrict__ A, const float *__restrict__ B, BinArgs a,
             const double *__restrict__ ph, const double *__restrict__ winv, double norm,
-            double Vinv, double Pshot, double npart, double *__restrict__ bins,
+            double Vinv, double Pshot, double npart, void *__restrict__ bins,
             const fgk_binjob *__restrict__ jobs) {
 #pragma clang fp contract(off)
     extern __shared__ double sh[];
+    a.fx = 1;   // as the host sized the LDS; constant here
     if (jobs) BIN_JOB(jobs);
     const int Ng = a.Ng;
     const int Mh = Ng / 2;
@@ -767,7 +861,8 @@ k_bin_plane(const float *__restrict__ A, const float *__restrict__ B, BinArgs a,
     axis_range(j0, j1, Ng, lo1, hi1);
     bin_window(a, lo0, hi0, lo1, hi1, 0, 0, blo, bhi);
     bin_window(a, lo0, hi0, lo1, hi1, Mh, Mh, blo2, bhi2);
-    const BinOut o = make_out(a, sh, blo, bhi, blo2, bhi2, bins, nullptr);
+    const BinOut o = make_out(a, sh, blo, bhi, blo2, bhi2, bins, nullptr,
+                              fix_scale(fix_exponent(Pshot)));
     window_clear(a, o);
     const int i = i0 + threadIdx.x / PJ, j = j0 + threadIdx.x % PJ;
     const double na = norm, nb = norm;
@@ -805,7 +900,7 @@ static int bin_bk(const fgk_binspec *b) {
     return (b->bk == 32 || b->bk == 64 || b->bk == 128) ? b->bk : 16;
 }
 
-static BinArgs make_args(const fgk_binspec *b) {
+static BinArgs make_args(const fgk_binspec *b, int fx) {
     const int BK = bin_bk(b);
     BinArgs a;
     a.Ng = b->Ng;
@@ -816,7 +911,8 @@ static BinArgs make_args(const fgk_binspec *b) {
     const int Wm = b->Nmu > 0 ? b->Nmu + 1 : 0;
     a.nvals = (b->do_pl ? 4 : 0) + Wm + (b->geometry ? 1 + Wm : 0);
     a.ncnt = b->geometry ? 1 + Wm : 0;
-    size_t per = (size_t)a.nvals * 8 + (size_t)a.ncnt * 4;
+    a.fx = fx;
+    size_t per = (size_t)a.nvals * 8 * lds_words(fx) + (size_t)a.ncnt * 4;
     // room for the widest window of a BI x BJ x BK block near the grid's
     // corner (about BK + 4 bins at dK = KF) even with many wedges
     a.wmax = per ? (int)(BIN_LDS_BYTES / per) : 1;
@@ -859,14 +955,14 @@ template <int BKT> static unsigned bin_blocks(int Ng, int Nk) {
     } while (0)
 
 static size_t bin_lds(const BinArgs &a) {
-    return (size_t)a.wmax * a.nvals * 8 + (size_t)a.wmax * a.ncnt * 4 + 16;
+    return (size_t)a.wmax * a.nvals * 8 * lds_words(a.fx) + (size_t)a.wmax * a.ncnt * 4 + 16;
 }
 
 extern "C" int fgk_bin_pgrid_f64(const double *f, const fgk_binspec *b, double *bins_dev,
                                  int64_t *counts_dev, fgk_stream_t s) {
     fgk_binspec b64 = *b;
     b64.bk = 64;             // MODE 0 and 2 run the default block
-    BinArgs a = make_args(&b64);
+    BinArgs a = make_args(&b64, 0);
     hipLaunchKernelGGL((k_bin<0, 64>), dim3(bin_blocks<64>(b->Ng, b->Ng / 2 + 1)), dim3(BIN_THREADS),
                        bin_lds(a), (hipStream_t)s, f, (const float *)nullptr,
                        (const float *)nullptr, a, (const double *)nullptr,
@@ -914,7 +1010,7 @@ extern "C" int fgk_bin_scratch(void **ptr, size_t *bytes) {
     return 0;
 }
 
-extern "C" int fgk_bin_geometry(const fgk_binspec *b, double *bins_dev, int64_t *counts_dev,
+extern "C" int fgk_bin_geometry(const fgk_binspec *b, int64_t *sums_dev, int64_t *counts_dev,
                                 fgk_stream_t s) {
     if (!b->geometry) {
         fgk_set_error("fgk_bin_geometry: binspec without geometry");
@@ -922,33 +1018,34 @@ extern "C" int fgk_bin_geometry(const fgk_binspec *b, double *bins_dev, int64_t 
     }
     fgk_binspec b64 = *b;
     b64.bk = 64;
-    BinArgs a = make_args(&b64);
+    BinArgs a = make_args(&b64, 2);
     hipLaunchKernelGGL((k_bin<2, 64>), dim3(bin_blocks<64>(b->Ng, b->Ng / 2 + 1)), dim3(BIN_THREADS),
                        bin_lds(a), (hipStream_t)s, (const double *)nullptr,
                        (const float *)nullptr, (const float *)nullptr, a,
                        (const double *)nullptr, (const double *)nullptr, 0.0, 0.0, 0.0, 0.0,
-                       bins_dev, (unsigned long long *)counts_dev, (const fgk_binjob *)nullptr);
+                       (void *)sums_dev, (unsigned long long *)counts_dev,
+                       (const fgk_binjob *)nullptr);
     FGK_LAUNCHED();
     return 0;
 }
 
 extern "C" int fgk_fused_bin_f32(const float *A, const float *B, const fgk_binspec *b,
                                  const double *phase_dev, const double *winv_dev, double norm,
-                                 double Vinv, double Pshot, double npart, double *bins_dev,
+                                 double Vinv, double Pshot, double npart, int64_t *sums_dev,
                                  fgk_stream_t s) {
     if (b->geometry) {
         fgk_set_error("fgk_fused_bin_f32: geometry is computed by fgk_bin_geometry");
         return -2;
     }
-    BinArgs a = make_args(b);
+    BinArgs a = make_args(b, 1);
     const int Ng = b->Ng;
     BIN_LAUNCH(1, bin_bk(b), Ng, Ng / 2, 1u, bin_lds(a), (hipStream_t)s, (const double *)nullptr,
-               A, B, a, phase_dev, winv_dev, norm, Vinv, Pshot, npart, bins_dev,
+               A, B, a, phase_dev, winv_dev, norm, Vinv, Pshot, npart, (void *)sums_dev,
                (unsigned long long *)nullptr, (const fgk_binjob *)nullptr);
     FGK_LAUNCHED();
     const unsigned nplane = (unsigned)(((Ng + PI - 1) / PI) * ((Ng + PJ - 1) / PJ));
     hipLaunchKernelGGL(k_bin_plane, dim3(nplane), dim3(PI * PJ), bin_lds(a), (hipStream_t)s, A, B,
-                       a, phase_dev, winv_dev, norm, Vinv, Pshot, npart, bins_dev,
+                       a, phase_dev, winv_dev, norm, Vinv, Pshot, npart, (void *)sums_dev,
                        (const fgk_binjob *)nullptr);
     FGK_LAUNCHED();
     return 0;
@@ -967,17 +1064,17 @@ extern "C" int fgk_fused_bin_f32_batch(const fgk_binspec *b, const fgk_binjob *j
     }
     fgk_binspec bb = *b;
     bb.NK = NK_max;          // the LDS window size does not depend on NK
-    BinArgs a = make_args(&bb);
+    BinArgs a = make_args(&bb, 1);
     const int Ng = b->Ng;
     BIN_LAUNCH(1, bin_bk(b), Ng, Ng / 2, (unsigned)njobs, bin_lds(a), (hipStream_t)s,
                (const double *)nullptr, (const float *)nullptr, (const float *)nullptr, a,
-               phase_dev, winv_dev, 0.0, 0.0, 0.0, 0.0, (double *)nullptr,
+               phase_dev, winv_dev, 0.0, 0.0, 0.0, 0.0, (void *)nullptr,
                (unsigned long long *)nullptr, jobs_dev);
     FGK_LAUNCHED();
     const unsigned nplane = (unsigned)(((Ng + PI - 1) / PI) * ((Ng + PJ - 1) / PJ));
     hipLaunchKernelGGL(k_bin_plane, dim3(nplane, (unsigned)njobs), dim3(PI * PJ), bin_lds(a),
                        (hipStream_t)s, (const float *)nullptr, (const float *)nullptr, a,
-                       phase_dev, winv_dev, 0.0, 0.0, 0.0, 0.0, (double *)nullptr, jobs_dev);
+                       phase_dev, winv_dev, 0.0, 0.0, 0.0, 0.0, (void *)nullptr, jobs_dev);
     FGK_LAUNCHED();
     return 0;
 }

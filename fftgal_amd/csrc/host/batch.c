@@ -47,9 +47,10 @@ struct fftgal_power {
     float *fA, *fB;
     double *d_phase, *d_winv;
     fgk_binspec spec;          /* per-run sums */
-    double *d_bins;
-    size_t nbins;
-    double *h_bins;            /* pinned */
+    int64_t *d_bins;           /* fixed-point pairs (fgk.h), 2 * nbins */
+    size_t nbins;              /* values */
+    int64_t *h_bins;           /* pinned copy */
+    double *h_sums;            /* the values in FP64 */
     /* geometry */
     double *gK, *gKmu;
     long *gN, *gNmu;
@@ -84,27 +85,29 @@ static int bin_geometry(const fgk_binspec *spec, int NK, int W, fgk_stream_t s, 
     g.geometry = 1;
     g.do_pl = 1;
     size_t nb = fgk_bins_doubles(&g), nc = fgk_bins_counts(&g);
-    double *db = NULL;
+    int64_t *db = NULL;
     int64_t *dc = NULL;
-    TRY(fgk_malloc((void **)&db, nb * sizeof(double)));
+    TRY(fgk_malloc((void **)&db, 2 * nb * sizeof(int64_t)));
     if (fgk_malloc((void **)&dc, nc * sizeof(int64_t))) { fgk_free(db); return -1; }
-    double *hb = (double *)malloc(nb * sizeof(double));
+    int64_t *hb = (int64_t *)malloc(2 * nb * sizeof(int64_t));
     long *hc = (long *)malloc(nc * sizeof(long));
     int rc = (!hb || !hc) ? -1 : 0;
-    if (!rc) rc = fgk_fill_zero(db, nb * sizeof(double), s);
+    if (!rc) rc = fgk_fill_zero(db, 2 * nb * sizeof(int64_t), s);
     if (!rc) rc = fgk_fill_zero(dc, nc * sizeof(int64_t), s);
     if (!rc) rc = fgk_bin_geometry(&g, db, dc, s);
-    if (!rc) rc = fgk_d2h(hb, db, nb * sizeof(double), s);
+    if (!rc) rc = fgk_d2h(hb, db, 2 * nb * sizeof(int64_t), s);
     if (!rc) rc = fgk_d2h(hc, dc, nc * sizeof(int64_t), s);
     if (!rc) rc = fgk_stream_sync(s);
     if (!rc) {
-        const double *Ks = hb + (size_t)NK * (4 + W);
-        const double *Kmu = Ks + NK;
+        /* K sums: fixed point with exponent FGK_BIN_K_EXP */
+        const int64_t *Ks = hb + 2 * (size_t)NK * (4 + W);
+        const int64_t *Kmu = Ks + 2 * (size_t)NK;
         for (int b = 0; b < NK; ++b) {
-            gK[b] = Ks[b];
+            gK[b] = fgk_bin_fixed_value(Ks + 2 * b, FGK_BIN_K_EXP);
             gN[b] = hc[b];
             for (int m = 0; m < W; ++m) {
-                gKmu[(size_t)b * W + m] = Kmu[(size_t)b * W + m];
+                gKmu[(size_t)b * W + m] =
+                    fgk_bin_fixed_value(Kmu + 2 * ((size_t)b * W + m), FGK_BIN_K_EXP);
                 gNmu[(size_t)b * W + m] = hc[NK + (size_t)b * W + m];
             }
         }
@@ -126,6 +129,13 @@ static int bin_geometry(const fgk_binspec *spec, int NK, int W, fgk_stream_t s, 
 
 static int compute_geometry(fftgal_power_t *p) {
     return bin_geometry(&p->spec, p->NK, p->W, p->s, p->gK, p->gN, p->gKmu, p->gNmu);
+}
+
+/* n fixed-point power sums (pairs, fgk.h) of a catalog with shot noise
+ * Pshot, the scale the bin kernel used, to FP64 */
+static void fixed_sums(const int64_t *hl, size_t n, double Pshot, double *out) {
+    const int e = fgk_bin_fixed_exponent(Pshot);
+    for (size_t i = 0; i < n; ++i) out[i] = fgk_bin_fixed_value(hl + 2 * i, e);
 }
 
 /* normalised Pl and Pmu of one catalog from its bin sums hb ([NK][4] then
@@ -175,6 +185,7 @@ int fftgal_power_destroy(fftgal_power_t *p) {
     fgk_free(p->d_winv);
     fgk_free(p->d_bins);
     fgk_host_free(p->h_bins);
+    free(p->h_sums);
     free(p->gK);
     free(p->gKmu);
     free(p->gN);
@@ -258,8 +269,12 @@ int fftgal_power_create(fftgal_power_t **out, const fftgal_power_desc *desc) {
         p->spec.bk = p->tune.bin_bk;
         p->nbins = fgk_bins_doubles(&p->spec);
     }
-    STEP(fgk_malloc((void **)&p->d_bins, p->nbins * sizeof(double)));
-    STEP(fgk_host_alloc((void **)&p->h_bins, p->nbins * sizeof(double)));
+    STEP(fgk_malloc((void **)&p->d_bins, 2 * p->nbins * sizeof(int64_t)));
+    STEP(fgk_host_alloc((void **)&p->h_bins, 2 * p->nbins * sizeof(int64_t)));
+    if (!rc && !(p->h_sums = (double *)malloc(p->nbins * sizeof(double)))) {
+        fgk_set_error("out of host memory");
+        rc = -1;
+    }
     if (!rc) {
         p->gK = (double *)calloc((size_t)p->NK, sizeof(double));
         p->gN = (long *)calloc((size_t)p->NK, sizeof(long));
@@ -356,7 +371,7 @@ static int enqueue_work(fftgal_power_t *p, const float *x, const float *y, const
     int st = 0;
 #define MARK() do { if (ev) TRY(fgk_event_record(ev[st], p->s)); ++st; } while (0)
     MARK();
-    TRY(fgk_fill_zero(p->d_bins, p->nbins * sizeof(double), p->s));
+    TRY(fgk_fill_zero(p->d_bins, 2 * p->nbins * sizeof(int64_t), p->s));
     for (int stage = 0; stage < NPAINT; ++stage) {
         MARK();
         TRY(fgk_paint_stage(p->paint, stage, x, y, z, Np, Hinv, gA, gB, p->s));
@@ -371,7 +386,7 @@ static int enqueue_work(fftgal_power_t *p, const float *x, const float *y, const
     TRY(fgk_fused_bin_f32(p->fA, p->d.interlace ? p->fB : NULL, &p->spec, p->d_phase, p->d_winv,
                           norm, Vinv, Pshot, (double)Np, p->d_bins, p->s));
     MARK();
-    TRY(fgk_d2h(p->h_bins, p->d_bins, p->nbins * sizeof(double), p->s));
+    TRY(fgk_d2h(p->h_bins, p->d_bins, 2 * p->nbins * sizeof(int64_t), p->s));
     MARK();
 #undef MARK
     return 0;
@@ -470,7 +485,10 @@ int fftgal_power_sync(fftgal_power_t *p) { return fgk_stream_sync(p->s); }
 
 int fftgal_power_results(fftgal_power_t *p, double *K, long *N, double *Pl, double *Kmu,
                          long *Nmu_count, double *Pmu) {
-    normalise_bins(p->NK, p->W, p->d.Nmu, p->KF, p->h_bins, p->gK, p->gN, p->gKmu, p->gNmu, K, N,
+    double norm, Vinv, Pshot;
+    catalog_norms(p->d.Ng, p->d.L, p->Np, p->V, &norm, &Vinv, &Pshot);
+    fixed_sums(p->h_bins, p->nbins, Pshot, p->h_sums);
+    normalise_bins(p->NK, p->W, p->d.Nmu, p->KF, p->h_sums, p->gK, p->gN, p->gKmu, p->gNmu, K, N,
                    Pl, Kmu, Nmu_count, Pmu);
     return 0;
 }
@@ -726,8 +744,9 @@ struct fftgal_batch {
     batch_group *grp;
     fgk_stream_t s;
     fgk_paintbatch *paint;
-    double *d_bins, *h_bins;
-    size_t nbins;
+    int64_t *d_bins, *h_bins;    /* fixed-point pairs (fgk.h), device and pinned */
+    double *h_sums;              /* the values in FP64 */
+    size_t nbins;                /* values */
     int tables_ok;
     int use_graph;
     fgk_graph_t graph[2];        /* one per grid set */
@@ -765,6 +784,7 @@ int fftgal_batch_destroy(fftgal_batch_t *b) {
     fgk_free(b->gset[1]);
     fgk_free(b->d_bins);
     fgk_host_free(b->h_bins);
+    free(b->h_sums);
     if (b->s) fgk_stream_destroy(b->s);
     free(b);
     return 0;
@@ -823,7 +843,7 @@ static int group_setup(fftgal_batch_t *b, batch_group *G, const char *wisdom) {
         fgk_binjob *h = &G->hjob[k];
         h->A = G->f + (size_t)k * NG * cells;
         h->B = G->interlace ? h->A + cells : NULL;
-        h->bins = b->d_bins + J->boff;
+        h->sums = b->d_bins + 2 * J->boff;
         h->NK = J->NK;
         h->dKinv = J->KF / J->d.dK;
         if (J->NK > G->NK_max) G->NK_max = J->NK;
@@ -901,8 +921,12 @@ int fftgal_batch_create(fftgal_batch_t **out, int njobs, const fftgal_job_desc *
         if (!G->jobs) { fgk_set_error("out of host memory"); rc = -1; }
     }
     for (int j = 0; j < njobs && !rc; ++j) b->grp[b->job[j].group].jobs[b->job[j].slot] = j;
-    STEP(fgk_malloc((void **)&b->d_bins, b->nbins * sizeof(double)));
-    STEP(fgk_host_alloc((void **)&b->h_bins, b->nbins * sizeof(double)));
+    STEP(fgk_malloc((void **)&b->d_bins, 2 * b->nbins * sizeof(int64_t)));
+    STEP(fgk_host_alloc((void **)&b->h_bins, 2 * b->nbins * sizeof(int64_t)));
+    if (!rc && !(b->h_sums = (double *)malloc(b->nbins * sizeof(double)))) {
+        fgk_set_error("out of host memory");
+        rc = -1;
+    }
     /* one block per grid set for the int64 grids of all groups (each group's
      * part 256-byte aligned), so that one region clears a whole set */
     for (int q = 0; q < b->ngroups; ++q) {
@@ -1020,7 +1044,7 @@ static int batch_work(fftgal_batch_t *b, fgk_event_t *ev) {
                                    b->gwords * sizeof(int64_t)));
 #define MARK() do { if (ev) TRY(fgk_event_record(ev[st], b->s)); ++st; } while (0)
     MARK();
-    TRY(fgk_fill_zero(b->d_bins, b->nbins * sizeof(double), b->s));
+    TRY(fgk_fill_zero(b->d_bins, 2 * b->nbins * sizeof(int64_t), b->s));
     for (int stage = 0; stage < NPAINT; ++stage) {
         MARK();
         TRY(fgk_paint_batch_stage(b->paint, stage, b->s));
@@ -1045,7 +1069,7 @@ static int batch_work(fftgal_batch_t *b, fgk_event_t *ev) {
                                     b->s));
     }
     MARK();
-    TRY(fgk_d2h(b->h_bins, b->d_bins, b->nbins * sizeof(double), b->s));
+    TRY(fgk_d2h(b->h_bins, b->d_bins, 2 * b->nbins * sizeof(int64_t), b->s));
     MARK();
 #undef MARK
     return 0;
@@ -1103,7 +1127,14 @@ int fftgal_batch_results(fftgal_batch_t *b, int job, double *K, long *N, double 
         return -2;
     }
     const batch_job *J = &b->job[job];
-    normalise_bins(J->NK, b->W, b->Nmu, J->KF, b->h_bins + J->boff, J->gK, J->gN, J->gKmu,
+    double norm, Vinv, Pshot;
+    catalog_norms(J->d.Ng, J->d.L, J->Np, J->V, &norm, &Vinv, &Pshot);
+    fgk_binspec bs = {0};
+    bs.NK = J->NK;
+    bs.Nmu = b->Nmu;
+    bs.do_pl = 1;
+    fixed_sums(b->h_bins + 2 * J->boff, fgk_bins_doubles(&bs), Pshot, b->h_sums + J->boff);
+    normalise_bins(J->NK, b->W, b->Nmu, J->KF, b->h_sums + J->boff, J->gK, J->gN, J->gKmu,
                    J->gNmu, K, N, Pl, Kmu, Nmu_count, Pmu);
     return 0;
 }
