@@ -815,25 +815,28 @@ __device__ __forceinline__ void rot4(const float w[4], int r, float o[4]) {
     o[3] = r1 ? t1 : t3;
 }
 
-// Fixed point of the tile paint.  A tile whose work items hold at most n
-// particles (n = min(cap, particles in the tile)) paints in units of
-// 1 / (216 C_S) particle, C_S = 2427 * 2^(S - 19) (2427 = floor(2^19 / 216)),
-// with the largest S <= 25 for which n * (64 C_S + 1) < 2^31: a footprint
-// cell gets at most one term of at most 64 C_S + 1 units per particle, so
-// the int32 LDS sums cannot overflow.  Dense tiles (8192 particles) get
-// S = 19, tiles of at most 216 particles S = 25.  The int64 grids hold units
-// of 1 / (216 C_25) particle; a tile's footprint is shifted left by 25 - S
-// when it is flushed, exactly.  S depends only on the tile's particle count,
-// so the grids stay bit-identical run to run.
-constexpr int FIX_S0 = 19, FIX_SMAX = 25;
+// Fixed point of the tile paint.  An LDS footprint that takes terms from at
+// most n particles paints in units of 1 / (216 C_S) particle,
+// C_S = 2427 * 2^(S - 19) (2427 = floor(2^19 / 216)), with the largest
+// S <= 25 for which n * (64 C_S + 1) < 2^31: a footprint cell gets at most
+// one term of at most 64 C_S + 1 units per particle, so the int32 LDS sums
+// cannot overflow.  n is at most 2 cap = 16384 (a pencil segment's two
+// neighbouring tiles): S = 18 there, 19 for a full dense item of 8192
+// particles, 25 for at most 216 particles.  The int64 grids hold units of
+// 1 / (216 C_25) particle; a footprint is shifted left by 25 - S when it is
+// flushed, exactly.  S depends only on particle counts, so the grids stay
+// bit-identical run to run.
+constexpr int FIX_S0 = 19, FIX_SMIN = 13, FIX_SMAX = 25;
 constexpr float FIX_C0 = 2427.0f;
+constexpr long FIX_TERM0 = 64l * 2427l;   // 64 C_19 = 2^6 * 2427: 64 C_S is exact for S >= 13
 __host__ __device__ constexpr long fix_limit(int S) {
-    return (2147483647l - 1) / (64l * (2427l << (S - FIX_S0)) + 1);
+    return (2147483647l - 1) /
+           ((S >= FIX_S0 ? FIX_TERM0 << (S - FIX_S0) : FIX_TERM0 >> (FIX_S0 - S)) + 1);
 }
 
-__device__ __forceinline__ int tile_bits(long n) {
+__host__ __device__ __forceinline__ int tile_bits(long n) {
     int S = FIX_SMAX;
-    while (S > FIX_S0 && n > fix_limit(S)) --S;
+    while (S > FIX_SMIN && n > fix_limit(S)) --S;
     return S;
 }
 
@@ -1056,9 +1059,7 @@ __device__ __forceinline__ void paint_range(const uint64_t *__restrict__ sorted,
     }
 }
 
-__device__ __forceinline__ float fix_unit(int S) {
-    return FIX_C0 * (float)(1 << (S - FIX_S0));
-}
+__device__ __forceinline__ float fix_unit(int S) { return ldexpf(FIX_C0, S - FIX_S0); }
 
 // Paint of the sorted particles into the int64 grids.  The plain footprint
 // of a tile of side T has T + 4 cells per side, the dual one T + 3, both
@@ -1368,11 +1369,7 @@ extern "C" void fgk_paint_plan_destroy(fgk_paintplan *pp) {
 
 extern "C" double fgk_paint_fixed_scale(void) { return fixed_scale(); }
 
-extern "C" int fgk_paint_tile_bits(long n) {
-    int S = FIX_SMAX;
-    while (S > FIX_S0 && n > fix_limit(S)) --S;
-    return S;
-}
+extern "C" int fgk_paint_tile_bits(long n) { return tile_bits(n); }
 extern "C" int fgk_paint_plan_tile(const fgk_paintplan *pp) { return pp->T; }
 
 static const char *const kPaintStages[] = {"sort1_count", "sort1_scan", "sort1_scatter",

@@ -18,6 +18,9 @@ lognormal(Np, L, seed, Ngs=256, ...)
       5. wrapped into [0, L), returned in a random order, float32 by default.
     Deterministic for a given seed (numpy PCG64 and pocketfft).
 
+lognormal_device(Np, L, seed, ...)
+    The same recipe drawn on the GPU with torch (catalogs of 2^30).
+
 batch_specs(nmocks)
     Parameters of the batched uneven workload: per seed s, L ~ U[250, 2000],
     Ng from {128, 192, 256, 384}, Np log-uniform in [5e5, 2e7].
@@ -104,3 +107,57 @@ def batch_specs(nmocks):
         Np = int(round(np.exp(r.uniform(np.log(5e5), np.log(2e7)))))
         out.append((s, L, Ng, Np))
     return out
+
+
+def lognormal_device(Np, L, seed, Ngs=256, A=2e4, k0=0.1, n=-1.5, f=0.76, rsd=True,
+                     device="cuda", chunk=1 << 26):
+    """The lognormal recipe of lognormal() drawn on the GPU with torch (for
+    catalogs of 2^30 particles, which numpy draws in minutes): the same
+    field, density and displacement, each particle's cell drawn from the
+    density (inverse CDF, i.e. the multinomial of lognormal() particle by
+    particle) and its position uniform in the cell.  Deterministic for a
+    given seed and device, but not the same catalog as lognormal() with
+    that seed.  Returns float32 x, y, z tensors on the device, in random
+    order, and sigma^2 of the field."""
+    import torch
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    H = L / Ngs
+    w = torch.randn((Ngs, Ngs, Ngs), generator=gen, device=device, dtype=torch.float64)
+    W = torch.fft.rfftn(w)
+    del w
+    kf = 2 * np.pi / L
+    ki = torch.fft.fftfreq(Ngs, d=1.0 / Ngs, device=device, dtype=torch.float64) * kf
+    kk = torch.fft.rfftfreq(Ngs, d=1.0 / Ngs, device=device, dtype=torch.float64) * kf
+    k2 = ki[:, None, None] ** 2 + ki[None, :, None] ** 2 + kk[None, None, :] ** 2
+    k2[0, 0, 0] = 1.0
+    amp = torch.sqrt(A * (torch.sqrt(k2) / k0) ** n / H ** 3)
+    amp[0, 0, 0] = 0.0
+    W *= amp
+    del amp
+    delta = torch.fft.irfftn(W, s=(Ngs, Ngs, Ngs))
+    sigma2 = float(delta.var())
+    cdf = torch.cumsum(torch.exp(delta - 0.5 * sigma2).reshape(-1), 0)
+    del delta
+    psi = None
+    if rsd and f != 0:
+        psi = torch.fft.irfftn(1j * kk[None, None, :] / k2 * W, s=(Ngs, Ngs, Ngs)).reshape(-1)
+    del W, k2
+    out = [torch.empty(Np, dtype=torch.float32, device=device) for _ in range(3)]
+    for s0 in range(0, Np, chunk):
+        m = min(chunk, Np - s0)
+        u = torch.rand(m, generator=gen, device=device, dtype=torch.float64) * cdf[-1]
+        cell = torch.searchsorted(cdf, u).clamp_(max=Ngs ** 3 - 1)
+        del u
+        ci = torch.div(cell, Ngs * Ngs, rounding_mode="floor")
+        cj = torch.div(cell, Ngs, rounding_mode="floor") % Ngs
+        ck = cell % Ngs
+        for a, c in zip(out, (ci, cj, ck)):
+            v = (c + torch.rand(m, generator=gen, device=device, dtype=torch.float64)) * H
+            if a is out[2] and psi is not None:
+                v += f * psi[cell]
+            v = torch.remainder(v, L).to(torch.float32)
+            v[v >= L] -= L          # float32 rounding can land exactly on L
+            a[s0:s0 + m] = v
+        del cell, ci, cj, ck
+    return tuple(out), sigma2

@@ -32,9 +32,12 @@ def _paint_vs_oracle(torch, orc, x, y, z, L, Ng, tile):
     tx, ty, tz = _dev(torch, x, y, z)
     p = fa.PowerPlan(Ng, L, interlace=True, Np_max=len(x), paint_tile=tile)
     A, B = p.paint_only(tx.data_ptr(), ty.data_ptr(), tz.data_ptr(), len(x))
-    # each tile rounds to 2^-S particle or finer, S >= 19 (8192-particle items)
-    S = p.fixed_bits(8192)
-    assert S == 19 and p.fixed_bits(1) == 25 and p.fixed_bits(216) == 25
+    # each footprint rounds to 2^-S particle or finer: S >= 18 (a pencil
+    # segment's two neighbouring tiles of up to 8192 particles each), 19 for a
+    # full 8192-particle dense item
+    S = p.fixed_bits(16384)
+    assert S == 18 and p.fixed_bits(8192) == 19
+    assert p.fixed_bits(1) == 25 and p.fixed_bits(216) == 25
     unit_fx = p.fixed_scale()
     assert abs(unit_fx * 2.0 ** 25 - 1) <= 1.1e-4
     a = A.astype(np.float64) * unit_fx
