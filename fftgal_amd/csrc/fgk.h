@@ -203,8 +203,8 @@ size_t fgk_bins_counts(const fgk_binspec *b);
  * (H, L) at 2i, 2i+1, worth (H 2^32 + L) 2^(e - FGK_BIN_FIX_BITS), e the
  * binary exponent (ilogb) of the catalog's Pshot for power sums,
  * FGK_BIN_K_EXP for the geometry's K sums (fgk_bin_fixed_value) */
-#define FGK_BIN_FIX_BITS 30
-#define FGK_BIN_K_EXP (-22)
+#define FGK_BIN_FIX_BITS 24
+#define FGK_BIN_K_EXP (-28)
 double fgk_bin_fixed_value(const int64_t *hl, int e);
 /* e of a catalog's power sums: ilogb(Pshot), 0 unless 0 < Pshot < 1e300 */
 int fgk_bin_fixed_exponent(double Pshot);

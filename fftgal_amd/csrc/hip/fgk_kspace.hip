@@ -311,13 +311,15 @@ __device__ __forceinline__ int kval(int i, int Ng) { return (2 * i < Ng) ? i : i
 // H = floor(Y / 2^32) and L = Y - H 2^32 in [0, 2^32), so every sum is an
 // exact integer whatever the order of the atomics and reruns give
 // bit-identical bins; the host forms (H_sum 2^32 + L_sum) / qs.  The
-// quantum is 2^-30 of Pshot's power of two for power sums (the rounding of
-// one mode is below 1e-9 Pshot) and 2^-52 KF for K sums.
+// quantum is 2^-24 of Pshot's power of two for power sums (the rounding of
+// one value, the sum of at most 4 modes, is below 3e-8 Pshot) and 2^-52 KF
+// for K sums.
 //   fx = 1 (fast path): the LDS window holds one int64 per value, Y from one
 //     FMA against a magic constant while |Y| < 2^50 (a block adds at most
 //     1024 such values to one entry, so the entry stays below 2^60), split
-//     into (H, L) when the window is flushed; larger values (|v| above about
-//     2^20 Pshot) go to the global pair directly.
+//     into (H, L) when the window is flushed; larger values, |v| above
+//     2^26 Pshot (lognormal mocks with 1 Mpc/h seed cells reach 1e7 Pshot in
+//     small boxes), go to the global pair directly.
 //   fx = 2 (geometry): the LDS window holds pairs, every value split.
 __host__ __device__ __forceinline__ int fix_exponent(double Pshot) {
     return (Pshot > 0.0 && Pshot < 1e300) ? ilogb(Pshot) : 0;
