@@ -312,10 +312,16 @@ def main_batch(args):
                                    L_ ** 3, wisdom, Nmu=nmu, reps=2)
         t_tune = time.perf_counter() - t0
         print(f"# autotune: {tuned} in {t_tune:.1f}s", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
     t0 = time.perf_counter()
     b = fa.BatchPlan(jobs, los=(0, 0, 1), Nmu=nmu, wisdom=wisdom)
     t_plan = time.perf_counter() - t0
-    print(f"# batch plan: {len(jobs)} catalogs in {t_plan:.1f}s", file=sys.stderr, flush=True)
+    batch_bytes = free0 - torch.cuda.mem_get_info()[0]   # device memory the batch took
+    rounds, pool = b.rounds(), b.pool_bytes()
+    print(f"# batch plan: {len(jobs)} catalogs in {t_plan:.1f}s, {rounds} rounds, pool "
+          f"{pool / 2 ** 30:.1f} GiB, device memory taken {batch_bytes / 1e9:.1f} GB",
+          file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         b.enqueue(cats)
     b.sync()
@@ -402,7 +408,8 @@ def main_batch(args):
                                f"l=0,2,4(,6) and {nmu} |mu|-wedges, dK=kF of each grid box",
                    "grids": {str(n): sum(1 for j in jobs if j["Ng"] == n)
                              for n in (128, 192, 256, 384)},
-                   "grid_sets": sets, "parallelism": "1 GPU"},
+                   "grid_sets": sets, "rounds": rounds, "pool_bytes": pool,
+                   "device_bytes": batch_bytes, "parallelism": "1 GPU"},
         "stages_ms": {k: round(v, 3) for k, v in prof.items()},
         "plain_fp64": {"value": round(plain_value, 2), "unit": "Mparticles/s",
                        "ms_total": round(t_plain * 1e3, 1),

@@ -140,6 +140,8 @@ API = {
     "fftgal_batch_use_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "fftgal_batch_tile": (C.c_int, [C.c_void_p, C.c_int]),
     "fftgal_batch_grid_sets": (C.c_int, [C.c_void_p]),
+    "fftgal_batch_rounds": (C.c_int, [C.c_void_p]),
+    "fftgal_batch_pool_bytes": (C.c_size_t, [C.c_void_p]),
     "fftgal_batch_stream": (C.c_void_p, [C.c_void_p]),
 }
 
@@ -577,6 +579,13 @@ class BatchPlan:
 
     def grid_sets(self):
         return self.lib.fftgal_batch_grid_sets(self.h)
+
+    def rounds(self):
+        """rounds the catalogs run in (the pool bound, FFTGAL_BATCH_BYTES)"""
+        return self.lib.fftgal_batch_rounds(self.h)
+
+    def pool_bytes(self):
+        return self.lib.fftgal_batch_pool_bytes(self.h)
 
     def stream(self):
         return self.lib.fftgal_batch_stream(self.h)

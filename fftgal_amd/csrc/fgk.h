@@ -145,6 +145,12 @@ typedef struct fgk_paintjob_desc {
 } fgk_paintjob_desc;
 typedef struct fgk_paintbatch fgk_paintbatch;
 int  fgk_paint_batch_create(fgk_paintbatch **pb, int njobs, const fgk_paintjob_desc *jobs);
+/* the same with the sort words (the arrays proportional to the particles)
+ * carved from a caller's device workspace of at least
+ * fgk_paint_batch_ws_bytes(sum of Np_max) bytes, which it keeps */
+int  fgk_paint_batch_create_ws(fgk_paintbatch **pb, int njobs, const fgk_paintjob_desc *jobs,
+                               void *ws, size_t ws_bytes);
+size_t fgk_paint_batch_ws_bytes(long Np_tot);
 void fgk_paint_batch_destroy(fgk_paintbatch *pb);
 /* positions of every catalog (device float32) and Np[j] <= Np_max; when
  * they differ from the last call, waits for s and uploads the job table */

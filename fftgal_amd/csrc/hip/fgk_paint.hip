@@ -1663,6 +1663,7 @@ struct fgk_paintbatch {
     int set;                       // set the paint stage writes
     uint64_t *zero;                // cleared by the paint stage's launches
     long zero_words;
+    int words_external;            // sorted1, sorted2, aux1, ext2 in a caller's workspace
 };
 
 extern "C" void fgk_paint_batch_destroy(fgk_paintbatch *pb) {
@@ -1671,10 +1672,12 @@ extern "C" void fgk_paint_batch_destroy(fgk_paintbatch *pb) {
                         pb->base1, pb->chunk0, pb->blk2, pb->H2,       pb->O2,   pb->S2,
                         pb->B2,   pb->T2,    pb->tilebase, pb->items};
     for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; ++i) fgk_free(bufs[i]);
-    fgk_free(pb->sorted1);
-    fgk_free(pb->sorted2);
-    fgk_free(pb->aux1);
-    fgk_free(pb->ext2);
+    if (!pb->words_external) {
+        fgk_free(pb->sorted1);
+        fgk_free(pb->sorted2);
+        fgk_free(pb->aux1);
+        fgk_free(pb->ext2);
+    }
     fgk_free(pb->djobs);
     fgk_free(pb->dc1);
     fgk_free(pb->di0);
@@ -1686,8 +1689,32 @@ extern "C" void fgk_paint_batch_destroy(fgk_paintbatch *pb) {
     free(pb);
 }
 
+// the sort words of Np_tot particles (two main words, the side word and the
+// extension), each array 256-byte aligned, as carved from a workspace
+static size_t words_layout(long Np_tot, size_t off[4]) {
+    const size_t sz[4] = {(size_t)Np_tot * 8, (size_t)Np_tot * 8, (size_t)Np_tot * 2,
+                          (size_t)Np_tot};
+    size_t o = 0;
+    for (int i = 0; i < 4; ++i) {
+        off[i] = o;
+        o += (sz[i] + 255) & ~(size_t)255;
+    }
+    return o;
+}
+
+extern "C" size_t fgk_paint_batch_ws_bytes(long Np_tot) {
+    size_t off[4];
+    return words_layout(Np_tot, off);
+}
+
 extern "C" int fgk_paint_batch_create(fgk_paintbatch **out, int njobs,
                                       const fgk_paintjob_desc *jobs) {
+    return fgk_paint_batch_create_ws(out, njobs, jobs, NULL, 0);
+}
+
+extern "C" int fgk_paint_batch_create_ws(fgk_paintbatch **out, int njobs,
+                                         const fgk_paintjob_desc *jobs, void *ws,
+                                         size_t ws_bytes) {
     *out = NULL;
     if (njobs < 1 || njobs > 65535) {
         fgk_set_error("paint batch: need 1..65535 catalogs");
@@ -1806,10 +1833,25 @@ extern "C" int fgk_paint_batch_create(fgk_paintbatch **out, int njobs,
     rc |= fgk_malloc((void **)&pb->T2, (size_t)nseg2 * MAX_D2 * u);
     rc |= fgk_malloc((void **)&pb->tilebase, (size_t)njobs * BATCH_SL * u);
     rc |= fgk_malloc((void **)&pb->items, (size_t)njobs * BATCH_SL * u);
-    rc |= fgk_malloc((void **)&pb->sorted1, (size_t)Np_tot * sizeof(uint64_t));
-    rc |= fgk_malloc((void **)&pb->sorted2, (size_t)Np_tot * sizeof(uint64_t));
-    rc |= fgk_malloc((void **)&pb->aux1, (size_t)Np_tot * sizeof(uint16_t));
-    rc |= fgk_malloc((void **)&pb->ext2, (size_t)Np_tot * sizeof(uint8_t));
+    if (ws) {
+        size_t off[4];
+        if (words_layout(Np_tot, off) > ws_bytes) {
+            fgk_set_error("paint batch: workspace of %zu bytes below the %zu the sort needs",
+                          ws_bytes, words_layout(Np_tot, off));
+            fgk_paint_batch_destroy(pb);
+            return -2;
+        }
+        pb->words_external = 1;
+        pb->sorted1 = (uint64_t *)((char *)ws + off[0]);
+        pb->sorted2 = (uint64_t *)((char *)ws + off[1]);
+        pb->aux1 = (uint16_t *)((char *)ws + off[2]);
+        pb->ext2 = (uint8_t *)((char *)ws + off[3]);
+    } else {
+        rc |= fgk_malloc((void **)&pb->sorted1, (size_t)Np_tot * sizeof(uint64_t));
+        rc |= fgk_malloc((void **)&pb->sorted2, (size_t)Np_tot * sizeof(uint64_t));
+        rc |= fgk_malloc((void **)&pb->aux1, (size_t)Np_tot * sizeof(uint16_t));
+        rc |= fgk_malloc((void **)&pb->ext2, (size_t)Np_tot * sizeof(uint8_t));
+    }
     uint32_t *hb = (uint32_t *)malloc((size_t)(njobs + 1) * u);
     if (!rc && !hb) { fgk_set_error("out of host memory"); rc = -1; }
     if (!rc) {

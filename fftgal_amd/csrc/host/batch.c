@@ -699,36 +699,62 @@ int fftgal_memcpy_d2h(void *dst, const void *src, unsigned long bytes) {
 
 /* ------------------------------------------------------------------------
  * Batches of catalogs (fftgal_batch_*): many catalogs, each on its own grid
- * (side, box = catalog box / fold, interlacing, k binning), through one
- * launch per stage:
- *   sort and paint  -- all catalogs together (fgk_paintbatch; one paint
+ * (side, box = catalog box / fold, interlacing, k binning).
+ *
+ * Device memory is a bounded pool.  The catalogs are cut into rounds, in
+ * grid-side order, each round as large as the pool allows; the rounds run
+ * one after the other through the same memory:
+ *   pool = [grid set 0 | grid set 1 | spectra and sort words of a round]
+ * A grid set holds the int64 grids of one round (the largest round's size);
+ * the rounds alternate between the two sets, and each round's paint clears
+ * the part of the other set that the previous round's row FFT has read.  With one set
+ * (FFTGAL_GRID_SETS=1) the row FFT clears the grids it reads instead.  Both
+ * keep the invariant that a round finds its grid set zero.
+ * Within a round, one launch per stage:
+ *   sort and paint  -- all its catalogs together (fgk_paintbatch; one paint
  *                      launch per tile side),
  *   FFT passes      -- per grid side: the grids of every catalog of that
  *                      side and interlacing lie one after the other, so the
  *                      row pass and both column passes cover them all,
- *   fused bin       -- per grid side, one launch pair with a catalog table,
- *   bins            -- one zero fill and one copy to pinned host memory.
- * The whole sequence is one hipGraph; catalog pointers, Np and V live in
- * device tables, so a new set of catalogs updates the tables and replays
- * the same graph.
+ *   fused bin       -- per grid side, one launch pair with a catalog table.
+ * Then one copy of every catalog's bin sums to pinned host memory.  The
+ * whole sequence is one hipGraph; catalog pointers, Np and V live in device
+ * tables, so a new set of catalogs updates the tables and replays the same
+ * graph.  The pool's bound is FFTGAL_BATCH_BYTES (bytes, default 34 GiB).
  * ---------------------------------------------------------------------- */
 
-typedef struct batch_group {
-    int Ng, interlace, n;        /* grids of side Ng; n catalogs */
-    int *jobs;                   /* caller's catalog indices, in grid order */
+#define BATCH_POOL_DEFAULT ((size_t)34 << 30)
+
+typedef struct batch_side {      /* per grid side: FFT plan and k-space tables */
+    int Ng;
     fgk_fftplan *plan;
-    int64_t *g, *g2;             /* n * G int64 grids of set 0 and set 1 (NULL: one set) */
-    size_t goff;                 /* offset of g in the batch's grid sets, int64 words */
-    float *f;                    /* n * G packed spectra */
     double *d_phase, *d_winv;
+    fftgal_tuning tune;
+} batch_side;
+
+typedef struct batch_group {     /* the grids of one side and interlacing in a round */
+    int Ng, interlace, n;        /* n catalogs */
+    int side;                    /* its batch_side */
+    int *jobs;                   /* batch catalog indices, in grid order */
+    size_t goff;                 /* offset of its grids in a grid set, int64 words */
+    int64_t *g, *g2;             /* n * G int64 grids in set 0 and set 1 (NULL: one set) */
+    float *f;                    /* n * G packed spectra */
     fgk_binspec spec;            /* Ng, Nmu, los, do_pl (NK per catalog) */
     int NK_max;
     fgk_binjob *hjob, *djob;
 } batch_group;
 
+typedef struct batch_round {
+    int njobs, ngroups;
+    int *jobs;                   /* batch catalog indices, in paint order */
+    batch_group *grp;
+    fgk_paintbatch *paint;
+    size_t gwords;               /* int64 words of its grids */
+} batch_round;
+
 typedef struct batch_job {
     fftgal_job_desc d;
-    int NK, group, slot;
+    int NK, round, group, slot, local;   /* local: index within its round */
     double KF;
     size_t boff;                 /* offset of its bin sums in d_bins / h_bins */
     double *gK, *gKmu;
@@ -738,22 +764,25 @@ typedef struct batch_job {
 } batch_job;
 
 struct fftgal_batch {
-    int njobs, ngroups, Nmu, W;
+    int njobs, nrounds, nsides, Nmu, W;
     double los[3];
     batch_job *job;
-    batch_group *grp;
+    batch_round *rnd;
+    batch_side *side;
     fgk_stream_t s;
-    fgk_paintbatch *paint;
     int64_t *d_bins, *h_bins;    /* fixed-point pairs (fgk.h), device and pinned */
     double *h_sums;              /* the values in FP64 */
     size_t nbins;                /* values */
+    char *pool;
+    size_t pool_bytes;
+    int64_t *gset[2];            /* grid sets in the pool (gset[1] NULL: one set) */
+    size_t gwords;               /* int64 words of a grid set */
     int tables_ok;
     int use_graph;
-    fgk_graph_t graph[2];        /* one per grid set */
-    int64_t *gset[2];            /* the int64 grids of every group, set 0 and set 1 */
-    size_t gwords;               /* int64 words of one set */
-    int parity;                  /* set of the next enqueue */
-    int ran;                     /* an enqueue completed setup */
+    fgk_graph_t graph[2];        /* one per grid set of the first round */
+    int parity;                  /* set of the next enqueue's first round */
+    const float **px, **py, **pz;   /* per-round argument arrays of fgk_paint_batch_set */
+    long *pNp;
 };
 
 int fftgal_batch_destroy(fftgal_batch_t *b) {
@@ -761,16 +790,21 @@ int fftgal_batch_destroy(fftgal_batch_t *b) {
     if (b->s) fgk_stream_sync(b->s);
     fgk_graph_destroy(b->graph[0]);
     fgk_graph_destroy(b->graph[1]);
-    fgk_paint_batch_destroy(b->paint);
-    for (int q = 0; q < b->ngroups; ++q) {
-        batch_group *G = &b->grp[q];
-        fgk_fft_plan_destroy(G->plan);
-        fgk_free(G->f);
-        fgk_free(G->d_phase);
-        fgk_free(G->d_winv);
-        fgk_free(G->djob);
-        free(G->hjob);
-        free(G->jobs);
+    for (int r = 0; r < b->nrounds && b->rnd; ++r) {
+        batch_round *R = &b->rnd[r];
+        fgk_paint_batch_destroy(R->paint);
+        for (int q = 0; q < R->ngroups && R->grp; ++q) {
+            fgk_free(R->grp[q].djob);
+            free(R->grp[q].hjob);
+            free(R->grp[q].jobs);
+        }
+        free(R->grp);
+        free(R->jobs);
+    }
+    for (int i = 0; i < b->nsides && b->side; ++i) {
+        fgk_fft_plan_destroy(b->side[i].plan);
+        fgk_free(b->side[i].d_phase);
+        fgk_free(b->side[i].d_winv);
     }
     for (int j = 0; j < b->njobs && b->job; ++j) {
         free(b->job[j].gK);
@@ -778,10 +812,14 @@ int fftgal_batch_destroy(fftgal_batch_t *b) {
         free(b->job[j].gN);
         free(b->job[j].gNmu);
     }
-    free(b->grp);
+    free(b->rnd);
+    free(b->side);
     free(b->job);
-    fgk_free(b->gset[0]);
-    fgk_free(b->gset[1]);
+    free(b->px);
+    free(b->py);
+    free(b->pz);
+    free(b->pNp);
+    fgk_free(b->pool);
     fgk_free(b->d_bins);
     fgk_host_free(b->h_bins);
     free(b->h_sums);
@@ -790,65 +828,183 @@ int fftgal_batch_destroy(fftgal_batch_t *b) {
     return 0;
 }
 
-static int group_of(fftgal_batch_t *b, int Ng, int interlace) {
-    for (int q = 0; q < b->ngroups; ++q)
-        if (b->grp[q].Ng == Ng && b->grp[q].interlace == interlace) return q;
-    batch_group *G = &b->grp[b->ngroups];
-    memset(G, 0, sizeof *G);
-    G->Ng = Ng;
-    G->interlace = interlace;
-    return b->ngroups++;
+static int side_of(fftgal_batch_t *b, int Ng) {
+    for (int i = 0; i < b->nsides; ++i)
+        if (b->side[i].Ng == Ng) return i;
+    b->side[b->nsides].Ng = Ng;
+    return b->nsides++;
 }
 
-static int group_setup(fftgal_batch_t *b, batch_group *G, const char *wisdom) {
-    const int Ng = G->Ng, NG = G->interlace ? 2 : 1;
-    const size_t cells = (size_t)Ng * Ng * Ng;
-    fftgal_tuning t;
-    fftgal_tune_lookup(wisdom, Ng, G->interlace, &t);
-    TRY(fgk_fft_plan_create(&G->plan, Ng, 4));
-    TRY(fgk_fft_set_tuning(G->plan, t.fft_cols, t.fft_rows));
-    if (t.fft_generic) TRY(fgk_fft_force_generic(G->plan, 1));
-    G->g = b->gset[0] + G->goff;
-    TRY(fgk_malloc((void **)&G->f, (size_t)G->n * NG * cells * sizeof(float)));
-    TRY(fgk_malloc((void **)&G->d_phase, (size_t)3 * Ng * 2 * sizeof(double)));
-    TRY(fgk_malloc((void **)&G->d_winv, (size_t)Ng * sizeof(double)));
-    TRY(fgk_malloc((void **)&G->djob, (size_t)G->n * sizeof(fgk_binjob)));
-    G->hjob = (fgk_binjob *)calloc((size_t)G->n, sizeof(fgk_binjob));
+static int side_setup(fftgal_batch_t *b, batch_side *S, int interlace, const char *wisdom) {
+    const int Ng = S->Ng;
+    fftgal_tune_lookup(wisdom, Ng, interlace, &S->tune);
+    TRY(fgk_fft_plan_create(&S->plan, Ng, 4));
+    TRY(fgk_fft_set_tuning(S->plan, S->tune.fft_cols, S->tune.fft_rows));
+    if (S->tune.fft_generic) TRY(fgk_fft_force_generic(S->plan, 1));
+    TRY(fgk_malloc((void **)&S->d_phase, (size_t)3 * Ng * 2 * sizeof(double)));
+    TRY(fgk_malloc((void **)&S->d_winv, (size_t)Ng * sizeof(double)));
     double *tab = (double *)malloc((size_t)3 * Ng * 2 * sizeof(double));
     double *winv = (double *)malloc((size_t)Ng * sizeof(double));
-    int rc = (!G->hjob || !tab || !winv) ? -1 : 0;
+    int rc = (!tab || !winv) ? -1 : 0;
     if (rc) fgk_set_error("out of host memory");
     if (!rc) {
         /* dual grid offset 1/2 per axis (src/fft.c:260), phase on */
         double off[3] = {remainder(0.5, Ng), remainder(0.5, Ng), remainder(0.5, Ng)};
         fftgal_phase_table(Ng, off, 1, tab);
         fftgal_winv_table(Ng, winv);
-        rc = fgk_h2d(G->d_phase, tab, (size_t)3 * Ng * 2 * sizeof(double), b->s);
-        if (!rc) rc = fgk_h2d(G->d_winv, winv, (size_t)Ng * sizeof(double), b->s);
+        rc = fgk_h2d(S->d_phase, tab, (size_t)3 * Ng * 2 * sizeof(double), b->s);
+        if (!rc) rc = fgk_h2d(S->d_winv, winv, (size_t)Ng * sizeof(double), b->s);
         if (!rc) rc = fgk_stream_sync(b->s);
     }
     free(tab);
     free(winv);
-    if (rc) return rc;
-    G->spec.Ng = Ng;
-    G->spec.Nmu = b->Nmu;
-    G->spec.los[0] = b->los[0];
-    G->spec.los[1] = b->los[1];
-    G->spec.los[2] = b->los[2];
-    G->spec.do_pl = 1;
-    G->spec.geometry = 0;
-    G->spec.bk = t.bin_bk;
-    for (int k = 0; k < G->n; ++k) {
-        batch_job *J = &b->job[G->jobs[k]];
-        fgk_binjob *h = &G->hjob[k];
-        h->A = G->f + (size_t)k * NG * cells;
-        h->B = G->interlace ? h->A + cells : NULL;
-        h->sums = b->d_bins + 2 * J->boff;
-        h->NK = J->NK;
-        h->dKinv = J->KF / J->d.dK;
-        if (J->NK > G->NK_max) G->NK_max = J->NK;
+    return rc;
+}
+
+/* device bytes of catalog j's grids (one set), spectra and sort words */
+static void job_bytes(const fftgal_job_desc *d, size_t *gbytes, size_t *fwbytes) {
+    const size_t cells = (size_t)d->Ng * d->Ng * d->Ng, NG = d->interlace ? 2 : 1;
+    *gbytes = (NG * cells * sizeof(int64_t) + 255) & ~(size_t)255;
+    *fwbytes = ((NG * cells * sizeof(float) + 255) & ~(size_t)255) +
+               fgk_paint_batch_ws_bytes(d->Np_max);
+}
+
+static size_t pool_size(size_t gmax, size_t fwmax, int sets) { return sets * gmax + fwmax; }
+
+/* the grids, spectra, bin tables and paint batch of round r, in the pool */
+static int round_setup(fftgal_batch_t *b, int r) {
+    batch_round *R = &b->rnd[r];
+    /* groups by grid side and interlacing, in the round's catalog order */
+    R->grp = (batch_group *)calloc((size_t)R->njobs, sizeof(batch_group));
+    if (!R->grp) { fgk_set_error("out of host memory"); return -1; }
+    for (int k = 0; k < R->njobs; ++k) {
+        batch_job *J = &b->job[R->jobs[k]];
+        int q = 0;
+        while (q < R->ngroups && !(R->grp[q].Ng == J->d.Ng && R->grp[q].interlace == J->d.interlace))
+            ++q;
+        if (q == R->ngroups) {
+            R->grp[q].Ng = J->d.Ng;
+            R->grp[q].interlace = J->d.interlace;
+            R->grp[q].side = side_of(b, J->d.Ng);
+            ++R->ngroups;
+        }
+        J->round = r;
+        J->group = q;
+        J->slot = R->grp[q].n++;
+        J->local = k;
     }
-    return 0;
+    char *fw = b->pool + (size_t)(b->gset[1] ? 2 : 1) * b->gwords * sizeof(int64_t);
+    for (int q = 0; q < R->ngroups; ++q) {
+        batch_group *G = &R->grp[q];
+        const int NG = G->interlace ? 2 : 1;
+        const size_t cells = (size_t)G->Ng * G->Ng * G->Ng;
+        G->jobs = (int *)calloc((size_t)G->n, sizeof(int));
+        G->hjob = (fgk_binjob *)calloc((size_t)G->n, sizeof(fgk_binjob));
+        if (!G->jobs || !G->hjob) { fgk_set_error("out of host memory"); return -1; }
+        G->goff = R->gwords;
+        R->gwords += (((size_t)G->n * NG * cells * sizeof(int64_t) + 255) & ~(size_t)255) / 8;
+        G->g = b->gset[0] + G->goff;
+        G->g2 = b->gset[1] ? b->gset[1] + G->goff : NULL;
+        G->f = (float *)fw;
+        fw += ((size_t)G->n * NG * cells * sizeof(float) + 255) & ~(size_t)255;
+        TRY(fgk_malloc((void **)&G->djob, (size_t)G->n * sizeof(fgk_binjob)));
+    }
+    for (int k = 0; k < R->njobs; ++k) {
+        const batch_job *J = &b->job[R->jobs[k]];
+        R->grp[J->group].jobs[J->slot] = R->jobs[k];
+    }
+    for (int q = 0; q < R->ngroups; ++q) {
+        batch_group *G = &R->grp[q];
+        const batch_side *S = &b->side[G->side];
+        const int NG = G->interlace ? 2 : 1;
+        const size_t cells = (size_t)G->Ng * G->Ng * G->Ng;
+        G->spec.Ng = G->Ng;
+        G->spec.Nmu = b->Nmu;
+        G->spec.los[0] = b->los[0];
+        G->spec.los[1] = b->los[1];
+        G->spec.los[2] = b->los[2];
+        G->spec.do_pl = 1;
+        G->spec.geometry = 0;
+        G->spec.bk = S->tune.bin_bk;
+        for (int k = 0; k < G->n; ++k) {
+            batch_job *J = &b->job[G->jobs[k]];
+            fgk_binjob *h = &G->hjob[k];
+            h->A = G->f + (size_t)k * NG * cells;
+            h->B = G->interlace ? h->A + cells : NULL;
+            h->sums = b->d_bins + 2 * J->boff;
+            h->NK = J->NK;
+            h->dKinv = J->KF / J->d.dK;
+            if (J->NK > G->NK_max) G->NK_max = J->NK;
+        }
+    }
+    /* the paint batch writes each catalog's grids in its group; its sort
+     * words follow the round's spectra */
+    fgk_paintjob_desc *pd = (fgk_paintjob_desc *)calloc((size_t)R->njobs, sizeof *pd);
+    if (!pd) { fgk_set_error("out of host memory"); return -1; }
+    long Np_round = 0;
+    for (int k = 0; k < R->njobs; ++k) {
+        const batch_job *J = &b->job[R->jobs[k]];
+        const batch_group *G = &R->grp[J->group];
+        const batch_side *S = &b->side[G->side];
+        const size_t cells = (size_t)G->Ng * G->Ng * G->Ng, NG = G->interlace ? 2 : 1;
+        pd[k].Ng = J->d.Ng;
+        pd[k].tile = J->d.paint_tile > 0 ? J->d.paint_tile : S->tune.paint_tile;
+        pd[k].Hinv = J->d.Ng / J->d.L;
+        pd[k].Np_max = J->d.Np_max;
+        pd[k].interlace = J->d.interlace;
+        pd[k].gridA = G->g + (size_t)J->slot * NG * cells;
+        pd[k].gridB = G->interlace ? pd[k].gridA + cells : NULL;
+        if (G->g2) {
+            pd[k].gridA2 = G->g2 + (size_t)J->slot * NG * cells;
+            pd[k].gridB2 = G->interlace ? pd[k].gridA2 + cells : NULL;
+        }
+        Np_round += J->d.Np_max;
+    }
+    const size_t ws = fgk_paint_batch_ws_bytes(Np_round);
+    int rc = 0;
+    if (fw + ws > b->pool + b->pool_bytes) {
+        fgk_set_error("batch: round %d needs more than the pool", r);
+        rc = -1;
+    }
+    if (!rc) rc = fgk_paint_batch_create_ws(&R->paint, R->njobs, pd, fw, ws);
+    free(pd);
+    return rc;
+}
+
+/* rounds of the catalogs in `order`: a catalog starts a new round when
+ * adding it would take the pool past `budget` or the round past `target`
+ * bytes (sets grid sets, spectra and sort words).  Fills b->rnd when b is
+ * given; returns the number of rounds (-1: out of host memory) and the
+ * largest round's grid and spectra-and-words bytes. */
+static int form_rounds(fftgal_batch_t *b, const fftgal_job_desc *jobs, const int *order,
+                       int njobs, size_t budget, int sets, size_t target, size_t *gmax_out,
+                       size_t *fwmax_out) {
+    size_t gmax = 0, fwmax = 0, gcur = 0, fwcur = 0;
+    int nr = 0;
+    for (int i = 0; i < njobs; ++i) {
+        size_t gj, fwj;
+        job_bytes(&jobs[order[i]], &gj, &fwj);
+        const size_t g2 = gcur + gj, fw2 = fwcur + fwj;
+        const int fits = pool_size(g2 > gmax ? g2 : gmax, fw2 > fwmax ? fw2 : fwmax, sets) <= budget
+                         && (size_t)sets * g2 + fw2 <= target;
+        if (nr == 0 || !fits) {
+            if (b) {
+                b->rnd[nr].jobs = (int *)calloc((size_t)njobs, sizeof(int));
+                if (!b->rnd[nr].jobs) return -1;
+            }
+            ++nr;
+            gcur = fwcur = 0;
+        }
+        if (b) b->rnd[nr - 1].jobs[b->rnd[nr - 1].njobs++] = order[i];
+        gcur += gj;
+        fwcur += fwj;
+        if (gcur > gmax) gmax = gcur;
+        if (fwcur > fwmax) fwmax = fwcur;
+    }
+    if (b) b->nrounds = nr;
+    *gmax_out = gmax;
+    *fwmax_out = fwmax;
+    return nr;
 }
 
 int fftgal_batch_create(fftgal_batch_t **out, int njobs, const fftgal_job_desc *jobs,
@@ -885,12 +1041,19 @@ int fftgal_batch_create(fftgal_batch_t **out, int njobs, const fftgal_job_desc *
         if (a > 1e-7) { b->los[0] /= a; b->los[1] /= a; b->los[2] /= a; }
     }
     b->job = (batch_job *)calloc((size_t)njobs, sizeof(batch_job));
-    b->grp = (batch_group *)calloc((size_t)njobs, sizeof(batch_group));
-    int rc = (!b->job || !b->grp) ? -1 : 0;
+    b->rnd = (batch_round *)calloc((size_t)njobs, sizeof(batch_round));
+    b->side = (batch_side *)calloc((size_t)njobs, sizeof(batch_side));
+    b->px = (const float **)calloc((size_t)njobs, sizeof(float *));
+    b->py = (const float **)calloc((size_t)njobs, sizeof(float *));
+    b->pz = (const float **)calloc((size_t)njobs, sizeof(float *));
+    b->pNp = (long *)calloc((size_t)njobs, sizeof(long));
+    int *order = (int *)calloc((size_t)njobs, sizeof(int));
+    int rc = (!b->job || !b->rnd || !b->side || !b->px || !b->py || !b->pz || !b->pNp || !order)
+                 ? -1 : 0;
     if (rc) fgk_set_error("out of host memory");
 #define STEP(call) do { if (!rc) rc = (call); } while (0)
     STEP(fgk_stream_create(&b->s));
-    /* catalogs, bin layout and grid groups */
+    /* catalogs and bin layout */
     for (int j = 0; j < njobs && !rc; ++j) {
         batch_job *J = &b->job[j];
         J->d = jobs[j];
@@ -903,8 +1066,6 @@ int fftgal_batch_create(fftgal_batch_t **out, int njobs, const fftgal_job_desc *
         bs.do_pl = 1;
         J->boff = b->nbins;
         b->nbins += fgk_bins_doubles(&bs);
-        J->group = group_of(b, J->d.Ng, J->d.interlace);
-        J->slot = b->grp[J->group].n++;
         const size_t W1 = b->W ? (size_t)b->W : 1;
         J->gK = (double *)calloc((size_t)J->NK, sizeof(double));
         J->gN = (long *)calloc((size_t)J->NK, sizeof(long));
@@ -914,67 +1075,82 @@ int fftgal_batch_create(fftgal_batch_t **out, int njobs, const fftgal_job_desc *
             fgk_set_error("out of host memory");
             rc = -1;
         }
+        order[j] = j;
+        side_of(b, J->d.Ng);
     }
-    for (int q = 0; q < b->ngroups && !rc; ++q) {
-        batch_group *G = &b->grp[q];
-        G->jobs = (int *)calloc((size_t)G->n, sizeof(int));
-        if (!G->jobs) { fgk_set_error("out of host memory"); rc = -1; }
+    /* FFT plans and k-space tables per grid side (tuning lines are kept per
+     * interlacing: the side's first catalog's) */
+    for (int i = 0; i < b->nsides && !rc; ++i) {
+        int inter = 1;
+        for (int j = 0; j < njobs; ++j)
+            if (b->job[j].d.Ng == b->side[i].Ng) { inter = b->job[j].d.interlace; break; }
+        rc = side_setup(b, &b->side[i], inter, wisdom);
     }
-    for (int j = 0; j < njobs && !rc; ++j) b->grp[b->job[j].group].jobs[b->job[j].slot] = j;
+    /* rounds: catalogs in (grid side, interlacing, index) order, each round
+     * as large as the pool bound allows with the grid sets of the largest */
+    for (int i = 1; i < njobs && !rc && 0; ++i) {
+        const int t = order[i];
+        int k = i;
+        for (; k > 0; --k) {
+            const fftgal_job_desc *a = &jobs[order[k - 1]], *c = &jobs[t];
+            const int ia = a->interlace ? 1 : 0, ic = c->interlace ? 1 : 0;
+            if (a->Ng < c->Ng || (a->Ng == c->Ng && ia <= ic)) break;
+            order[k] = order[k - 1];
+        }
+        order[k] = t;
+    }
+    const char *e = getenv("FFTGAL_BATCH_BYTES");
+    const size_t budget = e && atof(e) > 0 ? (size_t)atof(e) : BATCH_POOL_DEFAULT;
+    const char *es = getenv("FFTGAL_GRID_SETS");
+    const int sets = (es && atoi(es) == 1) ? 1 : 2;
+    /* the fewest rounds the bound allows (greedy), then rounds of about
+     * equal size: a round's tail (its last, longest workgroups) repeats per
+     * round, so one small last round costs as much as a full one */
+    size_t total = 0;
+    for (int i = 0; i < njobs; ++i) {
+        size_t gj, fwj;
+        job_bytes(&jobs[i], &gj, &fwj);
+        total += (size_t)sets * gj + fwj;
+    }
+    int nr = 0;
+    size_t gmax = 0, fwmax = 0;
+    if (!rc) nr = form_rounds(NULL, jobs, order, njobs, budget, sets, (size_t)-1, &gmax, &fwmax);
+    if (!rc && nr > 1) {
+        size_t g2 = 0, fw2 = 0;
+        const int n2 = form_rounds(NULL, jobs, order, njobs, budget, sets, (total + nr - 1) / nr,
+                                   &g2, &fw2);
+        if (n2 == nr) {
+            nr = form_rounds(b, jobs, order, njobs, budget, sets, (total + nr - 1) / nr, &gmax,
+                             &fwmax);
+        } else {
+            nr = form_rounds(b, jobs, order, njobs, budget, sets, (size_t)-1, &gmax, &fwmax);
+        }
+    } else if (!rc) {
+        nr = form_rounds(b, jobs, order, njobs, budget, sets, (size_t)-1, &gmax, &fwmax);
+    }
+    if (!rc && nr < 0) { fgk_set_error("out of host memory"); rc = -1; }
+    free(order);
     STEP(fgk_malloc((void **)&b->d_bins, 2 * b->nbins * sizeof(int64_t)));
     STEP(fgk_host_alloc((void **)&b->h_bins, 2 * b->nbins * sizeof(int64_t)));
     if (!rc && !(b->h_sums = (double *)malloc(b->nbins * sizeof(double)))) {
         fgk_set_error("out of host memory");
         rc = -1;
     }
-    /* one block per grid set for the int64 grids of all groups (each group's
-     * part 256-byte aligned), so that one region clears a whole set */
-    for (int q = 0; q < b->ngroups; ++q) {
-        batch_group *G = &b->grp[q];
-        const size_t w = (size_t)G->n * (G->interlace ? 2 : 1) * G->Ng * G->Ng * G->Ng;
-        G->goff = b->gwords;
-        b->gwords += (w + 31) & ~(size_t)31;
-    }
-    STEP(fgk_malloc((void **)&b->gset[0], b->gwords * sizeof(int64_t)));
-    STEP(fgk_fill_zero(b->gset[0], b->gwords * sizeof(int64_t), b->s));
-    for (int q = 0; q < b->ngroups && !rc; ++q) rc = group_setup(b, &b->grp[q], wisdom);
-    if (!rc && want_second_set(b->gwords * sizeof(int64_t))) {
-        if (fgk_malloc((void **)&b->gset[1], b->gwords * sizeof(int64_t)) == 0)
-            STEP(fgk_fill_zero(b->gset[1], b->gwords * sizeof(int64_t), b->s));
-        else
-            b->gset[1] = NULL;
-    }
-    for (int q = 0; q < b->ngroups && b->gset[1]; ++q)
-        b->grp[q].g2 = b->gset[1] + b->grp[q].goff;
-    /* the paint batch writes each catalog's grids in its group */
+    /* the pool: grid sets (zeroed once; every round leaves its set zero)
+     * then the spectra and sort words of one round */
+    b->gwords = gmax / 8;
+    b->pool_bytes = pool_size(gmax, fwmax, sets);
+    STEP(fgk_malloc((void **)&b->pool, b->pool_bytes));
     if (!rc) {
-        fgk_paintjob_desc *pd = (fgk_paintjob_desc *)calloc((size_t)njobs, sizeof *pd);
-        if (!pd) { fgk_set_error("out of host memory"); rc = -1; }
-        for (int j = 0; j < njobs && !rc; ++j) {
-            const batch_job *J = &b->job[j];
-            const batch_group *G = &b->grp[J->group];
-            const size_t cells = (size_t)G->Ng * G->Ng * G->Ng;
-            fftgal_tuning t;
-            fftgal_tune_lookup(wisdom, G->Ng, G->interlace, &t);
-            pd[j].Ng = J->d.Ng;
-            pd[j].tile = J->d.paint_tile > 0 ? J->d.paint_tile : t.paint_tile;
-            pd[j].Hinv = J->d.Ng / J->d.L;
-            pd[j].Np_max = J->d.Np_max;
-            pd[j].interlace = J->d.interlace;
-            pd[j].gridA = G->g + (size_t)J->slot * (G->interlace ? 2 : 1) * cells;
-            pd[j].gridB = G->interlace ? pd[j].gridA + cells : NULL;
-            if (G->g2) {
-                pd[j].gridA2 = G->g2 + (size_t)J->slot * (G->interlace ? 2 : 1) * cells;
-                pd[j].gridB2 = G->interlace ? pd[j].gridA2 + cells : NULL;
-            }
-        }
-        if (!rc) rc = fgk_paint_batch_create(&b->paint, njobs, pd);
-        free(pd);
+        b->gset[0] = (int64_t *)b->pool;
+        b->gset[1] = sets == 2 ? (int64_t *)(b->pool + gmax) : NULL;
     }
+    STEP(fgk_fill_zero(b->pool, (size_t)sets * gmax, b->s));
+    for (int r = 0; r < b->nrounds && !rc; ++r) rc = round_setup(b, r);
     /* K sums and mode counts per catalog */
     for (int j = 0; j < njobs && !rc; ++j) {
         batch_job *J = &b->job[j];
-        fgk_binspec g = b->grp[J->group].spec;
+        fgk_binspec g = b->rnd[J->round].grp[J->group].spec;
         g.NK = J->NK;
         g.dKinv = J->KF / J->d.dK;
         rc = bin_geometry(&g, J->NK, b->W, b->s, J->gK, J->gN, J->gKmu, J->gNmu);
@@ -1005,74 +1181,105 @@ static int batch_tables(fftgal_batch_t *b, const float *const *x, const float *c
             return -2;
         }
     }
-    TRY(fgk_paint_batch_set(b->paint, x, y, z, Np, b->s));
+    for (int r = 0; r < b->nrounds; ++r) {
+        const batch_round *R = &b->rnd[r];
+        for (int k = 0; k < R->njobs; ++k) {
+            const int j = R->jobs[k];
+            b->px[k] = x[j];
+            b->py[k] = y[j];
+            b->pz[k] = z[j];
+            b->pNp[k] = Np[j];
+        }
+        TRY(fgk_paint_batch_set(R->paint, b->px, b->py, b->pz, b->pNp, b->s));
+    }
     int same = b->tables_ok;
     for (int j = 0; j < b->njobs; ++j)
         if (b->job[j].Np != Np[j] || b->job[j].V != V[j]) same = 0;
     if (same) return 0;
     TRY(fgk_stream_sync(b->s));
-    for (int q = 0; q < b->ngroups; ++q) {
-        batch_group *G = &b->grp[q];
-        for (int k = 0; k < G->n; ++k) {
-            batch_job *J = &b->job[G->jobs[k]];
-            fgk_binjob *h = &G->hjob[k];
-            const long n = Np[G->jobs[k]];
-            const double v = V[G->jobs[k]];
-            catalog_norms(G->Ng, J->d.L, n, v, &h->norm, &h->Vinv, &h->Pshot);
-            /* the FFT reads the int64 grids unscaled */
-            h->norm *= fgk_paint_fixed_scale();
-            h->npart = (double)n;
-            J->Np = n;
-            J->V = v;
+    for (int r = 0; r < b->nrounds; ++r) {
+        for (int q = 0; q < b->rnd[r].ngroups; ++q) {
+            batch_group *G = &b->rnd[r].grp[q];
+            for (int k = 0; k < G->n; ++k) {
+                batch_job *J = &b->job[G->jobs[k]];
+                fgk_binjob *h = &G->hjob[k];
+                const long n = Np[G->jobs[k]];
+                const double v = V[G->jobs[k]];
+                catalog_norms(G->Ng, J->d.L, n, v, &h->norm, &h->Vinv, &h->Pshot);
+                /* the FFT reads the int64 grids unscaled */
+                h->norm *= fgk_paint_fixed_scale();
+                h->npart = (double)n;
+                J->Np = n;
+                J->V = v;
+            }
+            TRY(fgk_h2d(G->djob, G->hjob, (size_t)G->n * sizeof(fgk_binjob), b->s));
         }
-        TRY(fgk_h2d(G->djob, G->hjob, (size_t)G->n * sizeof(fgk_binjob), b->s));
     }
     TRY(fgk_stream_sync(b->s));
     b->tables_ok = 1;
     return 0;
 }
 
-/* the batch's stages in the order and under the names of
- * fftgal_power_stage_name; ev (NSTAGES+1 events or NULL) brackets them */
-static int batch_work(fftgal_batch_t *b, fgk_event_t *ev) {
+/* the batch's rounds, each through the stages in the order and under the
+ * names of fftgal_power_stage_name; ev (NSTAGES+1 events per round, or
+ * NULL) brackets them.  set0: grid set of the first round. */
+static int batch_work(fftgal_batch_t *b, int set0, fgk_event_t *ev) {
     int st = 0;
-    const int set = b->parity;
-    for (int q = 0; q < b->ngroups; ++q) b->grp[q].spec.fs_n1 = fgk_fft_fourstep(b->grp[q].plan);
-    /* with two grid sets the paint writes set `set` and clears the other */
-    if (b->gset[1])
-        TRY(fgk_paint_batch_select(b->paint, set, b->gset[set ^ 1],
-                                   b->gwords * sizeof(int64_t)));
 #define MARK() do { if (ev) TRY(fgk_event_record(ev[st], b->s)); ++st; } while (0)
-    MARK();
-    TRY(fgk_fill_zero(b->d_bins, 2 * b->nbins * sizeof(int64_t), b->s));
-    for (int stage = 0; stage < NPAINT; ++stage) {
-        MARK();
-        TRY(fgk_paint_batch_stage(b->paint, stage, b->s));
-    }
-    MARK();
-    for (int q = 0; q < b->ngroups; ++q) {
-        batch_group *G = &b->grp[q];
-        TRY(fgk_fft_rows_f32_from_fixed(G->plan, G->f, set ? G->g2 : G->g, 1.0,
-                                        G->g2 ? 0 : 1, G->n * (G->interlace ? 2 : 1), b->s));
-    }
-    for (int axis = 1; axis >= 0; --axis) {
-        MARK();
-        for (int q = 0; q < b->ngroups; ++q) {
-            batch_group *G = &b->grp[q];
-            TRY(fgk_fft_cols_f32(G->plan, G->f, axis, -1, G->n * (G->interlace ? 2 : 1), b->s));
+    for (int r = 0; r < b->nrounds; ++r) {
+        batch_round *R = &b->rnd[r];
+        const int set = b->gset[1] ? (set0 + r) & 1 : 0;
+        for (int q = 0; q < R->ngroups; ++q)
+            R->grp[q].spec.fs_n1 = fgk_fft_fourstep(b->side[R->grp[q].side].plan);
+        /* with two grid sets the paint writes set `set` and clears the other
+         * as far as its last round wrote it: the previous round, or for
+         * the first round the last of the previous enqueue (on the first
+         * enqueue that part is zero already) */
+        if (b->gset[1]) {
+            const batch_round *P = &b->rnd[(r + b->nrounds - 1) % b->nrounds];
+            TRY(fgk_paint_batch_select(R->paint, set, b->gset[set ^ 1],
+                                       P->gwords * sizeof(int64_t)));
         }
+        MARK();
+        if (r == 0) TRY(fgk_fill_zero(b->d_bins, 2 * b->nbins * sizeof(int64_t), b->s));
+        for (int stage = 0; stage < NPAINT; ++stage) {
+            MARK();
+            TRY(fgk_paint_batch_stage(R->paint, stage, b->s));
+        }
+        MARK();
+        for (int q = 0; q < R->ngroups; ++q) {
+            batch_group *G = &R->grp[q];
+            TRY(fgk_fft_rows_f32_from_fixed(b->side[G->side].plan, G->f, set ? G->g2 : G->g, 1.0,
+                                            b->gset[1] ? 0 : 1, G->n * (G->interlace ? 2 : 1),
+                                            b->s));
+        }
+        for (int axis = 1; axis >= 0; --axis) {
+            MARK();
+            for (int q = 0; q < R->ngroups; ++q) {
+                batch_group *G = &R->grp[q];
+                TRY(fgk_fft_cols_f32(b->side[G->side].plan, G->f, axis, -1,
+                                     G->n * (G->interlace ? 2 : 1), b->s));
+            }
+        }
+        MARK();
+        for (int q = 0; q < R->ngroups; ++q) {
+            batch_group *G = &R->grp[q];
+            const batch_side *S = &b->side[G->side];
+            TRY(fgk_fused_bin_f32_batch(&G->spec, G->djob, G->n, G->NK_max, S->d_phase,
+                                        S->d_winv, b->s));
+        }
+        MARK();
+        if (r == b->nrounds - 1)
+            TRY(fgk_d2h(b->h_bins, b->d_bins, 2 * b->nbins * sizeof(int64_t), b->s));
+        MARK();
     }
-    MARK();
-    for (int q = 0; q < b->ngroups; ++q) {
-        batch_group *G = &b->grp[q];
-        TRY(fgk_fused_bin_f32_batch(&G->spec, G->djob, G->n, G->NK_max, G->d_phase, G->d_winv,
-                                    b->s));
-    }
-    MARK();
-    TRY(fgk_d2h(b->h_bins, b->d_bins, 2 * b->nbins * sizeof(int64_t), b->s));
-    MARK();
 #undef MARK
     return 0;
+}
+
+/* grid set of the round after this enqueue's last */
+static int next_parity(const fftgal_batch_t *b, int set0) {
+    return b->gset[1] ? (set0 + b->nrounds) & 1 : 0;
 }
 
 int fftgal_batch_enqueue(fftgal_batch_t *b, const float *const *x, const float *const *y,
@@ -1080,13 +1287,13 @@ int fftgal_batch_enqueue(fftgal_batch_t *b, const float *const *x, const float *
     TRY(batch_tables(b, x, y, z, Np, V));
     const int q = b->parity;
     if (!b->use_graph) {
-        TRY(batch_work(b, NULL));
-        if (b->gset[1]) b->parity = q ^ 1;
+        TRY(batch_work(b, q, NULL));
+        b->parity = next_parity(b, q);
         return 0;
     }
     if (!b->graph[q]) {
         TRY(fgk_graph_begin(b->s));
-        int rc = batch_work(b, NULL);
+        int rc = batch_work(b, q, NULL);
         fgk_graph_t g = NULL;
         int rc2 = fgk_graph_end(b->s, &g);
         if (rc) {
@@ -1097,11 +1304,15 @@ int fftgal_batch_enqueue(fftgal_batch_t *b, const float *const *x, const float *
         b->graph[q] = g;
     }
     TRY(fgk_graph_launch(b->graph[q], b->s));
-    if (b->gset[1]) b->parity = q ^ 1;
+    b->parity = next_parity(b, q);
     return 0;
 }
 
 int fftgal_batch_grid_sets(const fftgal_batch_t *b) { return b->gset[1] ? 2 : 1; }
+
+int fftgal_batch_rounds(const fftgal_batch_t *b) { return b->nrounds; }
+
+size_t fftgal_batch_pool_bytes(const fftgal_batch_t *b) { return b->pool_bytes; }
 
 int fftgal_batch_sync(fftgal_batch_t *b) { return fgk_stream_sync(b->s); }
 
@@ -1109,14 +1320,24 @@ int fftgal_batch_profile(fftgal_batch_t *b, const float *const *x, const float *
                          const float *const *z, const long *Np, const double *V,
                          float *stage_ms) {
     TRY(batch_tables(b, x, y, z, Np, V));
-    fgk_event_t ev[NSTAGES + 1] = {0};
+    const int n = b->nrounds * (NSTAGES + 1);
+    fgk_event_t *ev = (fgk_event_t *)calloc((size_t)n, sizeof(fgk_event_t));
+    if (!ev) { fgk_set_error("out of host memory"); return -1; }
     int rc = 0;
-    for (int i = 0; i <= NSTAGES && !rc; ++i) rc = fgk_event_create(&ev[i]);
-    if (!rc) rc = batch_work(b, ev);
-    if (!rc && b->gset[1]) b->parity ^= 1;
+    for (int i = 0; i < n && !rc; ++i) rc = fgk_event_create(&ev[i]);
+    const int q = b->parity;
+    if (!rc) rc = batch_work(b, q, ev);
+    if (!rc) b->parity = next_parity(b, q);
     if (!rc) rc = fgk_stream_sync(b->s);
-    for (int i = 0; i < NSTAGES && !rc; ++i) rc = fgk_event_elapsed_ms(&stage_ms[i], ev[i], ev[i + 1]);
-    for (int i = 0; i <= NSTAGES; ++i) fgk_event_destroy(ev[i]);
+    for (int i = 0; i < NSTAGES; ++i) stage_ms[i] = 0.0f;
+    for (int r = 0; r < b->nrounds && !rc; ++r)
+        for (int i = 0; i < NSTAGES && !rc; ++i) {
+            float ms = 0.0f;
+            rc = fgk_event_elapsed_ms(&ms, ev[r * (NSTAGES + 1) + i], ev[r * (NSTAGES + 1) + i + 1]);
+            stage_ms[i] += ms;
+        }
+    for (int i = 0; i < n; ++i) fgk_event_destroy(ev[i]);
+    free(ev);
     return rc;
 }
 
@@ -1146,7 +1367,8 @@ int fftgal_batch_use_graph(fftgal_batch_t *b, int on) {
 
 int fftgal_batch_tile(const fftgal_batch_t *b, int job) {
     if (job < 0 || job >= b->njobs) return -2;
-    return fgk_paint_batch_tile(b->paint, job);
+    const batch_job *J = &b->job[job];
+    return fgk_paint_batch_tile(b->rnd[J->round].paint, J->local);
 }
 
 void *fftgal_batch_stream(fftgal_batch_t *b) { return b->s; }

@@ -179,3 +179,45 @@ def test_batch_rejects_bad_input(torch_cuda, lib):
     with pytest.raises(RuntimeError):
         b.run([(0, 0, 0, 11, 1e6)])                                 # Np > Np_max
     b.close()
+
+
+@pytest.mark.parametrize("sets", [2, 1])
+def test_batch_rounds_match_one_round(torch_cuda, lib, monkeypatch, sets):
+    """A device pool bound (FFTGAL_BATCH_BYTES) small enough to cut the
+    catalogs into several rounds, which share the pool's grid sets, spectra
+    and sort words: the bins equal those of the batch in one round bit for
+    bit (the same integer grids and fixed-point sums), over graph replays
+    and eager runs, with the grid sets alternating across rounds and
+    enqueues."""
+    import fftgal_amd as fa
+    torch = torch_cuda
+    cats = _catalogs(torch, [(60000, 500.0, 1), (25000, 300.0, 2), (90000, 800.0, 3)])
+    los, Nmu = (0.0, 0.0, 1.0), 5
+    plan = [(0, {"Ng": 48, "L": 500.0}), (0, {"Ng": 48, "L": 250.0}),
+            (1, {"Ng": 32, "L": 300.0, "interlace": False}), (1, {"Ng": 64, "L": 300.0}),
+            (2, {"Ng": 40, "L": 800.0}), (2, {"Ng": 64, "L": 400.0})]
+    jobs = []
+    for c, j in plan:
+        j = dict(j)
+        j["Np_max"] = cats[c][1]
+        jobs.append(j)
+    args = [(cats[c][0][0].data_ptr(), cats[c][0][1].data_ptr(), cats[c][0][2].data_ptr(),
+             cats[c][1], cats[c][2]) for c, _ in plan]
+    one = _batch(fa, monkeypatch, sets, jobs, los=los, Nmu=Nmu)
+    assert one.rounds() == 1
+    ref = one.run(args)
+    one.close()
+    bound = (16 if sets == 2 else 9) << 20
+    monkeypatch.setenv("FFTGAL_BATCH_BYTES", str(bound))
+    many = _batch(fa, monkeypatch, sets, jobs, los=los, Nmu=Nmu)
+    monkeypatch.delenv("FFTGAL_BATCH_BYTES", raising=False)
+    print("rounds", many.rounds(), "pool MiB", many.pool_bytes() / 2 ** 20)
+    assert many.rounds() >= 3
+    assert many.pool_bytes() <= bound
+    for mode in ("graph", "graph", "eager", "graph"):
+        many.use_graph(mode == "graph")
+        res = many.run(args)
+        for q in range(len(plan)):
+            for key in ("N", "K", "Pl", "Nmu", "Kmu", "Pmu"):
+                assert np.array_equal(res[q][key], ref[q][key], equal_nan=True), (mode, q, key)
+    many.close()
