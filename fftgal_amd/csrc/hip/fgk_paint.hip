@@ -244,7 +244,8 @@ __device__ __forceinline__ int batch_job(long e, const long *__restrict__ off, i
 // bits in all): the dual stencil has base floor(v) and fraction frac(v), the
 // plain one base floor(v + 1/2) and fraction frac(v + 1/2).
 __device__ __forceinline__ void axis_key(float xf, const KeyGeo &g, int &tc, uint32_t &q) {
-    const double u = __dmul_rn((double)xf, g.Hinv) - 1.5;   // exact subtraction
+#pragma clang fp contract(off)
+    const double u = (double)xf * g.Hinv - 1.5;   // the product rounded, then an exact subtraction
     const double fl = floor(u);
     const double d = u - fl;
     int gi = (int)fl;
