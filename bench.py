@@ -720,7 +720,7 @@ def main():
                    "Np": Np, "Ng": Ng, "L": L, "fold": args.fold, "interlace": True,
                    "Nmu": nmu, "grid_sets": sets,
                    "precision": "int64 fixed-point paint, FP32 FFT (FP64 twiddles), "
-                                "FP64 k-space math and bin sums",
+                                "FP64 k-space math, exact fixed-point bin sums",
                    "parallelism": "1 GPU"},
         "roofline": roofline,
         "paint_stencil": paint,

@@ -24,19 +24,23 @@
 //    8-bit extension of 2 more bits per axis; between the levels the
 //    extension shares a 16-bit side word with the tile-within-bucket digit,
 //    which is all the level-2 histogram reads.  The paint
-//    then runs one workgroup per work item (<= cap particles of one tile):
-//    the 4x4x4 stencils of both grids go into LDS footprints ((T+3)^3
-//    plain, (T+4)^3 dual cells) as int32 fixed point.  The footprints are
-//    stored in 4x4x4 bricks, so a cell's LDS bank is fixed by its
-//    coordinates mod 4, and each lane walks its stencil in the residue
-//    order given by its six low bits: at every step the 64 lanes' atomics
-//    fall in 64 distinct banks (and so never on one address) whatever
-//    particles the wave holds.  Non-zero footprint cells are flushed into
-//    the int64 grids with global integer atomics.
+//    then runs one workgroup per pencil segment (the tiles along k of one
+//    tile column of a coarse bucket) or per dense item (<= cap particles of
+//    a tile holding more): the 4x4x4 stencils of both grids go into LDS
+//    footprints ((T+4)^3 plain, (T+3)^3 dual cells, one-sided: a tile's
+//    stencils reach past its upper faces only) as int32 fixed point.  The
+//    footprints are stored in 4x4x4 bricks, so a cell's LDS bank is fixed
+//    by its coordinates mod 4, and each lane walks its stencil in the
+//    residue order given by its six low bits: at every step the 64 lanes'
+//    atomics fall in 64 distinct banks (and so never on one address)
+//    whatever particles the wave holds.  Non-zero footprint cells are
+//    flushed into the int64 grids with global integer atomics; along a
+//    pencil only the T planes a tile owns are flushed (aligned 128-byte
+//    rows) and the planes above are carried in LDS into the next tile.
 //    Integer addition is associative, so the grids are bit-identical run to
-//    run whatever the scheduling.  Each tile rounds its weights to the
-//    finest fixed point its particle count allows without int32 overflow
-//    (2^-19 particle for 8192-particle work items, up to 2^-25), with a
+//    run whatever the scheduling.  Each segment or item rounds its weights
+//    to the finest fixed point its particle count allows without int32
+//    overflow (2^-18 particle for 16384 particles, up to 2^-25), with a
 //    deterministic dither that makes every rounding unbiased, so no term is
 //    lost on average however small; the grids hold one common unit.
 //    Grids with more tiles than even the wide two levels address (over

@@ -701,9 +701,9 @@ int fftgal_memcpy_d2h(void *dst, const void *src, unsigned long bytes) {
  * Batches of catalogs (fftgal_batch_*): many catalogs, each on its own grid
  * (side, box = catalog box / fold, interlacing, k binning).
  *
- * Device memory is a bounded pool.  The catalogs are cut into rounds, in
- * grid-side order, each round as large as the pool allows; the rounds run
- * one after the other through the same memory:
+ * Device memory is a bounded pool.  The catalogs are cut into rounds of
+ * about equal size, in input order, as few as the pool allows; the rounds
+ * run one after the other through the same memory:
  *   pool = [grid set 0 | grid set 1 | spectra and sort words of a round]
  * A grid set holds the int64 grids of one round (the largest round's size);
  * the rounds alternate between the two sets, and each round's paint clears
@@ -1086,19 +1086,9 @@ int fftgal_batch_create(fftgal_batch_t **out, int njobs, const fftgal_job_desc *
             if (b->job[j].d.Ng == b->side[i].Ng) { inter = b->job[j].d.interlace; break; }
         rc = side_setup(b, &b->side[i], inter, wisdom);
     }
-    /* rounds: catalogs in (grid side, interlacing, index) order, each round
-     * as large as the pool bound allows with the grid sets of the largest */
-    for (int i = 1; i < njobs && !rc && 0; ++i) {
-        const int t = order[i];
-        int k = i;
-        for (; k > 0; --k) {
-            const fftgal_job_desc *a = &jobs[order[k - 1]], *c = &jobs[t];
-            const int ia = a->interlace ? 1 : 0, ic = c->interlace ? 1 : 0;
-            if (a->Ng < c->Ng || (a->Ng == c->Ng && ia <= ic)) break;
-            order[k] = order[k - 1];
-        }
-        order[k] = t;
-    }
+    /* rounds: catalogs in input order (each round then holds a mix of grid
+     * sides, as the caller's list does), each round as large as the pool
+     * bound allows with the grid sets of the largest */
     const char *e = getenv("FFTGAL_BATCH_BYTES");
     const size_t budget = e && atof(e) > 0 ? (size_t)atof(e) : BATCH_POOL_DEFAULT;
     const char *es = getenv("FFTGAL_GRID_SETS");

@@ -181,10 +181,10 @@ int fftgal_batch_tile(const fftgal_batch_t *b, int job);
 int fftgal_batch_grid_sets(const fftgal_batch_t *b);
 /* Device memory: the grids, spectra and sort words of a batch live in one
  * pool bounded by FFTGAL_BATCH_BYTES (environment, bytes, at creation;
- * default 34 GiB).  The catalogs run in rounds, in grid-side order, each as
- * large as the pool allows (a catalog larger than the bound gets a round
- * of its own); results do not depend on the rounds.  The number of rounds
- * and the pool's size: */
+ * default 34 GiB).  The catalogs run in rounds of about equal size, in
+ * input order, as few as the bound allows (a catalog larger than the bound
+ * gets a round of its own); results do not depend on the rounds.  The
+ * number of rounds and the pool's size: */
 int fftgal_batch_rounds(const fftgal_batch_t *b);
 size_t fftgal_batch_pool_bytes(const fftgal_batch_t *b);
 void *fftgal_batch_stream(fftgal_batch_t *b);

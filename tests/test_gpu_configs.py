@@ -17,7 +17,7 @@ of the same fixed-point grids:
   * its rounding is relative to the largest values it combines, so a bin
     whose power lies orders of magnitude below the peak (the high-k bins of
     folded grids) carries up to about 1e-8 P_peak: EPS_PEAK = 1e-3.
-The paint's share is below these: unbiased dithered rounding at 2^-19 to
+The paint's share is below these: unbiased dithered rounding at 2^-18 to
 2^-25 particle, positions packed to 2^-19 cell within their tile.
 The FP64 drop-in path is held to 1e-10 relative everywhere.
 
