@@ -892,26 +892,36 @@ __device__ __forceinline__ void dither_pair(uint64_t pk, uint32_t ext, uint32_t 
 constexpr int brick_n(int side) { return (side + 3) / 4; }
 constexpr int brick_words(int side) { return brick_n(side) * brick_n(side) * brick_n(side) * 64; }
 
-// Flush the planes k < NZ of a bricked LDS footprint of SIDE^2 rows along k,
+// k lanes per row of a footprint flush: whole aligned 128-byte rows of 16
+// int64 cells for tiles of 12 and 16, halves and quarters for smaller ones
+constexpr int flush_klanes(int side) { return side >= 15 ? 16 : side >= 8 ? 8 : 4; }
+
+// Flush the planes k < nz of a bricked LDS footprint of SIDE^2 rows along k,
 // whose cell (0,0,0) sits at grid cell (oi, oj, ok), into the int64 grid
 // with one integer atomic per non-zero cell.  (Plain stores of the cells no
 // other tile reaches were measured twice as slow: they share cache lines
-// with the atomics.)  Each wave takes groups of 64/NZ whole rows (runs of NZ
-// consecutive grid cells); a lane's k offset is fixed, so the periodic wrap
-// needs no division.  With NZ = T = 16 and ok a multiple of 16 every row is
-// one aligned 128-byte line; a row of a whole footprint spans three lines,
-// and the memory-side atomic unit charges per line as well as per atomic.
-template <int SIDE, int NZ>
-__device__ void flush_footprint(const int *__restrict__ fp, int oi, int oj, int ok, int Ng,
-                                int shift, unsigned long long *__restrict__ G) {
-    constexpr int RPG = 64 / NZ;               // rows per wave step
+// with the atomics.)  A wave step covers RPG = 64 / KL rows (fj) of one i
+// plane (fi) and KL consecutive k cells, so with KL = T = 16 and ok a
+// multiple of 16 every row is one aligned 128-byte line (the memory-side
+// atomic unit charges per line as well as per atomic); planes past the
+// first KL take a second step.  The plane and row group of a wave step are
+// wave-uniform (scalar), and since RPG is a multiple of 4 the rows of a
+// group start on a brick boundary: a lane's LDS word is a scalar base plus
+// a per-lane constant, and only the j and k wraps are per lane.  Sums are
+// non-negative (every term is), so the shift to the common unit is a
+// zero-extended one.
+template <int SIDE>
+__device__ void flush_planes(const int *__restrict__ fp, int nz, int oi, int oj, int ok, int Ng,
+                             int shift, unsigned long long *__restrict__ G) {
+    constexpr int KL = flush_klanes(SIDE), RPG = 64 / KL;
+    static_assert(RPG % 4 == 0, "row groups start on brick boundaries");
     constexpr int NBK = brick_n(SIDE);
+    constexpr int NJG = (SIDE + RPG - 1) / RPG;   // row groups per i plane
     typedef __attribute__((address_space(1))) unsigned long long gu64;
-    gu64 *g = (gu64 *)G;
     const int lane = __lane_id();
-    const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    const int ro = lane / NZ, fk = lane - ro * NZ;
-    if (ro >= RPG) return;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwv = blockDim.x >> 6;
+    const int ro = lane / KL, kl = lane - ro * KL;
     // a footprint ends at most SIDE cells past the grid's last index; grids
     // smaller than that wrap more than once
     auto wrap = [Ng](int v) {
@@ -919,18 +929,29 @@ __device__ void flush_footprint(const int *__restrict__ fp, int oi, int oj, int 
         if ((unsigned)v >= (unsigned)Ng) v = ((v % Ng) + Ng) % Ng;
         return v;
     };
-    const int gk = wrap(ok + fk);
-    const int wk = (fk >> 2) * 64 + (fk & 3);
-    for (int r = wv * RPG + ro; r < SIDE * SIDE; r += nwv * RPG) {
-        const int fi = r / SIDE, fj = r - fi * SIDE;
-        const int v = fp[((fi >> 2) * NBK + (fj >> 2)) * NBK * 64 + (fi & 3) * 16 + (fj & 3) * 4 + wk];
-        if (!v) continue;
-        const int gi = wrap(oi + fi), gj = wrap(oj + fj);
-        // Ng < 2^16: the row index fits 32 bits, one 32 x 32 -> 64 multiply-add
-        const uint32_t row = (uint32_t)gi * (uint32_t)Ng + (uint32_t)gj;
-        __hip_atomic_fetch_add(g + ((uint64_t)row * (uint32_t)Ng + (uint32_t)gk),
-                               (unsigned long long)((long long)v << shift), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+    const bool one_wrap = Ng >= SIDE;
+    const int lane_row = (ro >> 2) * NBK * 64 + (ro & 3) * 4;
+    for (int kb = 0; kb < nz; kb += KL) {
+        const int fk = kb + kl;
+        const bool kin = fk < nz;
+        const int gk = one_wrap ? (ok + fk >= Ng ? ok + fk - Ng : ok + fk) : wrap(ok + fk);
+        const int *fpl = fp + lane_row + (fk >> 2) * 64 + (fk & 3);
+        gu64 *gcol = (gu64 *)G + gk;
+        for (int p = wv; p < SIDE * NJG; p += nwv) {
+            const int pu = __builtin_amdgcn_readfirstlane(p);   // wave-uniform
+            const int fi = pu / NJG, s = pu - fi * NJG;
+            const int fj = s * RPG + ro;
+            if (!kin || fj >= SIDE) continue;
+            const int v = fpl[((fi >> 2) * NBK + s * (RPG / 4)) * NBK * 64 + (fi & 3) * 16];
+            if (!v) continue;
+            const int gi = wrap(oi + fi);
+            const int gj = one_wrap ? (oj + fj >= Ng ? oj + fj - Ng : oj + fj) : wrap(oj + fj);
+            // Ng < 2^16: the row index fits 32 bits
+            const uint32_t row = (uint32_t)gi * (uint32_t)Ng + (uint32_t)gj;
+            __hip_atomic_fetch_add(gcol + (uint64_t)row * (uint32_t)Ng,
+                                   (unsigned long long)(uint32_t)v << shift, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
@@ -1170,14 +1191,14 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint8_t *__restrict__ s
             __syncthreads();
             const int ok = ok0 + q * T;
             if (!CARRY || q == nz - 1) {
-                flush_footprint<SA, SA>(fa, oi, oj, ok, g.Ng, FIX_SMAX - S, GA);
-                if (interlace) flush_footprint<SB, SB>(fb, oi, oj, ok, g.Ng, FIX_SMAX - S, GB);
+                flush_planes<SA>(fa, SA, oi, oj, ok, g.Ng, FIX_SMAX - S, GA);
+                if (interlace) flush_planes<SB>(fb, SB, oi, oj, ok, g.Ng, FIX_SMAX - S, GB);
                 if (q == nz - 1) break;
                 __syncthreads();
                 for (int i = threadIdx.x; i < nl; i += blockDim.x) lds[i] = 0;
             } else {
-                flush_footprint<SA, T>(fa, oi, oj, ok, g.Ng, FIX_SMAX - S, GA);
-                if (interlace) flush_footprint<SB, T>(fb, oi, oj, ok, g.Ng, FIX_SMAX - S, GB);
+                flush_planes<SA>(fa, T, oi, oj, ok, g.Ng, FIX_SMAX - S, GA);
+                if (interlace) flush_planes<SB>(fb, T, oi, oj, ok, g.Ng, FIX_SMAX - S, GB);
                 __syncthreads();
                 if constexpr (CARRY) {
                     carry_down<SA, T>(fa);
@@ -1200,8 +1221,8 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint8_t *__restrict__ s
     __syncthreads();
     int oi, oj, ok;
     tile_origin(t, oi, oj, ok);
-    flush_footprint<SA, SA>(fa, oi, oj, ok, g.Ng, FIX_SMAX - S, GA);
-    if (interlace) flush_footprint<SB, SB>(fb, oi, oj, ok, g.Ng, FIX_SMAX - S, GB);
+    flush_planes<SA>(fa, SA, oi, oj, ok, g.Ng, FIX_SMAX - S, GA);
+    if (interlace) flush_planes<SB>(fb, SB, oi, oj, ok, g.Ng, FIX_SMAX - S, GB);
 }
 
 // ------------------------------------------------------------------ host
