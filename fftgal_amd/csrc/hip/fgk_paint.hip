@@ -186,6 +186,7 @@ struct KeyGeo {
     int nc0, nc1, nc2;   // coarse buckets per axis
     int lg0, lg1, lg2;   // log2 of cg0, cg1, cg2
     uint64_t Tmul;       // ceil(2^32 / T): i / T == (i * Tmul) >> 32 for 0 <= i < 2^24
+    int Tshift;          // log2 T when T is a power of two, else -1
 };
 
 struct fgk_paintplan {
@@ -253,11 +254,15 @@ __device__ __forceinline__ void axis_key(float xf, const KeyGeo &g, int &tc, uin
     const double fl = floor(u);
     const double d = u - fl;
     int gi = (int)fl;
+    // positions in [0, L) give gi in [-2, Ng): one conditional add wraps the
+    // first cells' stencil bases; other positions (folded grids, catalogs
+    // outside the box) take the remainder
+    gi = gi < 0 && gi >= -g.Ng ? gi + g.Ng : gi;
     if ((unsigned)gi >= (unsigned)g.Ng) {
         gi %= g.Ng;
         if (gi < 0) gi += g.Ng;
     }
-    tc = (int)(((uint64_t)(uint32_t)gi * g.Tmul) >> 32);
+    tc = g.Tshift >= 0 ? gi >> g.Tshift : (int)(((uint64_t)(uint32_t)gi * g.Tmul) >> 32);
     const int li = gi - tc * g.T;
     int qq = (li << g.F) + (int)rint(d * (double)(1 << g.F));
     const int qmax = (g.T << g.F) - 1;
@@ -1294,7 +1299,8 @@ static KeyGeo tile_geometry(int Ng, int T, int allow_wide, int &mode) {
     KeyGeo g = {0.0,   Ng,    T,     nt,    F,
                 cg[0], cg[1], cg[2], nc[0], nc[1],
                 nc[2], ceil_log2(cg[0]), ceil_log2(cg[1]), ceil_log2(cg[2]),
-                ((1ull << 32) + (uint64_t)T - 1) / (uint64_t)T};
+                ((1ull << 32) + (uint64_t)T - 1) / (uint64_t)T,
+                (T & (T - 1)) == 0 ? ceil_log2(T) : -1};
     return g;
 }
 
