@@ -76,6 +76,20 @@ int  fgk_fft_rows_f32_from_fixed(fgk_fftplan *p, float *f, int64_t *src,
                                  double in_scale, int clear_src, int nbatch, fgk_stream_t s);
 int  fgk_fft_cols_f32(fgk_fftplan *p, float *f, int axis, int sign, int nbatch,
                       fgk_stream_t s);
+/* One launch per FP32 pass over groups of grids of different sides
+ * (batches): group g = nbatch[g] grids of side plans[g]->Ng one after the
+ * other, spectra at f[g], int64 grids at src0[g] (grid set 0) and src1[g]
+ * (set 1, or src1 NULL).  *m stays NULL (return 0) when a group's side has
+ * no multi-size body (sides 32..512 of the specialised kernels, no
+ * four-step): the caller then launches per group.  The row pass reads the
+ * int64 grids of `set` and zeroes them when clear_src. */
+typedef struct fgk_fftmulti fgk_fftmulti;
+int  fgk_fft_multi_create(fgk_fftmulti **m, int ngroups, fgk_fftplan *const *plans,
+                          float *const *f, int64_t *const *src0, int64_t *const *src1,
+                          const int *nbatch);
+void fgk_fft_multi_destroy(fgk_fftmulti *m);
+int  fgk_fft_multi_rows(fgk_fftmulti *m, int set, int clear_src, fgk_stream_t s);
+int  fgk_fft_multi_cols(fgk_fftmulti *m, int axis, int sign, fgk_stream_t s);
 /* 1 if the FP32 passes of this Ng run compile-time specialised kernels
  * (otherwise the generic LDS Stockham kernels) */
 int  fgk_fft_f32_specialised(int Ng);
@@ -157,6 +171,9 @@ void fgk_paint_batch_destroy(fgk_paintbatch *pb);
 int  fgk_paint_batch_set(fgk_paintbatch *pb, const float *const *x, const float *const *y,
                          const float *const *z, const long *Np, fgk_stream_t s);
 int  fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t s);
+/* the paint stage of catalogs with tile sides 8, 12 and 16 runs as one
+ * launch (default), or one launch per tile side (on) */
+void fgk_paint_batch_per_side(fgk_paintbatch *pb, int on);
 /* grid set the paint stage writes (1: gridA2/gridB2) and the memory its
  * launches clear meanwhile (NULL: none; bytes a multiple of 16) */
 int  fgk_paint_batch_select(fgk_paintbatch *pb, int set, void *zero, size_t bytes);
@@ -249,12 +266,27 @@ typedef struct fgk_binjob {
     int NK;
     double dKinv;
     double norm, Vinv, Pshot, npart;
+    /* fgk_fused_bin_f32_multi only: the catalog's grid side, four-step split
+     * of its spectra, its side's offset-phase and deconvolution tables, and
+     * its first workgroup in the bin and plane launches */
+    int Ng, fs_n1;
+    const double *phase, *winv;
+    unsigned blk0, pblk0;
 } fgk_binjob;
 /* one launch pair for njobs catalogs (table in device memory) whose grids
  * share b->Ng, Nmu, los and do_pl; NK_max = largest NK of the catalogs */
 int  fgk_fused_bin_f32_batch(const fgk_binspec *b, const fgk_binjob *jobs_dev, int njobs,
                              int NK_max, const double *phase_dev, const double *winv_dev,
                              fgk_stream_t s);
+/* workgroups of one catalog of side Ng in the launches of
+ * fgk_fused_bin_f32_multi (bin block shape b->bk) */
+void fgk_bin_multi_blocks(const fgk_binspec *b, int Ng, unsigned *blocks, unsigned *pblocks);
+/* one launch pair for njobs catalogs of any grid sides (b->Ng unused; each
+ * job carries Ng, fs_n1, phase, winv, blk0, pblk0 with blocks laid out by
+ * fgk_bin_multi_blocks in job order), sharing Nmu, los, do_pl and the bin
+ * block shape; nblocks / npblocks: the launches' workgroups */
+int  fgk_fused_bin_f32_multi(const fgk_binspec *b, const fgk_binjob *jobs_dev, int njobs,
+                             int NK_max, unsigned nblocks, unsigned npblocks, fgk_stream_t s);
 
 #ifdef __cplusplus
 }

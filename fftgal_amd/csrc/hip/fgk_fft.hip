@@ -416,10 +416,13 @@ __device__ __forceinline__ cpx<float> twmul(cpx<float> v, cpx<float> w, int sign
 // stage S (span NS) of a length-L transform; TPP threads per pencil, q = this
 // thread's index within its pencil.  in(t): stage input (global in stage 0,
 // LDS after), ld/st: LDS, gout: global output of the last stage when GOUT.
+// act: this thread takes part (a multi-size launch runs more threads than
+// the pencils of a small transform need; the others only meet the barriers).
 template <int L, int TPP, int S, int NS, bool GOUT, class In, class Ld, class St, class GOut>
 __device__ __forceinline__ void spencil(int q, In in, Ld ld, St st, GOut gout,
                                         const cpx<float> *__restrict__ stw,
-                                        const cpx<float> *__restrict__ tw, int tws, int sign) {
+                                        const cpx<float> *__restrict__ tw, int tws, int sign,
+                                        bool act = true) {
     constexpr RSeq seq = radix_seq(L);
     constexpr int R = seq.r[S];
     constexpr bool last = S + 1 == seq.n;
@@ -429,7 +432,7 @@ __device__ __forceinline__ void spencil(int q, In in, Ld ld, St st, GOut gout,
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = q + b * TPP;
-        if (NB % TPP == 0 || j < NB) {
+        if (act && (NB % TPP == 0 || j < NB)) {
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b][r] = S == 0 ? in(j + r * NB) : ld(j + r * NB);
         }
@@ -437,7 +440,7 @@ __device__ __forceinline__ void spencil(int q, In in, Ld ld, St st, GOut gout,
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = q + b * TPP;
-        if (NB % TPP == 0 || j < NB) {
+        if (act && (NB % TPP == 0 || j < NB)) {
             if (NS > 1) {
                 const int k = j % NS;
                 const cpx<float> *w = stw + stw_offset(L, S) + k * (R - 1);
@@ -451,7 +454,7 @@ __device__ __forceinline__ void spencil(int q, In in, Ld ld, St st, GOut gout,
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             const int j = q + b * TPP;
-            if (NB % TPP == 0 || j < NB) {
+            if (act && (NB % TPP == 0 || j < NB)) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) gout(j + r * NB, v[b][r]);
             }
@@ -462,7 +465,7 @@ __device__ __forceinline__ void spencil(int q, In in, Ld ld, St st, GOut gout,
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = q + b * TPP;
-        if (NB % TPP == 0 || j < NB) {
+        if (act && (NB % TPP == 0 || j < NB)) {
             const int k = j % NS;
             const int base = (j - k) * R + k;
 #pragma unroll
@@ -471,25 +474,27 @@ __device__ __forceinline__ void spencil(int q, In in, Ld ld, St st, GOut gout,
     }
     __syncthreads();
     if constexpr (!last)
-        spencil<L, TPP, S + 1, NS * R, GOUT>(q, in, ld, st, gout, stw, tw, tws, sign);
+        spencil<L, TPP, S + 1, NS * R, GOUT>(q, in, ld, st, gout, stw, tw, tws, sign, act);
 }
 
 // column pass: C consecutive columns of one row/plane per workgroup, the
 // pencil index t at stride stride_t; LDS row stride C+1 complex.  With tw4,
 // output t of the pencils of outer group o is multiplied by tw4[o t] (the
 // twiddle between the two passes of a four-step transform).
+// (the body of k_cols32 and of the multi-size column pass: lb = the
+// workgroup's logical block, buf = its L (C + 1) complex of LDS)
 template <int L, int C, int TPP>
-__global__ void __launch_bounds__(C * TPP)
-k_cols32(cpx<float> *g, int ncols, long stride_t, long stride_o, const cpx<float> *stw,
-         const cpx<float> *tw, int tws, int sign, const cpx<float> *tw4) {
+__device__ __forceinline__ void cols32_body(unsigned lb, cpx<float> *buf, cpx<float> *g,
+                                            int ncols, long stride_t, long stride_o,
+                                            const cpx<float> *stw, const cpx<float> *tw, int tws,
+                                            int sign, const cpx<float> *tw4) {
     constexpr int LS = C + 1;
-    __shared__ cpx<float> buf[L * LS];
     const int c = threadIdx.x % C, q = threadIdx.x / C;
+    const bool act = q < TPP;
     const int ncb = (ncols + C - 1) / C;
-    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
     const int o = (int)(lb / (unsigned)ncb);
     const int c0 = (int)(lb - (unsigned)o * ncb) * C;
-    const bool live = c0 + c < ncols;
+    const bool live = act && c0 + c < ncols;
     cpx<float> *base = g + (long)o * stride_o + c0 + c;
     auto gin = [&](int t) -> cpx<float> {
         return live ? base[(long)t * stride_t] : cmk(0.0f, 0.0f);
@@ -500,7 +505,16 @@ k_cols32(cpx<float> *g, int ncols, long stride_t, long stride_o, const cpx<float
     };
     auto ld = [&](int t) -> cpx<float> { return buf[t * LS + c]; };
     auto st = [&](int t, cpx<float> v) { buf[t * LS + c] = v; };
-    spencil<L, TPP, 0, 1, true>(q, gin, ld, st, gout, stw, tw, tws, sign);
+    spencil<L, TPP, 0, 1, true>(q, gin, ld, st, gout, stw, tw, tws, sign, act);
+}
+
+template <int L, int C, int TPP>
+__global__ void __launch_bounds__(C * TPP)
+k_cols32(cpx<float> *g, int ncols, long stride_t, long stride_o, const cpx<float> *stw,
+         const cpx<float> *tw, int tws, int sign, const cpx<float> *tw4) {
+    __shared__ cpx<float> buf[L * (C + 1)];
+    cols32_body<L, C, TPP>(xcd_block(blockIdx.x, gridDim.x), buf, g, ncols, stride_t, stride_o,
+                           stw, tw, tws, sign, tw4);
 }
 
 // row pass: P rows of 2M int64 fixed-point values per workgroup, complex
@@ -509,16 +523,19 @@ k_cols32(cpx<float> *g, int ncols, long stride_t, long stride_o, const cpx<float
 // A = (Z[k] + conj Z[M-k]) / 2, B = (Z[k] - conj Z[M-k]) / 2i, W = e^{-2 pi i/2M},
 // written to the packed layout (column 0 = (X[0], X[M])).  LDS holds each row
 // with one pad slot per 8 elements.
+// (the body of k_rows32 and of the multi-size row pass: lb = the
+// workgroup's logical block, buf = its P (M + M/8 + 1) complex of LDS)
 template <int M, int P, int TPP>
-__global__ void __launch_bounds__(P * TPP)
-k_rows32(cpx<float> *out, int64_t *src, double in_scale, int clear_src, long nrows,
-         const cpx<float> *stw, const cpx<float> *tw) {
+__device__ __forceinline__ void rows32_body(unsigned lb, cpx<float> *buf, cpx<float> *out,
+                                            int64_t *src, double in_scale, int clear_src,
+                                            long nrows, const cpx<float> *stw,
+                                            const cpx<float> *tw) {
     constexpr int RS = M + M / 8 + 1;
-    __shared__ cpx<float> buf[P * RS];
     const int q = threadIdx.x % TPP, p = threadIdx.x / TPP;
-    const long row = (long)xcd_block(blockIdx.x, gridDim.x) * P + p;
-    const bool live = row < nrows;
-    cpx<float> *mine = buf + p * RS;
+    const bool act = p < P;
+    const long row = (long)lb * P + p;
+    const bool live = act && row < nrows;
+    cpx<float> *mine = buf + (act ? p : 0) * RS;
     longlong2 *srow = (longlong2 *)(src + row * (2L * M));
     auto gin = [&](int t) -> cpx<float> {
         if (!live) return cmk(0.0f, 0.0f);
@@ -529,7 +546,7 @@ k_rows32(cpx<float> *out, int64_t *src, double in_scale, int clear_src, long nro
     auto ld = [&](int t) -> cpx<float> { return mine[t + (t >> 3)]; };
     auto st = [&](int t, cpx<float> v) { mine[t + (t >> 3)] = v; };
     auto none = [&](int, cpx<float>) {};
-    spencil<M, TPP, 0, 1, false>(q, gin, ld, st, none, stw, tw, 2, -1);
+    spencil<M, TPP, 0, 1, false>(q, gin, ld, st, none, stw, tw, 2, -1, act);
     if (!live) return;
     cpx<float> *orow = out + row * (long)M;
     for (int k = q; k <= M / 2; k += TPP) {
@@ -548,6 +565,15 @@ k_rows32(cpx<float> *out, int64_t *src, double in_scale, int clear_src, long nro
         const cpx<float> Dm = csub(A, Tk);
         orow[M - k] = cmk(Dm.x, -Dm.y);
     }
+}
+
+template <int M, int P, int TPP>
+__global__ void __launch_bounds__(P * TPP)
+k_rows32(cpx<float> *out, int64_t *src, double in_scale, int clear_src, long nrows,
+         const cpx<float> *stw, const cpx<float> *tw) {
+    __shared__ cpx<float> buf[P * (M + M / 8 + 1)];
+    rows32_body<M, P, TPP>(xcd_block(blockIdx.x, gridDim.x), buf, out, src, in_scale, clear_src,
+                           nrows, stw, tw);
 }
 
 // launch shapes: columns per workgroup C (<= 16) and rows per workgroup P
@@ -656,6 +682,107 @@ static int launch_rows32(int N, long nrows, cpx<float> *out, int64_t *src, doubl
     FGK_LAUNCHED();
     return 0;
 }
+
+// ---------------------------------------------- multi-size passes (batches)
+//
+// One launch per FFT pass over groups of grids of different sides: group g
+// holds nbatch grids of side N one after the other.  Each workgroup finds
+// its group by the groups' first logical blocks (binary search, scalar
+// loads) and runs the compile-time specialised body of that side; the
+// launch has 1024 threads and the largest LDS of the sides it serves, and
+// threads a side does not need only meet the barriers.
+#define FGK_MULTI_NG(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+
+// columns per workgroup of a side in the multi-size column pass: the most
+// (a power of two) that 1024 threads and 80 KB of LDS hold, so that small
+// sides fill the launch's workgroups too
+constexpr int mcols_C(int L) {
+    int C = 1;
+    while (2 * C * cols_TPP(L) <= 1024 && L * (2 * C + 1) * 8 <= 80 * 1024 && 2 * C <= 64) C *= 2;
+    return C;
+}
+constexpr int multi_rows_lds(int N) {
+    return rows_P(N / 2) * (N / 2 + N / 16 + 1) * 8;
+}
+constexpr int multi_cols_lds(int N) { return N * (mcols_C(N) + 1) * 8; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+#define MAX_ROWS_LDS(n) cmax(multi_rows_lds(n),
+#define MAX_COLS_LDS(n) cmax(multi_cols_lds(n),
+constexpr int MULTI_ROWS_LDS =
+    FGK_MULTI_NG(MAX_ROWS_LDS) 0))))))))) ;
+constexpr int MULTI_COLS_LDS =
+    FGK_MULTI_NG(MAX_COLS_LDS) 0))))))))) ;
+#undef MAX_ROWS_LDS
+#undef MAX_COLS_LDS
+static_assert(MULTI_ROWS_LDS <= 80 * 1024 && MULTI_COLS_LDS <= 80 * 1024, "multi LDS");
+
+struct MultiGroup {
+    cpx<float> *f;               // spectra of the group's grids
+    int64_t *src[2];             // int64 grids, grid set 0 and 1 (row pass)
+    int N, nbatch;
+    unsigned rblk0, cblk0[2];    // first logical block: row pass, column pass axis 0 / 1
+    const cpx<float> *stw_rows, *stw_cols, *tw;
+    int ncols[2];
+    long stride_t[2], stride_o[2];
+};
+
+__device__ __forceinline__ int multi_group(const MultiGroup *__restrict__ G, int ng, unsigned lb,
+                                           int which) {
+    int lo = 0, hi = ng - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const unsigned b0 = which < 0 ? G[mid].rblk0 : G[mid].cblk0[which];
+        if (b0 <= lb) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(1024)
+k_rows32_multi(const MultiGroup *__restrict__ G, int ng, int set, int clear_src) {
+    __shared__ cpx<float> buf[MULTI_ROWS_LDS / 8];
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int gi = multi_group(G, ng, lb, -1);
+    const MultiGroup &g = G[gi];
+    const long nrows = (long)g.N * g.N * g.nbatch;
+    switch (g.N) {
+#define CASE_NG(n)                                                                            \
+    case n:                                                                                   \
+        rows32_body<n / 2, rows_P(n / 2), rows_TPP(n / 2)>(lb - g.rblk0, buf, g.f, g.src[set],  \
+                                                           1.0, clear_src, nrows, g.stw_rows, \
+                                                           g.tw);                             \
+        break;
+        FGK_MULTI_NG(CASE_NG)
+#undef CASE_NG
+    default:
+        break;
+    }
+}
+
+__global__ void __launch_bounds__(1024)
+k_cols32_multi(const MultiGroup *__restrict__ G, int ng, int axis, int sign) {
+    __shared__ cpx<float> buf[MULTI_COLS_LDS / 8];
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int gi = multi_group(G, ng, lb, axis);
+    const MultiGroup &g = G[gi];
+    switch (g.N) {
+#define CASE_NG(n)                                                                            \
+    case n:                                                                                   \
+        cols32_body<n, mcols_C(n), cols_TPP(n)>(lb - g.cblk0[axis], buf, g.f, g.ncols[axis],    \
+                                               g.stride_t[axis], g.stride_o[axis], g.stw_cols,\
+                                               g.tw, 1, sign, nullptr);                       \
+        break;
+        FGK_MULTI_NG(CASE_NG)
+#undef CASE_NG
+    default:
+        break;
+    }
+}
+
+struct fgk_fftmulti {
+    int ng;
+    MultiGroup *d;               // device table
+    unsigned rblocks, cblocks[2];
+};
 
 // concatenated per-stage twiddles of a length-L transform (FP64-built):
 // stage s, k < Ns_s, r = 1..R_s-1 -> exp(-2 pi i r k / (Ns_s R_s))
@@ -881,6 +1008,123 @@ extern "C" int fgk_fft_cols_f32(fgk_fftplan *p, float *f, int axis, int sign, in
     return launch_cols32(N, (cpx<float> *)f, cs.ncols, cs.stride_t, cs.stride_o, cs.nouter,
                          (const cpx<float> *)p->stw_cols, (const cpx<float> *)p->tw, sign,
                          (hipStream_t)s);
+}
+
+static bool multi_side(int N) {
+    switch (N) {
+#define CASE_NG(n) case n:
+        FGK_MULTI_NG(CASE_NG)
+#undef CASE_NG
+        return true;
+    default:
+        return false;
+    }
+}
+
+static int multi_rows_P(int N) {
+    switch (N) {
+#define CASE_NG(n) case n: return rows_P(n / 2);
+        FGK_MULTI_NG(CASE_NG)
+#undef CASE_NG
+    default:
+        return 1;
+    }
+}
+
+static int multi_cols_C(int N) {
+    switch (N) {
+#define CASE_NG(n) case n: return mcols_C(n);
+        FGK_MULTI_NG(CASE_NG)
+#undef CASE_NG
+    default:
+        return 1;
+    }
+}
+
+extern "C" int fgk_fft_multi_create(fgk_fftmulti **out, int ngroups, fgk_fftplan *const *plans,
+                                    float *const *f, int64_t *const *src0,
+                                    int64_t *const *src1, const int *nbatch) {
+    *out = nullptr;
+    if (ngroups < 1) return 0;
+    for (int g = 0; g < ngroups; ++g) {
+        const fgk_fftplan *p = plans[g];
+        if (p->prec != 4 || !p->spec || p->fs_n1 || !multi_side(p->Ng) || nbatch[g] < 1)
+            return 0;   // a side without a multi-size body: the caller launches per group
+    }
+    MultiGroup *h = (MultiGroup *)calloc((size_t)ngroups, sizeof(MultiGroup));
+    fgk_fftmulti *m = (fgk_fftmulti *)calloc(1, sizeof(fgk_fftmulti));
+    if (!h || !m) {
+        free(h);
+        free(m);
+        fgk_set_error("out of host memory");
+        return -1;
+    }
+    unsigned long rb = 0, cb[2] = {0, 0};
+    for (int g = 0; g < ngroups; ++g) {
+        const fgk_fftplan *p = plans[g];
+        const int N = p->Ng, M = N / 2;
+        MultiGroup &G = h[g];
+        G.f = (cpx<float> *)f[g];
+        G.src[0] = src0[g];
+        G.src[1] = src1 ? src1[g] : nullptr;
+        G.N = N;
+        G.nbatch = nbatch[g];
+        G.stw_rows = (const cpx<float> *)p->stw_rows;
+        G.stw_cols = (const cpx<float> *)p->stw_cols;
+        G.tw = (const cpx<float> *)p->tw;
+        G.rblk0 = (unsigned)rb;
+        const int P = multi_rows_P(N), C = multi_cols_C(N);
+        rb += ((unsigned long)N * N * nbatch[g] + P - 1) / P;
+        for (int axis = 0; axis < 2; ++axis) {
+            const ColShape cs = col_shape(N, M, axis, nbatch[g]);
+            G.ncols[axis] = cs.ncols;
+            G.stride_t[axis] = cs.stride_t;
+            G.stride_o[axis] = cs.stride_o;
+            G.cblk0[axis] = (unsigned)cb[axis];
+            cb[axis] += (unsigned long)cs.nouter * ((cs.ncols + C - 1) / C);
+        }
+    }
+    if (rb >= (1ul << 31) || cb[0] >= (1ul << 31) || cb[1] >= (1ul << 31)) {
+        free(h);
+        free(m);
+        fgk_set_error("FFT multi: too many workgroups for one launch");
+        return -2;
+    }
+    m->ng = ngroups;
+    m->rblocks = (unsigned)rb;
+    m->cblocks[0] = (unsigned)cb[0];
+    m->cblocks[1] = (unsigned)cb[1];
+    int rc = fgk_malloc((void **)&m->d, (size_t)ngroups * sizeof(MultiGroup));
+    if (!rc) rc = fgk_h2d(m->d, h, (size_t)ngroups * sizeof(MultiGroup), nullptr);
+    if (!rc) rc = fgk_stream_sync(nullptr);
+    free(h);
+    if (rc) {
+        fgk_free(m->d);
+        free(m);
+        return rc;
+    }
+    *out = m;
+    return 0;
+}
+
+extern "C" void fgk_fft_multi_destroy(fgk_fftmulti *m) {
+    if (!m) return;
+    fgk_free(m->d);
+    free(m);
+}
+
+extern "C" int fgk_fft_multi_rows(fgk_fftmulti *m, int set, int clear_src, fgk_stream_t s) {
+    hipLaunchKernelGGL(k_rows32_multi, dim3(m->rblocks), dim3(1024), 0, (hipStream_t)s, m->d,
+                       m->ng, set, clear_src);
+    FGK_LAUNCHED();
+    return 0;
+}
+
+extern "C" int fgk_fft_multi_cols(fgk_fftmulti *m, int axis, int sign, fgk_stream_t s) {
+    hipLaunchKernelGGL(k_cols32_multi, dim3(m->cblocks[axis]), dim3(1024), 0, (hipStream_t)s,
+                       m->d, m->ng, axis, sign);
+    FGK_LAUNCHED();
+    return 0;
 }
 
 extern "C" int fgk_fft_r2c_f64(fgk_fftplan *p, double *f, double scale, fgk_stream_t st) {

@@ -281,6 +281,7 @@ extern "C" size_t fgk_bins_counts(const fgk_binspec *b) {
 }
 
 struct BinArgs {
+    int multi;        // > 0: one launch over this many catalogs of any sides (jobs)
     int Ng;
     int NK;
     int Nmu;
@@ -696,18 +697,39 @@ __device__ __forceinline__ BinOut make_out(const BinArgs &a, double *sh, int blo
 // A batch of catalogs on grids of one side (jobs != nullptr): blockIdx.y
 // selects the catalog, whose spectra, bins, k binning and normalisations
 // replace A, B, bins, a.NK, a.dKinv, norm, Vinv, Pshot and npart.
-#define BIN_JOB(jobs)                      \
-    do {                                   \
-        const fgk_binjob &J = jobs[blockIdx.y]; \
-        A = J.A;                           \
-        B = J.B;                           \
-        bins = J.sums;                     \
-        a.NK = J.NK;                       \
-        a.dKinv = J.dKinv;                 \
-        norm = J.norm;                     \
-        Vinv = J.Vinv;                     \
-        Pshot = J.Pshot;                   \
-        npart = J.npart;                   \
+// With a.multi > 0 (one launch over catalogs of any sides) the catalog
+// owning workgroup blk is found by its first workgroup (blk0, or pblk0 for
+// the plane kernel), blk becomes the workgroup within it, and the job also
+// supplies the grid side, the four-step split and the side's tables.
+#define BIN_JOB(jobs, blk, plane)                                              \
+    do {                                                                       \
+        int jx_ = blockIdx.y;                                                  \
+        if (a.multi) {                                                         \
+            int lo_ = 0, hi_ = a.multi - 1;                                    \
+            while (lo_ < hi_) {                                                \
+                const int mid_ = (lo_ + hi_ + 1) >> 1;                         \
+                const unsigned b0_ = plane ? jobs[mid_].pblk0 : jobs[mid_].blk0; \
+                if (b0_ <= (unsigned)blk) lo_ = mid_; else hi_ = mid_ - 1;     \
+            }                                                                  \
+            jx_ = lo_;                                                         \
+        }                                                                      \
+        const fgk_binjob &J = jobs[jx_];                                       \
+        A = J.A;                                                               \
+        B = J.B;                                                               \
+        bins = J.sums;                                                         \
+        a.NK = J.NK;                                                           \
+        a.dKinv = J.dKinv;                                                     \
+        norm = J.norm;                                                         \
+        Vinv = J.Vinv;                                                         \
+        Pshot = J.Pshot;                                                       \
+        npart = J.npart;                                                       \
+        if (a.multi) {                                                         \
+            blk -= (int)(plane ? J.pblk0 : J.blk0);                            \
+            a.Ng = J.Ng;                                                       \
+            a.fs_n1 = J.fs_n1;                                                 \
+            ph = J.phase;                                                      \
+            winv = J.winv;                                                     \
+        }                                                                      \
     } while (0)
 
 // For BK <= 32 the kernel is held to 96 VGPRs, five waves per SIMD (no
@@ -722,14 +744,14 @@ k_bin(const double *__restrict__ fP, const float *__restrict__ A, const float *_
 #pragma clang fp contract(off)
     extern __shared__ double sh[];
     a.fx = MODE == 1 ? 1 : (MODE == 2 ? 2 : 0);   // as the host sized the LDS; constant here
-    if (MODE == 1 && jobs) BIN_JOB(jobs);
+    int blk = blockIdx.x;
+    if (MODE == 1 && jobs) BIN_JOB(jobs, blk, false);
     constexpr int BI = BinBlock<BKT>::BI, BJ = BinBlock<BKT>::BJ, BK = BinBlock<BKT>::BK;
     const int Ng = a.Ng;
     const int Mh = Ng / 2;
     const int Nk = MODE == 1 ? Mh : Mh + 1;      // k columns walked
     const int nbk = (Nk + BK - 1) / BK;
     const int nbj = (Ng + BJ - 1) / BJ;
-    int blk = blockIdx.x;
     const int bk = blk % nbk;
     blk /= nbk;
     const int bj = blk % nbj;
@@ -852,11 +874,12 @@ k_bin_plane(const float *__restrict__ A, const float *__restrict__ B, BinArgs a,
 #pragma clang fp contract(off)
     extern __shared__ double sh[];
     a.fx = 1;   // as the host sized the LDS; constant here
-    if (jobs) BIN_JOB(jobs);
+    int blk = blockIdx.x;
+    if (jobs) BIN_JOB(jobs, blk, true);
     const int Ng = a.Ng;
     const int Mh = Ng / 2;
     const int nbj = (Ng + PJ - 1) / PJ;
-    const int i0 = (blockIdx.x / nbj) * PI, j0 = (blockIdx.x % nbj) * PJ;
+    const int i0 = (blk / nbj) * PI, j0 = (blk % nbj) * PJ;
     const int i1 = min(i0 + PI, Ng), j1 = min(j0 + PJ, Ng);
     int lo0, hi0, lo1, hi1, blo, bhi, blo2, bhi2;
     axis_range(i0, i1, Ng, lo0, hi0);
@@ -905,6 +928,7 @@ static int bin_bk(const fgk_binspec *b) {
 static BinArgs make_args(const fgk_binspec *b, int fx) {
     const int BK = bin_bk(b);
     BinArgs a;
+    a.multi = 0;
     a.Ng = b->Ng;
     a.NK = b->NK;
     a.Nmu = b->Nmu;
@@ -1049,6 +1073,73 @@ extern "C" int fgk_fused_bin_f32(const float *A, const float *B, const fgk_binsp
     hipLaunchKernelGGL(k_bin_plane, dim3(nplane), dim3(PI * PJ), bin_lds(a), (hipStream_t)s, A, B,
                        a, phase_dev, winv_dev, norm, Vinv, Pshot, npart, (void *)sums_dev,
                        (const fgk_binjob *)nullptr);
+    FGK_LAUNCHED();
+    return 0;
+}
+
+extern "C" void fgk_bin_multi_blocks(const fgk_binspec *b, int Ng, unsigned *blocks,
+                                     unsigned *pblocks) {
+    unsigned nb = 0;
+    switch (bin_bk(b)) {
+    case 16: nb = bin_blocks<16>(Ng, Ng / 2); break;
+    case 32: nb = bin_blocks<32>(Ng, Ng / 2); break;
+    case 128: nb = bin_blocks<128>(Ng, Ng / 2); break;
+    default: nb = bin_blocks<64>(Ng, Ng / 2); break;
+    }
+    if (blocks) *blocks = nb;
+    if (pblocks) *pblocks = (unsigned)(((Ng + PI - 1) / PI) * ((Ng + PJ - 1) / PJ));
+}
+
+extern "C" int fgk_fused_bin_f32_multi(const fgk_binspec *b, const fgk_binjob *jobs_dev,
+                                       int njobs, int NK_max, unsigned nblocks,
+                                       unsigned npblocks, fgk_stream_t s) {
+    if (b->geometry) {
+        fgk_set_error("fgk_fused_bin_f32_multi: geometry is computed by fgk_bin_geometry");
+        return -2;
+    }
+    if (njobs < 1) {
+        fgk_set_error("fgk_fused_bin_f32_multi: no catalogs");
+        return -2;
+    }
+    fgk_binspec bb = *b;
+    bb.NK = NK_max;          // the LDS window size does not depend on NK
+    BinArgs a = make_args(&bb, 1);
+    a.multi = njobs;
+    const int bk = bin_bk(b);
+    switch (bk) {
+    case 16:
+        hipLaunchKernelGGL((k_bin<1, 16>), dim3(nblocks), dim3(BIN_THREADS), bin_lds(a),
+                           (hipStream_t)s, (const double *)nullptr, (const float *)nullptr,
+                           (const float *)nullptr, a, (const double *)nullptr,
+                           (const double *)nullptr, 0.0, 0.0, 0.0, 0.0, (void *)nullptr,
+                           (unsigned long long *)nullptr, jobs_dev);
+        break;
+    case 32:
+        hipLaunchKernelGGL((k_bin<1, 32>), dim3(nblocks), dim3(BIN_THREADS), bin_lds(a),
+                           (hipStream_t)s, (const double *)nullptr, (const float *)nullptr,
+                           (const float *)nullptr, a, (const double *)nullptr,
+                           (const double *)nullptr, 0.0, 0.0, 0.0, 0.0, (void *)nullptr,
+                           (unsigned long long *)nullptr, jobs_dev);
+        break;
+    case 128:
+        hipLaunchKernelGGL((k_bin<1, 128>), dim3(nblocks), dim3(BIN_THREADS), bin_lds(a),
+                           (hipStream_t)s, (const double *)nullptr, (const float *)nullptr,
+                           (const float *)nullptr, a, (const double *)nullptr,
+                           (const double *)nullptr, 0.0, 0.0, 0.0, 0.0, (void *)nullptr,
+                           (unsigned long long *)nullptr, jobs_dev);
+        break;
+    default:
+        hipLaunchKernelGGL((k_bin<1, 64>), dim3(nblocks), dim3(BIN_THREADS), bin_lds(a),
+                           (hipStream_t)s, (const double *)nullptr, (const float *)nullptr,
+                           (const float *)nullptr, a, (const double *)nullptr,
+                           (const double *)nullptr, 0.0, 0.0, 0.0, 0.0, (void *)nullptr,
+                           (unsigned long long *)nullptr, jobs_dev);
+    }
+    FGK_LAUNCHED();
+    hipLaunchKernelGGL(k_bin_plane, dim3(npblocks), dim3(PI * PJ), bin_lds(a), (hipStream_t)s,
+                       (const float *)nullptr, (const float *)nullptr, a,
+                       (const double *)nullptr, (const double *)nullptr, 0.0, 0.0, 0.0, 0.0,
+                       (void *)nullptr, jobs_dev);
     FGK_LAUNCHED();
     return 0;
 }

@@ -1116,15 +1116,17 @@ __device__ __forceinline__ float fix_unit(int S) { return ldexpf(FIX_C0, S - FIX
 // the job owning it (ji0[job] <= work < ji0[job+1], ja <= job < jb) supplies
 // the geometry, grids (set 0: GA/GB, set 1: GA2/GB2) and tile tables
 // (tilebase, items + job * sl, MAX_D2 slots per coarse bucket).
+// (k_paint_tiles<T> runs this body; k_paint_tiles_multi runs the body of
+// each batch catalog's tile side in one launch)
 template <int T>
-__global__ void __launch_bounds__(PAINT_THREADS, 8)
-k_paint_tiles(const uint64_t *__restrict__ sorted, const uint8_t *__restrict__ sorted_ext,
-              const uint32_t *__restrict__ tilebase,
-              const uint32_t *__restrict__ items, long nslots, KeyGeo g, int nb2,
-              int cap, int interlace, unsigned long long *__restrict__ GA,
-              unsigned long long *__restrict__ GB, const PaintJob *__restrict__ jobs,
-              const long *__restrict__ ji0, int ja, int jb, long sl, int set,
-              uint64_t *__restrict__ zbuf, long zwords) {
+__device__ __forceinline__ void
+paint_tiles_body(const uint64_t *__restrict__ sorted, const uint8_t *__restrict__ sorted_ext,
+                 const uint32_t *__restrict__ tilebase,
+                 const uint32_t *__restrict__ items, long nslots, KeyGeo g, int nb2,
+                 int cap, int interlace, unsigned long long *__restrict__ GA,
+                 unsigned long long *__restrict__ GB, const PaintJob *__restrict__ jobs,
+                 const long *__restrict__ ji0, int ja, int jb, long sl, int set,
+                 uint64_t *__restrict__ zbuf, long zwords) {
     constexpr int SA = T + 4, SB = T + 3;
     constexpr bool CARRY = T % 4 == 0;
     if (zbuf) {   // this workgroup's slice of the memory to clear (16-byte stores)
@@ -1228,6 +1230,46 @@ k_paint_tiles(const uint64_t *__restrict__ sorted, const uint8_t *__restrict__ s
     tile_origin(t, oi, oj, ok);
     flush_planes<SA>(fa, SA, oi, oj, ok, g.Ng, FIX_SMAX - S, GA);
     if (interlace) flush_planes<SB>(fb, SB, oi, oj, ok, g.Ng, FIX_SMAX - S, GB);
+}
+
+template <int T>
+__global__ void __launch_bounds__(PAINT_THREADS, 8)
+k_paint_tiles(const uint64_t *__restrict__ sorted, const uint8_t *__restrict__ sorted_ext,
+              const uint32_t *__restrict__ tilebase,
+              const uint32_t *__restrict__ items, long nslots, KeyGeo g, int nb2,
+              int cap, int interlace, unsigned long long *__restrict__ GA,
+              unsigned long long *__restrict__ GB, const PaintJob *__restrict__ jobs,
+              const long *__restrict__ ji0, int ja, int jb, long sl, int set,
+              uint64_t *__restrict__ zbuf, long zwords) {
+    paint_tiles_body<T>(sorted, sorted_ext, tilebase, items, nslots, g, nb2, cap, interlace, GA,
+                        GB, jobs, ji0, ja, jb, sl, set, zbuf, zwords);
+}
+
+// One paint launch for every catalog of a batch whatever its tile side:
+// workgroup b does work b of the batch (ji0[0] = 0), and the side of the
+// catalog owning it (its KeyGeo) selects the compile-time body; the
+// dynamic LDS is that of the largest side in the launch.
+__global__ void __launch_bounds__(PAINT_THREADS, 8)
+k_paint_tiles_multi(const uint64_t *__restrict__ sorted, const uint8_t *__restrict__ sorted_ext,
+                    const uint32_t *__restrict__ tilebase, const uint32_t *__restrict__ items,
+                    int cap, const PaintJob *__restrict__ jobs, const long *__restrict__ ji0,
+                    int njobs, long sl, int set, uint64_t *__restrict__ zbuf, long zwords) {
+    const int job = batch_job(blockIdx.x, ji0, njobs);
+    const KeyGeo g0 = jobs[job].g;
+    switch (g0.T) {
+#define PAINT_MULTI_T(TT)                                                                     \
+    case TT:                                                                                  \
+        paint_tiles_body<TT>(sorted, sorted_ext, tilebase, items, 0l, g0, MAX_D2, cap, 0,     \
+                             (unsigned long long *)nullptr, (unsigned long long *)nullptr,    \
+                             jobs, ji0, 0, njobs, sl, set, zbuf, zwords);                     \
+        break;
+        PAINT_MULTI_T(16)
+        PAINT_MULTI_T(12)
+        PAINT_MULTI_T(8)
+#undef PAINT_MULTI_T
+    default:
+        break;
+    }
 }
 
 // ------------------------------------------------------------------ host
@@ -1696,6 +1738,7 @@ struct fgk_paintbatch {
     uint64_t *zero;                // cleared by the paint stage's launches
     long zero_words;
     int words_external;            // sorted1, sorted2, aux1, ext2 in a caller's workspace
+    int per_side;                  // paint launches per tile side (fgk_paint_batch_per_side)
 };
 
 extern "C" void fgk_paint_batch_destroy(fgk_paintbatch *pb) {
@@ -1952,6 +1995,8 @@ extern "C" int fgk_paint_batch_set(fgk_paintbatch *pb, const float *const *x,
     return 0;
 }
 
+extern "C" void fgk_paint_batch_per_side(fgk_paintbatch *pb, int on) { pb->per_side = on ? 1 : 0; }
+
 extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t st) {
     hipStream_t s = (hipStream_t)st;
     if (!pb->table_ok) {
@@ -2039,8 +2084,26 @@ extern "C" int fgk_paint_batch_stage(fgk_paintbatch *pb, int stage, fgk_stream_t
         FGK_LAUNCHED();
         return 0;
     case 6: {
-        // the region to clear, in even-word pieces proportional to each
-        // launch's work items
+        bool multi = pb->nsides > 1 && !pb->per_side;
+        int smax = 0;
+        for (int i = 0; i < pb->nsides; ++i) {
+            const int t = pb->side[i];
+            if (t != 16 && t != 12 && t != 8) multi = false;
+            if (t > smax) smax = t;
+        }
+        if (multi) {   // one launch for every catalog, whatever its tile side
+            const long nitems = pb->hi0[pb->njobs];
+            if (nitems <= 0) return 0;
+            const size_t lds = (size_t)paint_lds_bytes(smax, pb->interlace_any);
+            hipLaunchKernelGGL(k_paint_tiles_multi, dim3((unsigned)nitems), dim3(PAINT_THREADS),
+                               lds, s, pb->sorted2, pb->ext2, pb->tilebase, pb->items, pb->cap,
+                               (const PaintJob *)pb->djobs, (const long *)pb->di0, pb->njobs,
+                               BATCH_SL, pb->set, pb->zero, pb->zero ? pb->zero_words : 0l);
+            FGK_LAUNCHED();
+            return 0;
+        }
+        // per tile side: the region to clear, in even-word pieces proportional
+        // to each launch's work items
         const long pairs = pb->zero_words / 2, all = pb->hi0[pb->sja[pb->nsides]];
         for (int i = 0; i < pb->nsides; ++i) {
             const int ja = pb->sja[i], jb = pb->sja[i + 1];

@@ -711,12 +711,16 @@ int fftgal_memcpy_d2h(void *dst, const void *src, unsigned long bytes) {
  * (FFTGAL_GRID_SETS=1) the row FFT clears the grids it reads instead.  Both
  * keep the invariant that a round finds its grid set zero.
  * Within a round, one launch per stage:
- *   sort and paint  -- all its catalogs together (fgk_paintbatch; one paint
- *                      launch per tile side),
- *   FFT passes      -- per grid side: the grids of every catalog of that
- *                      side and interlacing lie one after the other, so the
- *                      row pass and both column passes cover them all,
- *   fused bin       -- per grid side, one launch pair with a catalog table.
+ *   sort and paint  -- all its catalogs together (fgk_paintbatch; the paint
+ *                      kernel dispatches on each catalog's tile side),
+ *   FFT passes      -- the grids of every catalog of one side and
+ *                      interlacing lie one after the other (a group); the
+ *                      multi-size row and column kernels cover every group
+ *                      (fgk_fftmulti), each workgroup finding its group,
+ *   fused bin       -- one launch pair with a table of every catalog (grid
+ *                      side, tables and first workgroup per catalog).
+ * (FFTGAL_BATCH_MULTI=0, or a side without a multi-size body: one launch
+ * per tile side, per group and per group respectively.)
  * Then one copy of every catalog's bin sums to pinned host memory.  The
  * whole sequence is one hipGraph; catalog pointers, Np and V live in device
  * tables, so a new set of catalogs updates the tables and replays the same
@@ -750,6 +754,12 @@ typedef struct batch_round {
     batch_group *grp;
     fgk_paintbatch *paint;
     size_t gwords;               /* int64 words of its grids */
+    /* one launch per stage over all its groups (NULL: a launch per group) */
+    fgk_fftmulti *fm;            /* FFT row and column passes */
+    fgk_binjob *hjall, *djall;   /* fused bin: every catalog, blocks laid out in order */
+    fgk_binspec bspec;           /* its shared Nmu, los and bin block shape */
+    int NK_max;
+    unsigned bblocks, bpblocks;
 } batch_round;
 
 typedef struct batch_job {
@@ -798,6 +808,9 @@ int fftgal_batch_destroy(fftgal_batch_t *b) {
             free(R->grp[q].hjob);
             free(R->grp[q].jobs);
         }
+        fgk_fft_multi_destroy(R->fm);
+        fgk_free(R->djall);
+        free(R->hjall);
         free(R->grp);
         free(R->jobs);
     }
@@ -870,6 +883,79 @@ static void job_bytes(const fftgal_job_desc *d, size_t *gbytes, size_t *fwbytes)
 }
 
 static size_t pool_size(size_t gmax, size_t fwmax, int sets) { return sets * gmax + fwmax; }
+
+/* one launch per stage over the groups of a round (FFT passes and the
+ * fused bin), unless FFTGAL_BATCH_MULTI=0 */
+static int want_multi(void) {
+    const char *e = getenv("FFTGAL_BATCH_MULTI");
+    return !(e && atoi(e) == 0);
+}
+
+/* the one-launch tables of round R (groups of several grid sides): the
+ * FFT passes (when every side has a multi-size body) and the fused bin over
+ * every catalog, which shares Nmu, los and one bin block shape (the largest
+ * grid's; the exact fixed-point sums do not depend on it) */
+static int round_multi(fftgal_batch_t *b, batch_round *R) {
+    fgk_fftplan **plans = (fgk_fftplan **)calloc((size_t)R->ngroups, sizeof(fgk_fftplan *));
+    float **f = (float **)calloc((size_t)R->ngroups, sizeof(float *));
+    int64_t **s0 = (int64_t **)calloc((size_t)R->ngroups, sizeof(int64_t *));
+    int64_t **s1 = (int64_t **)calloc((size_t)R->ngroups, sizeof(int64_t *));
+    int *nb = (int *)calloc((size_t)R->ngroups, sizeof(int));
+    int rc = (!plans || !f || !s0 || !s1 || !nb) ? -1 : 0;
+    if (rc) fgk_set_error("out of host memory");
+    int big = 0;
+    for (int q = 0; q < R->ngroups && !rc; ++q) {
+        const batch_group *G = &R->grp[q];
+        plans[q] = b->side[G->side].plan;
+        f[q] = G->f;
+        s0[q] = G->g;
+        s1[q] = G->g2;
+        nb[q] = G->n * (G->interlace ? 2 : 1);
+        if (G->Ng > R->grp[big].Ng) big = q;
+    }
+    if (!rc) rc = fgk_fft_multi_create(&R->fm, R->ngroups, plans, f, s0, b->gset[1] ? s1 : NULL, nb);
+    free(plans);
+    free(f);
+    free(s0);
+    free(s1);
+    free(nb);
+    if (rc) return rc;
+    for (int q = 0; q < R->ngroups; ++q)   /* the bin kernel reads natural plane order */
+        if (fgk_fft_fourstep(b->side[R->grp[q].side].plan)) return 0;
+    R->bspec = R->grp[big].spec;
+    R->hjall = (fgk_binjob *)calloc((size_t)R->njobs, sizeof(fgk_binjob));
+    if (!R->hjall) { fgk_set_error("out of host memory"); return -1; }
+    unsigned long blk = 0, pblk = 0;
+    int k = 0;
+    for (int q = 0; q < R->ngroups; ++q) {
+        const batch_group *G = &R->grp[q];
+        const batch_side *S = &b->side[G->side];
+        unsigned nbk = 0, npk = 0;
+        fgk_bin_multi_blocks(&R->bspec, G->Ng, &nbk, &npk);
+        for (int i = 0; i < G->n; ++i, ++k) {
+            fgk_binjob *h = &R->hjall[k];
+            *h = G->hjob[i];
+            h->Ng = G->Ng;
+            h->fs_n1 = 0;
+            h->phase = S->d_phase;
+            h->winv = S->d_winv;
+            h->blk0 = (unsigned)blk;
+            h->pblk0 = (unsigned)pblk;
+            blk += nbk;
+            pblk += npk;
+        }
+        if (G->NK_max > R->NK_max) R->NK_max = G->NK_max;
+    }
+    if (blk >= (1ul << 31) || pblk >= (1ul << 31)) {
+        free(R->hjall);
+        R->hjall = NULL;
+        return 0;
+    }
+    R->bblocks = (unsigned)blk;
+    R->bpblocks = (unsigned)pblk;
+    TRY(fgk_malloc((void **)&R->djall, (size_t)R->njobs * sizeof(fgk_binjob)));
+    return 0;
+}
 
 /* the grids, spectra, bin tables and paint batch of round r, in the pool */
 static int round_setup(fftgal_batch_t *b, int r) {
@@ -968,6 +1054,8 @@ static int round_setup(fftgal_batch_t *b, int r) {
     }
     if (!rc) rc = fgk_paint_batch_create_ws(&R->paint, R->njobs, pd, fw, ws);
     free(pd);
+    if (!rc) fgk_paint_batch_per_side(R->paint, !want_multi());
+    if (!rc && R->ngroups > 1 && want_multi()) rc = round_multi(b, R);
     return rc;
 }
 
@@ -1204,6 +1292,19 @@ static int batch_tables(fftgal_batch_t *b, const float *const *x, const float *c
             }
             TRY(fgk_h2d(G->djob, G->hjob, (size_t)G->n * sizeof(fgk_binjob), b->s));
         }
+        batch_round *R = &b->rnd[r];
+        if (R->djall) {   /* the one-launch bin table: the same catalogs in group order */
+            int k = 0;
+            for (int q = 0; q < R->ngroups; ++q)
+                for (int i = 0; i < R->grp[q].n; ++i, ++k) {
+                    const fgk_binjob *h = &R->grp[q].hjob[i];
+                    R->hjall[k].norm = h->norm;
+                    R->hjall[k].Vinv = h->Vinv;
+                    R->hjall[k].Pshot = h->Pshot;
+                    R->hjall[k].npart = h->npart;
+                }
+            TRY(fgk_h2d(R->djall, R->hjall, (size_t)R->njobs * sizeof(fgk_binjob), b->s));
+        }
     }
     TRY(fgk_stream_sync(b->s));
     b->tables_ok = 1;
@@ -1237,14 +1338,22 @@ static int batch_work(fftgal_batch_t *b, int set0, fgk_event_t *ev) {
             TRY(fgk_paint_batch_stage(R->paint, stage, b->s));
         }
         MARK();
-        for (int q = 0; q < R->ngroups; ++q) {
-            batch_group *G = &R->grp[q];
-            TRY(fgk_fft_rows_f32_from_fixed(b->side[G->side].plan, G->f, set ? G->g2 : G->g, 1.0,
-                                            b->gset[1] ? 0 : 1, G->n * (G->interlace ? 2 : 1),
-                                            b->s));
+        if (R->fm) {
+            TRY(fgk_fft_multi_rows(R->fm, set, b->gset[1] ? 0 : 1, b->s));
+        } else {
+            for (int q = 0; q < R->ngroups; ++q) {
+                batch_group *G = &R->grp[q];
+                TRY(fgk_fft_rows_f32_from_fixed(b->side[G->side].plan, G->f, set ? G->g2 : G->g,
+                                                1.0, b->gset[1] ? 0 : 1,
+                                                G->n * (G->interlace ? 2 : 1), b->s));
+            }
         }
         for (int axis = 1; axis >= 0; --axis) {
             MARK();
+            if (R->fm) {
+                TRY(fgk_fft_multi_cols(R->fm, axis, -1, b->s));
+                continue;
+            }
             for (int q = 0; q < R->ngroups; ++q) {
                 batch_group *G = &R->grp[q];
                 TRY(fgk_fft_cols_f32(b->side[G->side].plan, G->f, axis, -1,
@@ -1252,11 +1361,16 @@ static int batch_work(fftgal_batch_t *b, int set0, fgk_event_t *ev) {
             }
         }
         MARK();
-        for (int q = 0; q < R->ngroups; ++q) {
-            batch_group *G = &R->grp[q];
-            const batch_side *S = &b->side[G->side];
-            TRY(fgk_fused_bin_f32_batch(&G->spec, G->djob, G->n, G->NK_max, S->d_phase,
-                                        S->d_winv, b->s));
+        if (R->djall) {
+            TRY(fgk_fused_bin_f32_multi(&R->bspec, R->djall, R->njobs, R->NK_max, R->bblocks,
+                                        R->bpblocks, b->s));
+        } else {
+            for (int q = 0; q < R->ngroups; ++q) {
+                batch_group *G = &R->grp[q];
+                const batch_side *S = &b->side[G->side];
+                TRY(fgk_fused_bin_f32_batch(&G->spec, G->djob, G->n, G->NK_max, S->d_phase,
+                                            S->d_winv, b->s));
+            }
         }
         MARK();
         if (r == b->nrounds - 1)

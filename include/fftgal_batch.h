@@ -136,14 +136,19 @@ int fftgal_memcpy_d2h(void *dst, const void *src, unsigned long bytes);
  * boxes and sizes, and the same catalog folded (grid box L/fold with the
  * positions in box L, src/app/P.c:35-36) -- through the pipeline above
  * with one launch per stage for all of them: one sort launch per stage,
- * one paint launch per tile side, one launch per FFT pass and grid side,
- * one fused bin launch pair per grid side, one copy of all bin sums.  The
- * sequence is captured once in a hipGraph; the catalogs' pointers, Np and
- * V are read from device tables, so later enqueues replay the same graph.
- * Every catalog's grids stay allocated: about 24 Ng^3 bytes per
- * interlaced catalog (int64 grids and FP32 spectra).  A catalog's grid may
- * need at most 256 x 128 paint tiles (Ng <= 512 with tiles of 16); larger
- * grids take a fftgal_power_t of their own. */
+ * one paint launch (tile sides 8, 12 and 16 in one kernel, each workgroup
+ * dispatching on its catalog's side), one launch per FFT pass (grid sides
+ * 32 to 512: multi-size kernels, each workgroup running the specialised
+ * transform of its grid's side), one fused bin launch pair, one copy of all
+ * bin sums.  Sides outside those sets, and FFTGAL_BATCH_MULTI=0 in the
+ * environment at creation, run one launch per tile or grid side instead.
+ * The catalogs are cut into rounds through a bounded device pool
+ * (FFTGAL_BATCH_BYTES, default 34 GiB; fftgal_batch_pool_bytes), and every
+ * round, every stage, is captured once in a hipGraph; the catalogs'
+ * pointers, Np and V are read from device tables, so later enqueues replay
+ * the same graph.  A catalog's grid may need at most 256 x 128 paint tiles
+ * (Ng <= 512 with tiles of 16); larger grids take a fftgal_power_t of their
+ * own. */
 typedef struct fftgal_batch fftgal_batch_t;
 
 typedef struct fftgal_job_desc {

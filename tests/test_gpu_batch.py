@@ -15,6 +15,16 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True, params=[1, 0], ids=["one_launch", "per_side"])
+def _launch_form(request, monkeypatch):
+    """Every test twice: rounds of several grid sides run each FFT pass,
+    the paint and the fused bin as one launch (multi-size kernels), or one
+    launch per grid or tile side (FFTGAL_BATCH_MULTI=0); both must give the
+    same bins."""
+    monkeypatch.setenv("FFTGAL_BATCH_MULTI", str(request.param))
+    return request.param
+
+
 def _dev(torch, *arrs):
     ts = [torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda() for a in arrs]
     torch.cuda.synchronize()
