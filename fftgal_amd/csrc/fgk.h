@@ -57,6 +57,10 @@ typedef struct fgk_fftplan fgk_fftplan;
 int  fgk_fft_plan_create(fgk_fftplan **plan, int Ng, int precision);
 void fgk_fft_plan_destroy(fgk_fftplan *plan);
 int  fgk_fft_supported(int Ng);     /* 1 if Ng = 2^a 3^b 5^c 7^d 11^e 13^f, even */
+/* 1 if an FP64 plan takes Ng: any Ng in [1, 1024] (odd sides, or prime
+ * factors above 13, run complex transforms of whole rows in pairs; Bluestein
+ * for lengths with a prime factor above 13), or any side fgk_fft_supported */
+int  fgk_fft_supported_f64(int Ng);
 /* FP64, in place on the padded layout f[(i*Ng+j)*Ng_pad+k], unnormalised,
  * forward sign -1; the result is multiplied by scale */
 int  fgk_fft_r2c_f64(fgk_fftplan *p, double *f, double scale, fgk_stream_t s);

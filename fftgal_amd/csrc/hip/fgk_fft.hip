@@ -42,6 +42,15 @@ struct fgk_fftplan {
     int fs_n1;           // i-axis pass as a four-step Ng = n1 x (Ng / n1), 0 = one pass
     void *stw_fs1;       // stage twiddles, lengths fs_n1 and Ng / fs_n1
     void *stw_fs2;
+    // FP64 grids of any side (odd, or a prime factor above 13): rows are
+    // transformed in pairs as one complex row, every pencil by a complex
+    // length-Ng transform (Stockham, or Bluestein through a power of two M)
+    int general;
+    int blue_M;          // Bluestein length (0: Stockham of length Ng)
+    Stages sM;
+    void *twM;           // exp(-2 pi i t / M), t < M
+    void *chirp;         // exp(-i pi (n^2 mod 2 Ng) / Ng), n < Ng
+    void *bf[2];         // FFT_M of the convolution kernel: [0] forward, [1] inverse
 };
 
 static int factor_stages(int n, Stages *s) {
@@ -64,6 +73,13 @@ extern "C" int fgk_fft_supported(int Ng) {
     if (Ng < 2 || (Ng & 1)) return 0;
     Stages s;
     return factor_stages(Ng, &s) == 0 && factor_stages(Ng / 2, &s) == 0;
+}
+
+// FP64 plans take any side up to FGK_GENERAL_MAX (Bluestein lengths up to
+// 2048: two LDS buffers of 32 KB per pencil)
+#define FGK_GENERAL_MAX 1024
+extern "C" int fgk_fft_supported_f64(int Ng) {
+    return fgk_fft_supported(Ng) || (Ng >= 1 && Ng <= FGK_GENERAL_MAX);
 }
 
 // ------------------------------------------------------------ butterflies
@@ -342,6 +358,145 @@ k_cols(cpx<T> *g, int N, int Nk, long stride_t, long stride_o, int C, int lgC, S
     }
     __syncthreads();
     cpx<T> *res = lds_fft<T>(a, b, S, C, lgC, 1, C, 1, tw, 1, sign);
+    for (int idx = threadIdx.x; idx < N * C; idx += blockDim.x) {
+        const int c = idx & (C - 1);
+        const int t = idx >> lgC;
+        if (c < kc) base[(long)t * stride_t + c] = cscale(res[idx], scale);
+    }
+}
+
+// ---------------------------------------------- FP64 grids of any side
+//
+// A complex transform of length N on P pencils in LDS (element t of pencil p
+// at a[t*st + p*sp], buffers of L = N, or M for Bluestein, elements per
+// pencil).  N whose prime factors are at most 13 runs the Stockham stages;
+// otherwise Bluestein's chirp-z form (Bluestein 1970): with
+// w[n] = exp(s i pi n^2 / N) (s = the transform's sign),
+//   X[k] = w[k] sum_n (x[n] w[n]) conj(w[k - n]),
+// a circular convolution of length M >= 2N - 1 (a power of two) done as
+// FFT_M, a product with the precomputed FFT_M of the kernel conj(w[m])
+// (m in (-N, N), wrapped), and an inverse FFT_M scaled by 1/M.
+struct GenFFT {
+    int N, M;                    // M: Bluestein length, 0 for Stockham
+    Stages sN, sM;
+    const cpx<double> *tw, *twM, *chirp;
+    const cpx<double> *bf[2];    // [0] sign -1, [1] sign +1
+};
+
+__device__ __forceinline__ cpx<double> gen_chirp(const GenFFT &G, int t, int sign) {
+    cpx<double> w = G.chirp[t];
+    if (sign > 0) w.y = -w.y;
+    return w;
+}
+
+__device__ cpx<double> *gen_cfft(cpx<double> *a, cpx<double> *b, const GenFFT &G, int P, int lgP,
+                                 int pfast, int st, int sp, int sign) {
+    if (!G.M) return lds_fft<double>(a, b, G.sN, P, lgP, pfast, st, sp, G.tw, 1, sign);
+    const int N = G.N, M = G.M;
+    for (int idx = threadIdx.x; idx < M * P; idx += blockDim.x) {
+        int p, t;
+        if (pfast) { p = idx & (P - 1); t = idx >> lgP; } else { p = idx / M; t = idx - p * M; }
+        cpx<double> &v = a[t * st + p * sp];
+        v = t < N ? cmul(v, gen_chirp(G, t, sign)) : cmk(0.0, 0.0);
+    }
+    __syncthreads();
+    cpx<double> *r = lds_fft<double>(a, b, G.sM, P, lgP, pfast, st, sp, G.twM, 1, -1);
+    cpx<double> *o = r == a ? b : a;
+    const cpx<double> *bf = G.bf[sign > 0 ? 1 : 0];
+    for (int idx = threadIdx.x; idx < M * P; idx += blockDim.x) {
+        int p, t;
+        if (pfast) { p = idx & (P - 1); t = idx >> lgP; } else { p = idx / M; t = idx - p * M; }
+        cpx<double> &v = r[t * st + p * sp];
+        v = cmul(v, bf[t]);
+    }
+    __syncthreads();
+    r = lds_fft<double>(r, o, G.sM, P, lgP, pfast, st, sp, G.twM, 1, +1);
+    const double inv = 1.0 / (double)M;
+    for (int idx = threadIdx.x; idx < N * P; idx += blockDim.x) {
+        int p, t;
+        if (pfast) { p = idx & (P - 1); t = idx >> lgP; } else { p = idx / N; t = idx - p * N; }
+        cpx<double> &v = r[t * st + p * sp];
+        v = cscale(cmul(v, gen_chirp(G, t, sign)), inv);
+    }
+    __syncthreads();
+    return r;
+}
+
+// rows of the padded layout (Ng_pad doubles each) in pairs (2q, 2q+1) as
+// z = x_{2q} + i x_{2q+1}: r2c (dir 0) splits Z into the two half spectra,
+// X1[k] = (Z[k] + conj Z[-k]) / 2, X2[k] = (Z[k] - conj Z[-k]) / 2i,
+// k <= N/2; c2r (dir 1) forms Z[k] = X1[k] + i X2[k] from the Hermitian
+// extensions, transforms with sign +1 and writes Re, Im to the two rows.
+// P row pairs per workgroup, L = N or M complex of LDS per pair and buffer.
+__global__ void __launch_bounds__(256)
+k_rows_gen(double *f, int Ng_pad, long nrows, GenFFT G, int P, int dir, double out_scale) {
+    extern __shared__ char smem_raw[];
+    const int N = G.N, L = G.M ? G.M : N, K1 = N / 2 + 1;
+    cpx<double> *a = (cpx<double> *)smem_raw;
+    cpx<double> *b = a + (size_t)P * L;
+    const long q0 = (long)xcd_block(blockIdx.x, gridDim.x) * P;
+    auto rowp = [&](long r) -> double * { return r < nrows ? f + r * (long)Ng_pad : nullptr; };
+    for (int idx = threadIdx.x; idx < P * N; idx += blockDim.x) {
+        const int p = idx / N, t = idx - p * N;
+        const double *r0 = rowp(2 * (q0 + p)), *r1 = rowp(2 * (q0 + p) + 1);
+        cpx<double> z = cmk(0.0, 0.0);
+        if (dir == 0) {
+            z = cmk(r0 ? r0[t] : 0.0, r1 ? r1[t] : 0.0);
+        } else {
+            const int k = t <= N / 2 ? t : N - t;
+            cpx<double> x1 = r0 ? ((const cpx<double> *)r0)[k] : cmk(0.0, 0.0);
+            cpx<double> x2 = r1 ? ((const cpx<double> *)r1)[k] : cmk(0.0, 0.0);
+            if (t > N / 2) { x1.y = -x1.y; x2.y = -x2.y; }
+            z = cmk(x1.x - x2.y, x1.y + x2.x);   // X1 + i X2
+        }
+        a[p * L + t] = z;
+    }
+    __syncthreads();
+    cpx<double> *res = gen_cfft(a, b, G, P, 0, 0, 1, L, dir == 0 ? -1 : +1);
+    if (dir == 0) {
+        for (int idx = threadIdx.x; idx < P * K1; idx += blockDim.x) {
+            const int p = idx / K1, k = idx - p * K1;
+            double *r0 = rowp(2 * (q0 + p)), *r1 = rowp(2 * (q0 + p) + 1);
+            const cpx<double> Z = res[p * L + k];
+            cpx<double> Zc = res[p * L + (k ? N - k : 0)];
+            Zc.y = -Zc.y;
+            const cpx<double> E = cscale(cadd(Z, Zc), 0.5);
+            const cpx<double> D = csub(Z, Zc);
+            if (r0) ((cpx<double> *)r0)[k] = cscale(E, out_scale);
+            if (r1) ((cpx<double> *)r1)[k] = cscale(cmk(0.5 * D.y, -0.5 * D.x), out_scale);
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < P * N; idx += blockDim.x) {
+            const int p = idx / N, t = idx - p * N;
+            double *r0 = rowp(2 * (q0 + p)), *r1 = rowp(2 * (q0 + p) + 1);
+            const cpx<double> z = res[p * L + t];
+            if (r0) r0[t] = z.x * out_scale;
+            if (r1) r1[t] = z.y * out_scale;
+        }
+    }
+}
+
+// columns: C consecutive columns of one plane per workgroup (as k_cols)
+__global__ void __launch_bounds__(256)
+k_cols_gen(cpx<double> *g, int Nk, long stride_t, long stride_o, int C, int lgC, GenFFT G,
+           int sign, double scale) {
+    extern __shared__ char smem_raw[];
+    const int N = G.N, L = G.M ? G.M : N;
+    cpx<double> *a = (cpx<double> *)smem_raw;
+    cpx<double> *b = a + (size_t)L * C;
+    const int nkb = (Nk + C - 1) / C;
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int o = lb / nkb;
+    const int k0 = (lb - o * nkb) * C;
+    cpx<double> *base = g + (long)o * stride_o + k0;
+    const int kc = Nk - k0 < C ? Nk - k0 : C;
+    for (int idx = threadIdx.x; idx < N * C; idx += blockDim.x) {
+        const int c = idx & (C - 1);
+        const int t = idx >> lgC;
+        a[idx] = c < kc ? base[(long)t * stride_t + c] : cmk(0.0, 0.0);
+    }
+    __syncthreads();
+    cpx<double> *res = gen_cfft(a, b, G, C, lgC, 1, C, 1, sign);
     for (int idx = threadIdx.x; idx < N * C; idx += blockDim.x) {
         const int c = idx & (C - 1);
         const int t = idx >> lgC;
@@ -815,22 +970,94 @@ static int stage_twiddles(int L, void **dev) {
 
 // ------------------------------------------------------------------ host
 
+// FP64 tables of a general plan: Bluestein length, twiddles of M, the chirp
+// and the kernel's FFT for both signs (a direct FP64 DFT on the host, once)
+static int general_setup(fgk_fftplan *p) {
+    const int N = p->Ng;
+    p->general = 1;
+    if (factor_stages(N, &p->sN) == 0) return 0;   // Stockham of length N
+    int M = 1;
+    while (M < 2 * N - 1) M <<= 1;
+    p->blue_M = M;
+    factor_stages(M, &p->sM);
+    double *twM = (double *)malloc((size_t)M * 2 * sizeof(double));
+    double *chirp = (double *)malloc((size_t)N * 2 * sizeof(double));
+    double *ker = (double *)malloc((size_t)M * 2 * sizeof(double));
+    double *bf = (double *)malloc((size_t)M * 2 * sizeof(double));
+    int rc = (!twM || !chirp || !ker || !bf) ? -1 : 0;
+    if (rc) fgk_set_error("out of host memory");
+    if (!rc) {
+        for (int t = 0; t < M; ++t) {
+            const double ang = 2.0 * M_PI * (double)t / (double)M;
+            twM[2 * t] = cos(ang);
+            twM[2 * t + 1] = -sin(ang);
+        }
+        for (int n = 0; n < N; ++n) {   // exp(-i pi (n^2 mod 2N) / N), exact argument
+            const long e = ((long)n * n) % (2L * N);
+            const double ang = M_PI * (double)e / (double)N;
+            chirp[2 * n] = cos(ang);
+            chirp[2 * n + 1] = -sin(ang);
+        }
+    }
+    for (int sg = 0; sg < 2 && !rc; ++sg) {
+        // kernel conj(w_s[m]) for m in (-N, N) wrapped into M, w_s = chirp
+        // (sign -1) or its conjugate (sign +1)
+        for (int m = 0; m < M; ++m) ker[2 * m] = ker[2 * m + 1] = 0.0;
+        for (int m = 0; m < N; ++m) {
+            const double re = chirp[2 * m], im = sg == 0 ? -chirp[2 * m + 1] : chirp[2 * m + 1];
+            ker[2 * m] = re;
+            ker[2 * m + 1] = im;
+            if (m) {
+                ker[2 * (M - m)] = re;
+                ker[2 * (M - m) + 1] = im;
+            }
+        }
+        for (int k = 0; k < M; ++k) {   // FFT_M, sign -1
+            double sr = 0.0, si = 0.0;
+            for (int m = 0; m < M; ++m) {
+                const long e = ((long)k * m) % M;
+                const double c = twM[2 * e], sn = twM[2 * e + 1];
+                sr += ker[2 * m] * c - ker[2 * m + 1] * sn;
+                si += ker[2 * m] * sn + ker[2 * m + 1] * c;
+            }
+            bf[2 * k] = sr;
+            bf[2 * k + 1] = si;
+        }
+        if (!rc) rc = fgk_malloc(&p->bf[sg], (size_t)M * 2 * sizeof(double));
+        if (!rc) rc = fgk_h2d(p->bf[sg], bf, (size_t)M * 2 * sizeof(double), NULL);
+    }
+    if (!rc) rc = fgk_malloc(&p->twM, (size_t)M * 2 * sizeof(double));
+    if (!rc) rc = fgk_h2d(p->twM, twM, (size_t)M * 2 * sizeof(double), NULL);
+    if (!rc) rc = fgk_malloc(&p->chirp, (size_t)N * 2 * sizeof(double));
+    if (!rc) rc = fgk_h2d(p->chirp, chirp, (size_t)N * 2 * sizeof(double), NULL);
+    if (!rc) rc = fgk_stream_sync(NULL);
+    free(twM);
+    free(chirp);
+    free(ker);
+    free(bf);
+    return rc;
+}
+
 extern "C" int fgk_fft_plan_create(fgk_fftplan **out, int Ng, int precision) {
     *out = NULL;
-    if (!fgk_fft_supported(Ng)) {
-        fgk_set_error("grid size Ng=%d unsupported: need even Ng = 2^a 3^b 5^c 7^d 11^e 13^f", Ng);
-        return -2;
-    }
     if (precision != 4 && precision != 8) {
         fgk_set_error("precision must be 4 or 8");
+        return -2;
+    }
+    const int general = !fgk_fft_supported(Ng);
+    if (general && (precision != 8 || !fgk_fft_supported_f64(Ng))) {
+        fgk_set_error("grid size Ng=%d unsupported: need even Ng = 2^a 3^b 5^c 7^d 11^e 13^f "
+                      "(FP64 plans: any Ng in [1, %d])", Ng, FGK_GENERAL_MAX);
         return -2;
     }
     fgk_fftplan *p = (fgk_fftplan *)calloc(1, sizeof(fgk_fftplan));
     if (!p) { fgk_set_error("out of host memory"); return -1; }
     p->Ng = Ng;
     p->prec = precision;
-    factor_stages(Ng, &p->sN);
-    factor_stages(Ng / 2, &p->sH);
+    if (!general) {
+        factor_stages(Ng, &p->sN);
+        factor_stages(Ng / 2, &p->sH);
+    }
     size_t bytes = (size_t)Ng * 2 * precision;
     void *host = malloc(bytes);
     if (!host) { free(p); fgk_set_error("out of host memory"); return -1; }
@@ -852,6 +1079,10 @@ extern "C" int fgk_fft_plan_create(fgk_fftplan **out, int Ng, int precision) {
         fgk_free(p->tw);
         free(p);
         return fgk_hip_fail(e, "hipMemcpy twiddles", __FILE__, __LINE__);
+    }
+    if (general && general_setup(p)) {
+        fgk_fft_plan_destroy(p);
+        return -1;
     }
     if (precision == 4 && fgk_fft_f32_specialised(Ng)) {
         p->spec = 1;
@@ -890,6 +1121,10 @@ extern "C" int fgk_fft_force_generic(fgk_fftplan *p, int on) {
 extern "C" void fgk_fft_plan_destroy(fgk_fftplan *p) {
     if (!p) return;
     fgk_free(p->tw);
+    fgk_free(p->twM);
+    fgk_free(p->chirp);
+    fgk_free(p->bf[0]);
+    fgk_free(p->bf[1]);
     fgk_free(p->stw_cols);
     fgk_free(p->stw_rows);
     fgk_free(p->stw_fs1);
@@ -1127,9 +1362,60 @@ extern "C" int fgk_fft_multi_cols(fgk_fftmulti *m, int axis, int sign, fgk_strea
     return 0;
 }
 
+static GenFFT gen_args(const fgk_fftplan *p) {
+    GenFFT G;
+    G.N = p->Ng;
+    G.M = p->blue_M;
+    G.sN = p->sN;
+    G.sM = p->sM;
+    G.tw = (const cpx<double> *)p->tw;
+    G.twM = (const cpx<double> *)p->twM;
+    G.chirp = (const cpx<double> *)p->chirp;
+    G.bf[0] = (const cpx<double> *)p->bf[0];
+    G.bf[1] = (const cpx<double> *)p->bf[1];
+    return G;
+}
+
+// LDS of general pencils: two buffers of L complex doubles per pencil, up
+// to 64 KB per workgroup
+static const size_t kGenLds = 64 * 1024;
+
+static int general_rows(fgk_fftplan *p, double *f, int dir, double scale, hipStream_t s) {
+    const GenFFT G = gen_args(p);
+    const int N = p->Ng, L = G.M ? G.M : N, Ng_pad = 2 * (N / 2 + 1);
+    const long nrows = (long)N * N, npairs = (nrows + 1) / 2;
+    int P = 1;
+    while (2 * P * L < 1024 && (size_t)2 * 2 * P * L * sizeof(cpx<double>) <= kGenLds) P *= 2;
+    const size_t lds = (size_t)2 * P * L * sizeof(cpx<double>);
+    hipLaunchKernelGGL(k_rows_gen, dim3(fgk_ceil_div(npairs, P)), dim3(256), lds, s, f, Ng_pad,
+                       nrows, G, P, dir, scale);
+    FGK_LAUNCHED();
+    return 0;
+}
+
+static int general_cols(fgk_fftplan *p, double *f, int axis, int sign, double scale,
+                        hipStream_t s) {
+    const GenFFT G = gen_args(p);
+    const int N = p->Ng, L = G.M ? G.M : N, Nk = N / 2 + 1;
+    int C = 16;
+    while (C > 1 && (size_t)2 * L * C * sizeof(cpx<double>) > kGenLds) C >>= 1;
+    const ColShape cs = col_shape(N, Nk, axis, 1);
+    const long nb = cs.nouter * ((cs.ncols + C - 1) / C);
+    const size_t lds = (size_t)2 * L * C * sizeof(cpx<double>);
+    hipLaunchKernelGGL(k_cols_gen, dim3((unsigned)nb), dim3(256), lds, s, (cpx<double> *)f,
+                       cs.ncols, cs.stride_t, cs.stride_o, C, ilog2(C), G, sign, scale);
+    FGK_LAUNCHED();
+    return 0;
+}
+
 extern "C" int fgk_fft_r2c_f64(fgk_fftplan *p, double *f, double scale, fgk_stream_t st) {
     if (p->prec != 8) { fgk_set_error("plan precision is not FP64"); return -2; }
     hipStream_t s = (hipStream_t)st;
+    if (p->general) {
+        if (general_rows(p, f, 0, 1.0, s)) return -1;
+        if (general_cols(p, f, 1, -1, 1.0, s)) return -1;
+        return general_cols(p, f, 0, -1, scale, s);
+    }
     const int N = p->Ng, M = N / 2;
     const long nrows = (long)N * N;
     const int P = rows_per_wg(p, sizeof(cpx<double>), 0);
@@ -1145,6 +1431,11 @@ extern "C" int fgk_fft_r2c_f64(fgk_fftplan *p, double *f, double scale, fgk_stre
 extern "C" int fgk_fft_c2r_f64(fgk_fftplan *p, double *f, double scale, fgk_stream_t st) {
     if (p->prec != 8) { fgk_set_error("plan precision is not FP64"); return -2; }
     hipStream_t s = (hipStream_t)st;
+    if (p->general) {
+        if (general_cols(p, f, 0, +1, 1.0, s)) return -1;
+        if (general_cols(p, f, 1, +1, 1.0, s)) return -1;
+        return general_rows(p, f, 1, scale, s);
+    }
     const int N = p->Ng, M = N / 2;
     const long nrows = (long)N * N;
     if (launch_cols<double>(p, f, 0, +1, 1.0, s)) return -1;

@@ -129,8 +129,8 @@ static void measure_fft_shape(fft_dev *d, long n, fftgal_tuning *best) {
 fft_t *fft_init(int Ng, double L, char *wisdom) {
     if (fftgal_verbose())
         fprintf(stderr, "fft_init() making a %d^3 grid in a %g^3 box\n", Ng, L);
-    DI_ASSERT(Ng >= 2 && fgk_fft_supported(Ng),
-              "fft_init: Ng must be even with prime factors in {2,3,5,7,11,13}");
+    DI_ASSERT(fgk_fft_supported_f64(Ng), "fft_init: Ng must be in [1, 1024] (or an even "
+              "Ng with prime factors in {2,3,5,7,11,13})");
     /* a wisdom file that has no line for this grid will be written after
      * measuring: fail before any device work if it cannot be (the reference
      * asserts on the export, src/fft.c:40) */

@@ -43,10 +43,11 @@ typedef struct fft {
     double offset[3];  /* grid wrt boundary [0,L)^3 in grid unit */
 } fft_t;
 
-/* src/fft.c:13-47.  wisdom: "FFTW_ESTIMATE" (default launch shapes) or a
- * tuning cache path: a line for this Ng is read if the file holds one;
- * otherwise the FP64 transform's launch shapes are timed on the new grid
- * (as FFTW_MEASURE) and the fastest is written to the file, and an
+/* src/fft.c:13-47.  Ng: any side in [1, 1024], or a larger even side with
+ * prime factors in {2,3,5,7,11,13}.  wisdom: "FFTW_ESTIMATE" (default launch
+ * shapes) or a tuning cache path: a line for this Ng is read if the file
+ * holds one; otherwise the FP64 transform's launch shapes are timed on the
+ * new grid (as FFTW_MEASURE) and the fastest is written to the file, and an
  * unwritable path aborts, as the reference's failed wisdom export does */
 fft_t *fft_init(int Ng, double L, char *wisdom);
 /* src/fft.c:50-54 */

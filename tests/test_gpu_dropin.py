@@ -38,7 +38,11 @@ def test_octet_known_answer(torch_cuda, lib):
                 assert abs(g.f[i, j, k] - 64.0 / 27 / (i + 1) / (j + 1) / (k + 1)) < eps
 
 
-@pytest.mark.parametrize("Ng", [2, 4, 6, 8, 10, 12, 16, 24, 30, 48, 64, 96])
+@pytest.mark.parametrize("Ng", [2, 4, 6, 8, 10, 12, 16, 24, 30, 48, 64, 96,
+                                # any side, as FFTW plans it (src/fft.c:28-29): odd
+                                # sides (paired real rows, Stockham), prime factors
+                                # above 13 (Bluestein), and the smallest grids
+                                1, 3, 5, 9, 15, 17, 21, 34, 38, 45, 97])
 def test_x2k_k2x_vs_oracle(torch_cuda, lib, orc, Ng):
     import fftgal_amd as fa
     rng = np.random.default_rng(Ng)
@@ -117,10 +121,13 @@ def test_offset_phase_roundtrip_and_deconv(torch_cuda, lib, orc):
     assert np.abs(gg.fk() - go.fk()).max() <= 1e-13 * np.abs(go.fk()).max()
 
 
-@pytest.mark.parametrize("Ng,Np,interlace", [(32, 20000, True), (48, 30000, False), (64, 50000, True)])
+@pytest.mark.parametrize("Ng,Np,interlace", [(32, 20000, True), (48, 30000, False), (64, 50000, True),
+                                             (45, 30000, True), (34, 20000, False),
+                                             (37, 20000, True)])
 def test_pipeline_bins_vs_oracle(torch_cuda, lib, orc, Ng, Np, interlace):
     """fft_p2k (or p2g+x2k+deconv) -> P -> Pl/Pmu bins, reference call order
-    (src/app/P.c:53-66); counts exact, sums to FP64 rounding."""
+    (src/app/P.c:53-66); counts exact, sums to FP64 rounding.  Odd sides and
+    sides with a prime factor above 13 included."""
     import fftgal_amd as fa
     from fftgal_amd import mockgen
     L = 500.0
