@@ -346,7 +346,9 @@ def main_batch(args):
     b.close()
     torch.cuda.empty_cache()
     if args.profile_only:   # for kernel traces: no plain FP64 comparison
-        print(json.dumps({"ms_per_step": ms_per_step, "value": value, "grid_sets": sets}))
+        print(json.dumps({"ms_per_step": ms_per_step, "value": value, "grid_sets": sets,
+                          "rounds": rounds, "pool_bytes": pool, "device_bytes": batch_bytes,
+                          "stages_ms": {k: round(v, 3) for k, v in prof.items()}}))
         return
 
     # plain FP64 form, catalog by catalog (device resident)
