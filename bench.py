@@ -536,8 +536,9 @@ def main_extras(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (independent "
+              "catalogs would shard across GPUs, one process each, with no communication)",
+              file=sys.stderr)
         sys.exit(2)
     if args.extras:
         return main_extras(args)
