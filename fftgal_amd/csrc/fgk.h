@@ -41,6 +41,8 @@ int  fgk_event_destroy(fgk_event_t e);
 int  fgk_event_record(fgk_event_t e, fgk_stream_t s);
 int  fgk_event_elapsed_ms(float *ms, fgk_event_t a, fgk_event_t b);
 int  fgk_event_sync(fgk_event_t e);         /* wait for the work before its record */
+int  fgk_event_create_dep(fgk_event_t *e);  /* ordering only, no timestamps */
+int  fgk_stream_wait_event(fgk_stream_t s, fgk_event_t e);  /* s waits for e's record */
 int  fgk_device_info(char *name, size_t len, int *cu_count, size_t *mem_bytes);
 /* hipGraph capture of the work enqueued on a stream between begin/end */
 typedef void *fgk_graph_t;

@@ -148,6 +148,19 @@ extern "C" int fgk_event_create(fgk_event_t *e) {
     return 0;
 }
 
+// an event that only orders work (no timestamps)
+extern "C" int fgk_event_create_dep(fgk_event_t *e) {
+    hipEvent_t h;
+    FGK_HIP(hipEventCreateWithFlags(&h, hipEventDisableTiming));
+    *e = (fgk_event_t)h;
+    return 0;
+}
+
+extern "C" int fgk_stream_wait_event(fgk_stream_t s, fgk_event_t e) {
+    FGK_HIP(hipStreamWaitEvent((hipStream_t)s, (hipEvent_t)e, 0));
+    return 0;
+}
+
 extern "C" int fgk_event_destroy(fgk_event_t e) {
     if (e) FGK_HIP(hipEventDestroy((hipEvent_t)e));
     return 0;

@@ -704,7 +704,10 @@ int fftgal_memcpy_d2h(void *dst, const void *src, unsigned long bytes) {
  * Device memory is a bounded pool.  The catalogs are cut into rounds of
  * about equal size, in input order, as few as the pool allows; the rounds
  * run one after the other through the same memory:
- *   pool = [grid set 0 | grid set 1 | spectra and sort words of a round]
+ *   pool = [grid set 0 | grid set 1 | spectra of a round | sort words of a round]
+ * The sort words have a region of their own so that a round's sort (on a
+ * second stream) can run while the previous round's FFT passes and bin
+ * still read its spectra; the next paint waits for both.
  * A grid set holds the int64 grids of one round (the largest round's size);
  * the rounds alternate between the two sets, and each round's paint clears
  * the part of the other set that the previous round's row FFT has read.  With one set
@@ -780,6 +783,9 @@ struct fftgal_batch {
     batch_round *rnd;
     batch_side *side;
     fgk_stream_t s;
+    fgk_stream_t s2;             /* a round's sort beside the previous round's FFT and bin */
+    fgk_event_t *evp, *evs;      /* per round: its paint done (s), its sort done (s2) */
+    char *ws_region;             /* the sort words of every round */
     int64_t *d_bins, *h_bins;    /* fixed-point pairs (fgk.h), device and pinned */
     double *h_sums;              /* the values in FP64 */
     size_t nbins;                /* values */
@@ -836,6 +842,13 @@ int fftgal_batch_destroy(fftgal_batch_t *b) {
     fgk_free(b->d_bins);
     fgk_host_free(b->h_bins);
     free(b->h_sums);
+    for (int r = 0; r < b->nrounds; ++r) {
+        if (b->evp && b->evp[r]) fgk_event_destroy(b->evp[r]);
+        if (b->evs && b->evs[r]) fgk_event_destroy(b->evs[r]);
+    }
+    free(b->evp);
+    free(b->evs);
+    if (b->s2) fgk_stream_destroy(b->s2);
     if (b->s) fgk_stream_destroy(b->s);
     free(b);
     return 0;
@@ -875,14 +888,18 @@ static int side_setup(fftgal_batch_t *b, batch_side *S, int interlace, const cha
 }
 
 /* device bytes of catalog j's grids (one set), spectra and sort words */
-static void job_bytes(const fftgal_job_desc *d, size_t *gbytes, size_t *fwbytes) {
+/* a catalog's int64 grids, FP32 spectra and sort words (the sort words of a
+ * round of catalogs take at most the sum of theirs) */
+static void job_bytes(const fftgal_job_desc *d, size_t *gbytes, size_t *fbytes, size_t *wbytes) {
     const size_t cells = (size_t)d->Ng * d->Ng * d->Ng, NG = d->interlace ? 2 : 1;
     *gbytes = (NG * cells * sizeof(int64_t) + 255) & ~(size_t)255;
-    *fwbytes = ((NG * cells * sizeof(float) + 255) & ~(size_t)255) +
-               fgk_paint_batch_ws_bytes(d->Np_max);
+    *fbytes = (NG * cells * sizeof(float) + 255) & ~(size_t)255;
+    *wbytes = (fgk_paint_batch_ws_bytes(d->Np_max) + 255) & ~(size_t)255;
 }
 
-static size_t pool_size(size_t gmax, size_t fwmax, int sets) { return sets * gmax + fwmax; }
+static size_t pool_size(size_t gmax, size_t fmax, size_t wmax, int sets) {
+    return sets * gmax + fmax + wmax;
+}
 
 /* one launch per stage over the groups of a round (FFT passes and the
  * fused bin), unless FFTGAL_BATCH_MULTI=0 */
@@ -1048,11 +1065,11 @@ static int round_setup(fftgal_batch_t *b, int r) {
     }
     const size_t ws = fgk_paint_batch_ws_bytes(Np_round);
     int rc = 0;
-    if (fw + ws > b->pool + b->pool_bytes) {
+    if (fw > b->ws_region || b->ws_region + ws > b->pool + b->pool_bytes) {
         fgk_set_error("batch: round %d needs more than the pool", r);
         rc = -1;
     }
-    if (!rc) rc = fgk_paint_batch_create_ws(&R->paint, R->njobs, pd, fw, ws);
+    if (!rc) rc = fgk_paint_batch_create_ws(&R->paint, R->njobs, pd, b->ws_region, ws);
     free(pd);
     if (!rc) fgk_paint_batch_per_side(R->paint, !want_multi());
     if (!rc && R->ngroups > 1 && want_multi()) rc = round_multi(b, R);
@@ -1063,35 +1080,39 @@ static int round_setup(fftgal_batch_t *b, int r) {
  * adding it would take the pool past `budget` or the round past `target`
  * bytes (sets grid sets, spectra and sort words).  Fills b->rnd when b is
  * given; returns the number of rounds (-1: out of host memory) and the
- * largest round's grid and spectra-and-words bytes. */
+ * largest round's grid, spectra and sort-word bytes. */
 static int form_rounds(fftgal_batch_t *b, const fftgal_job_desc *jobs, const int *order,
                        int njobs, size_t budget, int sets, size_t target, size_t *gmax_out,
-                       size_t *fwmax_out) {
-    size_t gmax = 0, fwmax = 0, gcur = 0, fwcur = 0;
+                       size_t *fmax_out, size_t *wmax_out) {
+    size_t gmax = 0, fmax = 0, wmax = 0, gcur = 0, fcur = 0, wcur = 0;
     int nr = 0;
     for (int i = 0; i < njobs; ++i) {
-        size_t gj, fwj;
-        job_bytes(&jobs[order[i]], &gj, &fwj);
-        const size_t g2 = gcur + gj, fw2 = fwcur + fwj;
-        const int fits = pool_size(g2 > gmax ? g2 : gmax, fw2 > fwmax ? fw2 : fwmax, sets) <= budget
-                         && (size_t)sets * g2 + fw2 <= target;
+        size_t gj, fj, wj;
+        job_bytes(&jobs[order[i]], &gj, &fj, &wj);
+        const size_t g2 = gcur + gj, f2 = fcur + fj, w2 = wcur + wj;
+        const int fits = pool_size(g2 > gmax ? g2 : gmax, f2 > fmax ? f2 : fmax,
+                                   w2 > wmax ? w2 : wmax, sets) <= budget &&
+                         (size_t)sets * g2 + f2 + w2 <= target;
         if (nr == 0 || !fits) {
             if (b) {
                 b->rnd[nr].jobs = (int *)calloc((size_t)njobs, sizeof(int));
                 if (!b->rnd[nr].jobs) return -1;
             }
             ++nr;
-            gcur = fwcur = 0;
+            gcur = fcur = wcur = 0;
         }
         if (b) b->rnd[nr - 1].jobs[b->rnd[nr - 1].njobs++] = order[i];
         gcur += gj;
-        fwcur += fwj;
+        fcur += fj;
+        wcur += wj;
         if (gcur > gmax) gmax = gcur;
-        if (fwcur > fwmax) fwmax = fwcur;
+        if (fcur > fmax) fmax = fcur;
+        if (wcur > wmax) wmax = wcur;
     }
     if (b) b->nrounds = nr;
     *gmax_out = gmax;
-    *fwmax_out = fwmax;
+    *fmax_out = fmax;
+    *wmax_out = wmax;
     return nr;
 }
 
@@ -1186,25 +1207,22 @@ int fftgal_batch_create(fftgal_batch_t **out, int njobs, const fftgal_job_desc *
      * round, so one small last round costs as much as a full one */
     size_t total = 0;
     for (int i = 0; i < njobs; ++i) {
-        size_t gj, fwj;
-        job_bytes(&jobs[i], &gj, &fwj);
-        total += (size_t)sets * gj + fwj;
+        size_t gj, fj, wj;
+        job_bytes(&jobs[i], &gj, &fj, &wj);
+        total += (size_t)sets * gj + fj + wj;
     }
     int nr = 0;
-    size_t gmax = 0, fwmax = 0;
-    if (!rc) nr = form_rounds(NULL, jobs, order, njobs, budget, sets, (size_t)-1, &gmax, &fwmax);
+    size_t gmax = 0, fmax = 0, wmax = 0;
+    if (!rc)
+        nr = form_rounds(NULL, jobs, order, njobs, budget, sets, (size_t)-1, &gmax, &fmax, &wmax);
     if (!rc && nr > 1) {
-        size_t g2 = 0, fw2 = 0;
-        const int n2 = form_rounds(NULL, jobs, order, njobs, budget, sets, (total + nr - 1) / nr,
-                                   &g2, &fw2);
-        if (n2 == nr) {
-            nr = form_rounds(b, jobs, order, njobs, budget, sets, (total + nr - 1) / nr, &gmax,
-                             &fwmax);
-        } else {
-            nr = form_rounds(b, jobs, order, njobs, budget, sets, (size_t)-1, &gmax, &fwmax);
-        }
+        size_t g2 = 0, f2 = 0, w2 = 0;
+        const size_t target = (total + nr - 1) / nr;
+        const int n2 = form_rounds(NULL, jobs, order, njobs, budget, sets, target, &g2, &f2, &w2);
+        nr = form_rounds(b, jobs, order, njobs, budget, sets, n2 == nr ? target : (size_t)-1,
+                         &gmax, &fmax, &wmax);
     } else if (!rc) {
-        nr = form_rounds(b, jobs, order, njobs, budget, sets, (size_t)-1, &gmax, &fwmax);
+        nr = form_rounds(b, jobs, order, njobs, budget, sets, (size_t)-1, &gmax, &fmax, &wmax);
     }
     if (!rc && nr < 0) { fgk_set_error("out of host memory"); rc = -1; }
     free(order);
@@ -1214,14 +1232,30 @@ int fftgal_batch_create(fftgal_batch_t **out, int njobs, const fftgal_job_desc *
         fgk_set_error("out of host memory");
         rc = -1;
     }
-    /* the pool: grid sets (zeroed once; every round leaves its set zero)
-     * then the spectra and sort words of one round */
+    /* the pool: grid sets (zeroed once; every round leaves its set zero),
+     * the spectra of one round, the sort words of one round */
     b->gwords = gmax / 8;
-    b->pool_bytes = pool_size(gmax, fwmax, sets);
+    b->pool_bytes = pool_size(gmax, fmax, wmax, sets);
     STEP(fgk_malloc((void **)&b->pool, b->pool_bytes));
     if (!rc) {
         b->gset[0] = (int64_t *)b->pool;
         b->gset[1] = sets == 2 ? (int64_t *)(b->pool + gmax) : NULL;
+        b->ws_region = b->pool + (size_t)sets * gmax + fmax;
+    }
+    /* more than one round: each round's sort runs on s2 beside the previous
+     * round's FFT passes and bin */
+    if (!rc && b->nrounds > 1) {
+        STEP(fgk_stream_create(&b->s2));
+        b->evp = (fgk_event_t *)calloc((size_t)b->nrounds, sizeof(fgk_event_t));
+        b->evs = (fgk_event_t *)calloc((size_t)b->nrounds, sizeof(fgk_event_t));
+        if (!rc && (!b->evp || !b->evs)) {
+            fgk_set_error("out of host memory");
+            rc = -1;
+        }
+        for (int r = 0; r < b->nrounds && !rc; ++r) {
+            STEP(fgk_event_create_dep(&b->evp[r]));
+            STEP(fgk_event_create_dep(&b->evs[r]));
+        }
     }
     STEP(fgk_fill_zero(b->pool, (size_t)sets * gmax, b->s));
     for (int r = 0; r < b->nrounds && !rc; ++r) rc = round_setup(b, r);
@@ -1316,6 +1350,8 @@ static int batch_tables(fftgal_batch_t *b, const float *const *x, const float *c
  * NULL) brackets them.  set0: grid set of the first round. */
 static int batch_work(fftgal_batch_t *b, int set0, fgk_event_t *ev) {
     int st = 0;
+    /* rounds overlapped on two streams (one stream when timing stages) */
+    const int ov = !ev && b->s2 && !getenv("FFTGAL_BATCH_SERIAL");
 #define MARK() do { if (ev) TRY(fgk_event_record(ev[st], b->s)); ++st; } while (0)
     for (int r = 0; r < b->nrounds; ++r) {
         batch_round *R = &b->rnd[r];
@@ -1335,7 +1371,18 @@ static int batch_work(fftgal_batch_t *b, int set0, fgk_event_t *ev) {
         if (r == 0) TRY(fgk_fill_zero(b->d_bins, 2 * b->nbins * sizeof(int64_t), b->s));
         for (int stage = 0; stage < NPAINT; ++stage) {
             MARK();
+            if (ov && r > 0 && stage < NPAINT - 1) continue;   /* sorted on s2 */
+            if (ov && r > 0 && stage == NPAINT - 1) TRY(fgk_stream_wait_event(b->s, b->evs[r]));
             TRY(fgk_paint_batch_stage(R->paint, stage, b->s));
+        }
+        if (ov && r + 1 < b->nrounds) {
+            /* the next round's sort, once this paint has read the sort words,
+             * beside this round's FFT passes and bin */
+            TRY(fgk_event_record(b->evp[r], b->s));
+            TRY(fgk_stream_wait_event(b->s2, b->evp[r]));
+            for (int stage = 0; stage < NPAINT - 1; ++stage)
+                TRY(fgk_paint_batch_stage(b->rnd[r + 1].paint, stage, b->s2));
+            TRY(fgk_event_record(b->evs[r + 1], b->s2));
         }
         MARK();
         if (R->fm) {

@@ -191,14 +191,21 @@ def test_batch_rejects_bad_input(torch_cuda, lib):
     b.close()
 
 
+@pytest.mark.parametrize("serial", [0, 1], ids=["two_streams", "one_stream"])
 @pytest.mark.parametrize("sets", [2, 1])
-def test_batch_rounds_match_one_round(torch_cuda, lib, monkeypatch, sets):
+def test_batch_rounds_match_one_round(torch_cuda, lib, monkeypatch, sets, serial):
     """A device pool bound (FFTGAL_BATCH_BYTES) small enough to cut the
     catalogs into several rounds, which share the pool's grid sets, spectra
     and sort words: the bins equal those of the batch in one round bit for
     bit (the same integer grids and fixed-point sums), over graph replays
     and eager runs, with the grid sets alternating across rounds and
-    enqueues."""
+    enqueues.  Each round's sort runs on a second stream beside the previous
+    round's FFT passes and bin, or (FFTGAL_BATCH_SERIAL=1) on the one
+    stream."""
+    if serial:
+        monkeypatch.setenv("FFTGAL_BATCH_SERIAL", "1")
+    else:
+        monkeypatch.delenv("FFTGAL_BATCH_SERIAL", raising=False)
     import fftgal_amd as fa
     torch = torch_cuda
     cats = _catalogs(torch, [(60000, 500.0, 1), (25000, 300.0, 2), (90000, 800.0, 3)])
