@@ -705,9 +705,11 @@ int fftgal_memcpy_d2h(void *dst, const void *src, unsigned long bytes) {
  * about equal size, in input order, as few as the pool allows; the rounds
  * run one after the other through the same memory:
  *   pool = [grid set 0 | grid set 1 | spectra of a round | sort words of a round]
- * The sort words have a region of their own so that a round's sort (on a
- * second stream) can run while the previous round's FFT passes and bin
- * still read its spectra; the next paint waits for both.
+ * With more than one round, a second stream sorts and paints the rounds
+ * after the first while the batch's stream transforms and bins the one
+ * before (batch_work_overlap); the sort words have a region of their own
+ * so that a round's sort never writes the spectra the previous round's FFT
+ * passes and bin still use.
  * A grid set holds the int64 grids of one round (the largest round's size);
  * the rounds alternate between the two sets, and each round's paint clears
  * the part of the other set that the previous round's row FFT has read.  With one set
@@ -784,7 +786,7 @@ struct fftgal_batch {
     batch_side *side;
     fgk_stream_t s;
     fgk_stream_t s2;             /* a round's sort beside the previous round's FFT and bin */
-    fgk_event_t *evp, *evs;      /* per round: its paint done (s), its sort done (s2) */
+    fgk_event_t *evp, *evr;      /* per round: its paint done, its row pass done */
     char *ws_region;             /* the sort words of every round */
     int64_t *d_bins, *h_bins;    /* fixed-point pairs (fgk.h), device and pinned */
     double *h_sums;              /* the values in FP64 */
@@ -844,10 +846,10 @@ int fftgal_batch_destroy(fftgal_batch_t *b) {
     free(b->h_sums);
     for (int r = 0; r < b->nrounds; ++r) {
         if (b->evp && b->evp[r]) fgk_event_destroy(b->evp[r]);
-        if (b->evs && b->evs[r]) fgk_event_destroy(b->evs[r]);
+        if (b->evr && b->evr[r]) fgk_event_destroy(b->evr[r]);
     }
     free(b->evp);
-    free(b->evs);
+    free(b->evr);
     if (b->s2) fgk_stream_destroy(b->s2);
     if (b->s) fgk_stream_destroy(b->s);
     free(b);
@@ -1247,14 +1249,14 @@ int fftgal_batch_create(fftgal_batch_t **out, int njobs, const fftgal_job_desc *
     if (!rc && b->nrounds > 1) {
         STEP(fgk_stream_create(&b->s2));
         b->evp = (fgk_event_t *)calloc((size_t)b->nrounds, sizeof(fgk_event_t));
-        b->evs = (fgk_event_t *)calloc((size_t)b->nrounds, sizeof(fgk_event_t));
-        if (!rc && (!b->evp || !b->evs)) {
+        b->evr = (fgk_event_t *)calloc((size_t)b->nrounds, sizeof(fgk_event_t));
+        if (!rc && (!b->evp || !b->evr)) {
             fgk_set_error("out of host memory");
             rc = -1;
         }
         for (int r = 0; r < b->nrounds && !rc; ++r) {
             STEP(fgk_event_create_dep(&b->evp[r]));
-            STEP(fgk_event_create_dep(&b->evs[r]));
+            STEP(fgk_event_create_dep(&b->evr[r]));
         }
     }
     STEP(fgk_fill_zero(b->pool, (size_t)sets * gmax, b->s));
@@ -1348,84 +1350,135 @@ static int batch_tables(fftgal_batch_t *b, const float *const *x, const float *c
 /* the batch's rounds, each through the stages in the order and under the
  * names of fftgal_power_stage_name; ev (NSTAGES+1 events per round, or
  * NULL) brackets them.  set0: grid set of the first round. */
-static int batch_work(fftgal_batch_t *b, int set0, fgk_event_t *ev) {
+/* host-side set-up of round r before its paint: spectra order of its FFT
+ * plans, and with two grid sets the set its paint writes and the part of
+ * the other set it clears (as far as the previous round, or for the first
+ * round the last round of the previous enqueue, wrote it; on the first
+ * enqueue that part is zero already) */
+static int round_select(fftgal_batch_t *b, int r, int set) {
+    batch_round *R = &b->rnd[r];
+    for (int q = 0; q < R->ngroups; ++q)
+        R->grp[q].spec.fs_n1 = fgk_fft_fourstep(b->side[R->grp[q].side].plan);
+    if (b->gset[1]) {
+        const batch_round *P = &b->rnd[(r + b->nrounds - 1) % b->nrounds];
+        TRY(fgk_paint_batch_select(R->paint, set, b->gset[set ^ 1], P->gwords * sizeof(int64_t)));
+    }
+    return 0;
+}
+
+/* the row pass, both column passes and the fused bin of round r on s;
+ * mark(i) records stage i's event (profiling) */
+static int round_fft_bin(fftgal_batch_t *b, int r, int set, fgk_stream_t s, fgk_event_t *ev,
+                         int *st, fgk_event_t after_rows) {
+    batch_round *R = &b->rnd[r];
+#define MARK() do { if (ev) TRY(fgk_event_record(ev[*st], s)); ++*st; } while (0)
+    if (R->fm) {
+        TRY(fgk_fft_multi_rows(R->fm, set, b->gset[1] ? 0 : 1, s));
+    } else {
+        for (int q = 0; q < R->ngroups; ++q) {
+            batch_group *G = &R->grp[q];
+            TRY(fgk_fft_rows_f32_from_fixed(b->side[G->side].plan, G->f, set ? G->g2 : G->g, 1.0,
+                                            b->gset[1] ? 0 : 1, G->n * (G->interlace ? 2 : 1),
+                                            s));
+        }
+    }
+    if (after_rows) TRY(fgk_event_record(after_rows, s));
+    for (int axis = 1; axis >= 0; --axis) {
+        MARK();
+        if (R->fm) {
+            TRY(fgk_fft_multi_cols(R->fm, axis, -1, s));
+            continue;
+        }
+        for (int q = 0; q < R->ngroups; ++q) {
+            batch_group *G = &R->grp[q];
+            TRY(fgk_fft_cols_f32(b->side[G->side].plan, G->f, axis, -1,
+                                 G->n * (G->interlace ? 2 : 1), s));
+        }
+    }
+    MARK();
+    if (R->djall) {
+        TRY(fgk_fused_bin_f32_multi(&R->bspec, R->djall, R->njobs, R->NK_max, R->bblocks,
+                                    R->bpblocks, s));
+    } else {
+        for (int q = 0; q < R->ngroups; ++q) {
+            batch_group *G = &R->grp[q];
+            const batch_side *S = &b->side[G->side];
+            TRY(fgk_fused_bin_f32_batch(&G->spec, G->djob, G->n, G->NK_max, S->d_phase,
+                                        S->d_winv, s));
+        }
+    }
+    MARK();
+    if (r == b->nrounds - 1)
+        TRY(fgk_d2h(b->h_bins, b->d_bins, 2 * b->nbins * sizeof(int64_t), s));
+    MARK();
+#undef MARK
+    return 0;
+}
+
+/* every round on the batch's stream, with stage events when ev is given */
+static int batch_work_serial(fftgal_batch_t *b, int set0, fgk_event_t *ev) {
     int st = 0;
-    /* rounds overlapped on two streams (one stream when timing stages) */
-    const int ov = !ev && b->s2 && !getenv("FFTGAL_BATCH_SERIAL");
 #define MARK() do { if (ev) TRY(fgk_event_record(ev[st], b->s)); ++st; } while (0)
     for (int r = 0; r < b->nrounds; ++r) {
-        batch_round *R = &b->rnd[r];
         const int set = b->gset[1] ? (set0 + r) & 1 : 0;
-        for (int q = 0; q < R->ngroups; ++q)
-            R->grp[q].spec.fs_n1 = fgk_fft_fourstep(b->side[R->grp[q].side].plan);
-        /* with two grid sets the paint writes set `set` and clears the other
-         * as far as its last round wrote it: the previous round, or for
-         * the first round the last of the previous enqueue (on the first
-         * enqueue that part is zero already) */
-        if (b->gset[1]) {
-            const batch_round *P = &b->rnd[(r + b->nrounds - 1) % b->nrounds];
-            TRY(fgk_paint_batch_select(R->paint, set, b->gset[set ^ 1],
-                                       P->gwords * sizeof(int64_t)));
-        }
+        TRY(round_select(b, r, set));
         MARK();
         if (r == 0) TRY(fgk_fill_zero(b->d_bins, 2 * b->nbins * sizeof(int64_t), b->s));
         for (int stage = 0; stage < NPAINT; ++stage) {
             MARK();
-            if (ov && r > 0 && stage < NPAINT - 1) continue;   /* sorted on s2 */
-            if (ov && r > 0 && stage == NPAINT - 1) TRY(fgk_stream_wait_event(b->s, b->evs[r]));
-            TRY(fgk_paint_batch_stage(R->paint, stage, b->s));
-        }
-        if (ov && r + 1 < b->nrounds) {
-            /* the next round's sort, once this paint has read the sort words,
-             * beside this round's FFT passes and bin */
-            TRY(fgk_event_record(b->evp[r], b->s));
-            TRY(fgk_stream_wait_event(b->s2, b->evp[r]));
-            for (int stage = 0; stage < NPAINT - 1; ++stage)
-                TRY(fgk_paint_batch_stage(b->rnd[r + 1].paint, stage, b->s2));
-            TRY(fgk_event_record(b->evs[r + 1], b->s2));
+            TRY(fgk_paint_batch_stage(b->rnd[r].paint, stage, b->s));
         }
         MARK();
-        if (R->fm) {
-            TRY(fgk_fft_multi_rows(R->fm, set, b->gset[1] ? 0 : 1, b->s));
-        } else {
-            for (int q = 0; q < R->ngroups; ++q) {
-                batch_group *G = &R->grp[q];
-                TRY(fgk_fft_rows_f32_from_fixed(b->side[G->side].plan, G->f, set ? G->g2 : G->g,
-                                                1.0, b->gset[1] ? 0 : 1,
-                                                G->n * (G->interlace ? 2 : 1), b->s));
-            }
-        }
-        for (int axis = 1; axis >= 0; --axis) {
-            MARK();
-            if (R->fm) {
-                TRY(fgk_fft_multi_cols(R->fm, axis, -1, b->s));
-                continue;
-            }
-            for (int q = 0; q < R->ngroups; ++q) {
-                batch_group *G = &R->grp[q];
-                TRY(fgk_fft_cols_f32(b->side[G->side].plan, G->f, axis, -1,
-                                     G->n * (G->interlace ? 2 : 1), b->s));
-            }
-        }
-        MARK();
-        if (R->djall) {
-            TRY(fgk_fused_bin_f32_multi(&R->bspec, R->djall, R->njobs, R->NK_max, R->bblocks,
-                                        R->bpblocks, b->s));
-        } else {
-            for (int q = 0; q < R->ngroups; ++q) {
-                batch_group *G = &R->grp[q];
-                const batch_side *S = &b->side[G->side];
-                TRY(fgk_fused_bin_f32_batch(&G->spec, G->djob, G->n, G->NK_max, S->d_phase,
-                                            S->d_winv, b->s));
-            }
-        }
-        MARK();
-        if (r == b->nrounds - 1)
-            TRY(fgk_d2h(b->h_bins, b->d_bins, 2 * b->nbins * sizeof(int64_t), b->s));
-        MARK();
+        TRY(round_fft_bin(b, r, set, b->s, ev, &st, NULL));
     }
 #undef MARK
     return 0;
+}
+
+/* Rounds on two streams: s2 sorts and paints rounds 1.., s transforms and
+ * bins every round (and sorts and paints round 0).  Round r+1's sort starts
+ * once round r's paint has read the sort words (the same stream), and its
+ * paint once round r's row pass has read (and with one grid set cleared)
+ * the grids that paint clears or writes, so both run beside round r's
+ * column passes and bin; round r+1's row pass waits for its paint, and
+ * after round r's bin has read the spectra (the same stream). */
+static int batch_work_overlap(fftgal_batch_t *b, int set0) {
+    int st = 0;
+    const int n = b->nrounds;
+    for (int r = 0; r < n; ++r) {
+        const int set = b->gset[1] ? (set0 + r) & 1 : 0;
+        if (r == 0) {
+            TRY(round_select(b, 0, set));
+            TRY(fgk_fill_zero(b->d_bins, 2 * b->nbins * sizeof(int64_t), b->s));
+            for (int stage = 0; stage < NPAINT; ++stage)
+                TRY(fgk_paint_batch_stage(b->rnd[0].paint, stage, b->s));
+            TRY(fgk_event_record(b->evp[0], b->s));
+            TRY(fgk_stream_wait_event(b->s2, b->evp[0]));
+            for (int stage = 0; stage < NPAINT - 1; ++stage)
+                TRY(fgk_paint_batch_stage(b->rnd[1].paint, stage, b->s2));
+        } else {
+            TRY(fgk_stream_wait_event(b->s, b->evp[r]));
+        }
+        const int next = r + 1 < n;
+        TRY(round_fft_bin(b, r, set, b->s, NULL, &st, next ? b->evr[r] : NULL));
+        if (!next) break;
+        /* issued after round r's work on s, so that s2's wait finds the
+         * record of round r's row pass */
+        const int set1 = b->gset[1] ? (set0 + r + 1) & 1 : 0;
+        TRY(round_select(b, r + 1, set1));
+        TRY(fgk_stream_wait_event(b->s2, b->evr[r]));
+        TRY(fgk_paint_batch_stage(b->rnd[r + 1].paint, NPAINT - 1, b->s2));
+        TRY(fgk_event_record(b->evp[r + 1], b->s2));
+        if (r + 2 < n)
+            for (int stage = 0; stage < NPAINT - 1; ++stage)
+                TRY(fgk_paint_batch_stage(b->rnd[r + 2].paint, stage, b->s2));
+    }
+    return 0;
+}
+
+static int batch_work(fftgal_batch_t *b, int set0, fgk_event_t *ev) {
+    if (!ev && b->s2 && !getenv("FFTGAL_BATCH_SERIAL")) return batch_work_overlap(b, set0);
+    return batch_work_serial(b, set0, ev);
 }
 
 /* grid set of the round after this enqueue's last */

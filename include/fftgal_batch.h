@@ -144,9 +144,10 @@ int fftgal_memcpy_d2h(void *dst, const void *src, unsigned long bytes);
  * environment at creation, run one launch per tile or grid side instead.
  * The catalogs are cut into rounds through a bounded device pool
  * (FFTGAL_BATCH_BYTES, default 34 GiB; fftgal_batch_pool_bytes); each
- * round's sort runs on a second stream beside the previous round's FFT
- * passes and bin (FFTGAL_BATCH_SERIAL=1 in the environment at enqueue keeps
- * one stream).  Every round, every stage, is captured once in a hipGraph; the catalogs'
+ * round's sort and paint run on a second stream beside the previous
+ * round's FFT passes and bin (FFTGAL_BATCH_SERIAL=1 in the environment at
+ * enqueue keeps one stream).  Every round, every stage, is captured once in
+ * a hipGraph; the catalogs'
  * pointers, Np and V are read from device tables, so later enqueues replay
  * the same graph.  A catalog's grid may need at most 256 x 128 paint tiles
  * (Ng <= 512 with tiles of 16); larger grids take a fftgal_power_t of their
