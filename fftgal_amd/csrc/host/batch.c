@@ -806,6 +806,7 @@ struct fftgal_batch {
 int fftgal_batch_destroy(fftgal_batch_t *b) {
     if (!b) return 0;
     if (b->s) fgk_stream_sync(b->s);
+    if (b->s2) fgk_stream_sync(b->s2);   /* joined into s by every enqueue that completes */
     fgk_graph_destroy(b->graph[0]);
     fgk_graph_destroy(b->graph[1]);
     for (int r = 0; r < b->nrounds && b->rnd; ++r) {
