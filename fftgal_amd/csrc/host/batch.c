@@ -1367,8 +1367,10 @@ static int round_select(fftgal_batch_t *b, int r, int set) {
     return 0;
 }
 
-/* the row pass, both column passes and the fused bin of round r on s;
- * mark(i) records stage i's event (profiling) */
+/* the row pass, both column passes and the fused bin of round r on s (and
+ * after the last round the copy of the bin sums); with ev, each stage
+ * boundary records the next of the profile's events (*st counts them);
+ * after_rows, when given, is recorded once the row pass is issued */
 static int round_fft_bin(fftgal_batch_t *b, int r, int set, fgk_stream_t s, fgk_event_t *ev,
                          int *st, fgk_event_t after_rows) {
     batch_round *R = &b->rnd[r];
