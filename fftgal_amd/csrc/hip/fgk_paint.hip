@@ -610,7 +610,9 @@ __global__ void __launch_bounds__(SCAN_THREADS)
 k_digitscan(const uint32_t *__restrict__ v, long n, int mode, int cap,
             uint32_t *__restrict__ base, uint32_t *__restrict__ aux,
             uint32_t *__restrict__ aux_blk) {
-    __shared__ uint32_t sa[SCAN_THREADS], sb[SCAN_THREADS];
+    constexpr int NW = SCAN_THREADS / 64;
+    __shared__ uint32_t sa[NW], sb[NW];
+    __shared__ uint32_t ta, tb;                  // totals
     const long per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
     const long i0 = threadIdx.x * per;
     const long i1 = i0 + per < n ? i0 + per : n;
@@ -619,14 +621,28 @@ k_digitscan(const uint32_t *__restrict__ v, long n, int mode, int cap,
                          : dense_items(c, cap);
     };
     // up to RP entries per thread: loaded into registers together (one
-    // memory latency rather than one per entry)
+    // memory latency rather than one per entry), 16 bytes at a time when the
+    // thread's run is whole and 16-byte aligned
     constexpr int RP = 32;
     uint32_t vals[RP];
     const bool inreg = per <= RP;
+    const bool vec = inreg && per % 4 == 0 && i1 - i0 == per && ((uintptr_t)(v + i0) & 15) == 0;
     uint32_t ca = 0, cb = 0;
-    if (inreg) {
+    if (vec) {
+#pragma unroll
+        for (int q = 0; q < RP; q += 4) {
+            uint4 w = make_uint4(0u, 0u, 0u, 0u);
+            if (q < per) w = *(const uint4 *)(v + i0 + q);
+            vals[q] = w.x;
+            vals[q + 1] = w.y;
+            vals[q + 2] = w.z;
+            vals[q + 3] = w.w;
+        }
+    } else if (inreg) {
 #pragma unroll
         for (int q = 0; q < RP; ++q) vals[q] = i0 + q < i1 ? v[i0 + q] : 0u;
+    }
+    if (inreg) {
 #pragma unroll
         for (int q = 0; q < RP; ++q) {
             ca += vals[q];
@@ -639,19 +655,60 @@ k_digitscan(const uint32_t *__restrict__ v, long n, int mode, int cap,
             cb += aux_of(c);
         }
     }
-    sa[threadIdx.x] = ca;
-    sb[threadIdx.x] = cb;
-    __syncthreads();
-    for (int o = 1; o < SCAN_THREADS; o <<= 1) {
-        const uint32_t va = threadIdx.x >= o ? sa[threadIdx.x - o] : 0;
-        const uint32_t vb = threadIdx.x >= o ? sb[threadIdx.x - o] : 0;
-        __syncthreads();
-        sa[threadIdx.x] += va;
-        sb[threadIdx.x] += vb;
-        __syncthreads();
+    // block-wide exclusive scan of the threads' (ca, cb): within each wave by
+    // shuffles, then across the waves' totals
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t ia = ca, ib = cb;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t xa = __shfl_up(ia, o), xb = __shfl_up(ib, o);
+        if (lane >= o) {
+            ia += xa;
+            ib += xb;
+        }
     }
-    uint32_t ra = sa[threadIdx.x] - ca, rb = sb[threadIdx.x] - cb;
-    if (inreg) {
+    if (lane == 63) {
+        sa[wv] = ia;
+        sb[wv] = ib;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        uint32_t wa = lane < NW ? sa[lane] : 0u, wb = lane < NW ? sb[lane] : 0u;
+        const uint32_t ea = wa, eb = wb;
+#pragma unroll
+        for (int o = 1; o < NW; o <<= 1) {
+            const uint32_t xa = __shfl_up(wa, o), xb = __shfl_up(wb, o);
+            if (lane >= o) {
+                wa += xa;
+                wb += xb;
+            }
+        }
+        if (lane < NW) {
+            sa[lane] = wa - ea;                  // exclusive prefix of the waves
+            sb[lane] = wb - eb;
+        }
+        if (lane == NW - 1) {
+            ta = wa;
+            tb = wb;
+        }
+    }
+    __syncthreads();
+    uint32_t ra = sa[wv] + ia - ca, rb = sb[wv] + ib - cb;
+    const bool vst = vec && aux_blk == nullptr &&
+                     (((uintptr_t)(base + i0) | (uintptr_t)(aux + i0)) & 15) == 0;
+    if (vst) {
+#pragma unroll
+        for (int q = 0; q < RP; q += 4) {
+            if (q >= per) break;
+            uint4 bo, ao;
+            bo.x = ra; ao.x = rb; ra += vals[q]; rb += aux_of(vals[q]);
+            bo.y = ra; ao.y = rb; ra += vals[q + 1]; rb += aux_of(vals[q + 1]);
+            bo.z = ra; ao.z = rb; ra += vals[q + 2]; rb += aux_of(vals[q + 2]);
+            bo.w = ra; ao.w = rb; ra += vals[q + 3]; rb += aux_of(vals[q + 3]);
+            *(uint4 *)(base + i0 + q) = bo;
+            *(uint4 *)(aux + i0 + q) = ao;
+        }
+    } else if (inreg) {
 #pragma unroll
         for (int q = 0; q < RP; ++q) {
             if (i0 + q >= i1) break;
@@ -672,9 +729,9 @@ k_digitscan(const uint32_t *__restrict__ v, long n, int mode, int cap,
         }
     }
     if (threadIdx.x == SCAN_THREADS - 1) {
-        base[n] = sa[SCAN_THREADS - 1];
-        aux[n] = sb[SCAN_THREADS - 1];
-        if (aux_blk) aux_blk[n] = sb[SCAN_THREADS - 1] / RB;
+        base[n] = ta;
+        aux[n] = tb;
+        if (aux_blk) aux_blk[n] = tb / RB;
     }
 }
 
