@@ -13,7 +13,9 @@ extern "C" {
  * x y z vx vy vz, then halo mass, satellite flag and halo id, which are
  * skipped (src/io/io.h:4-6, src/io/qpm.c:9-34).  V = L^3; velocities are
  * allocated and filled.  Aborts with a message on an unreadable file or a
- * line that does not start with six numbers, as the reference asserts. */
+ * line that does not start with six numbers, as the reference asserts.
+ * Parses with one thread per online core, or FFTGAL_IO_THREADS (else
+ * OMP_NUM_THREADS) threads when that is fewer. */
 gal_t *gal_loadqpm_cubic(char *file, double L);
 
 #ifdef __cplusplus
